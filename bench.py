@@ -1,0 +1,236 @@
+"""Throughput of the HIP lattice Boltzmann step on one MI355X, against the HBM roofline.
+
+Workload (the headline configuration): D3Q19, float32, in-place AA pattern, 256^3
+fully periodic box, TRT (Lambda = 3/16) with tau = 0.6, uniform Guo body force
+(1e-6, 0, 0), initialised with the decaying ABC flow A = B = C = 0.02,
+k = 2 pi / N. A "step" is one time step: every node streamed, bounced and
+collided once (an AA pair of half-steps is two steps).
+
+Protocol (as the reference's benchmarks/bench_mlups.py:104-154): warm-up
+steps, re-measure the device-copy peak on 2 GiB buffers (above the 256 MB
+Infinity Cache), then synchronize -> run K steps -> synchronize.
+MLUPS = N * K / seconds / 1e6. Algorithmic bytes per node update are
+2 * Q * sizeof(float) = 152 B (19 loads + 19 stores), so achieved GB/s =
+MLUPS * 152 / 1e3. Each AA kernel launch updates all N nodes once.
+
+The JSON line carries ``roofline`` (achieved GB/s of the dominant kernel from
+HIP events around its launches on the launch stream, against the 8 TB/s HBM
+spec, with ``traffic`` = measured HBM bytes per launch from rocprofv3 counters
+when a summary is present under profiles/) and ``cpu_baseline`` (the NumPy
+oracle timed on this host on a bounded sample of the same workload, 1 core).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "high-performance-lbm-on-gpu-_amd"))
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MLUPS & %HBM peak, D3Q19 fp32 AA 256³; fwd+adjoint/fwd MLUPS ratio"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_d3q19_aa_256.json"
+
+
+def abc_params(n: int, dtype: str = "float32", tau: float = 0.6):
+    from lbm import SimulationParams
+
+    return SimulationParams(nx=n, ny=n, nz=n, tau=tau, collision="trt", force=(1e-6, 0.0, 0.0), dtype=dtype)
+
+
+def abc_fields(n: int, amplitude: float = 0.02):
+    k = 2.0 * np.pi / n
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64),) * 3, indexing="ij")
+    ux = amplitude * np.sin(k * z) + amplitude * np.cos(k * y)
+    uy = amplitude * np.sin(k * x) + amplitude * np.cos(k * z)
+    uz = amplitude * np.sin(k * y) + amplitude * np.cos(k * x)
+    rho = 1.0 + (-0.5 * (ux**2 + uy**2 + uz**2)) * 3.0
+    return rho, ux, uy, uz
+
+
+def copy_peak_gbs(torch, nbytes: int = 2 * 2**30, repeats: int = 10) -> float:
+    """Read + write GB/s of the library's grid-stride copy kernel on 2 GiB buffers."""
+    from lbm.backends._native import library
+
+    lib = library()
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    lib.call("lbm_device_copy", dst.data_ptr(), src.data_ptr(), nbytes, stream.cuda_stream)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(repeats):
+        lib.call("lbm_device_copy", dst.data_ptr(), src.data_ptr(), nbytes, stream.cuda_stream)
+    stop.record(stream)
+    stop.synchronize()
+    seconds = start.elapsed_time(stop) / 1e3
+    del src, dst
+    return 2 * nbytes * repeats / seconds / 1e9
+
+
+def kernel_times_ms(torch, solver, pairs: int = 20) -> dict[str, float]:
+    """Average duration of each AA half-step kernel, from events around each launch."""
+    stream = torch.cuda.current_stream()
+    events = [torch.cuda.Event(enable_timing=True) for _ in range(2 * pairs + 1)]
+    events[0].record(stream)
+    for p in range(pairs):
+        solver._pre_pass("aa_odd")
+        solver._call("lbm_aa_odd", solver._lattice_code, solver._dtype_code, solver._f.data_ptr(),
+                     solver._kind.data_ptr(), solver._wall.data_ptr(), None, None, *solver._step_args())
+        events[2 * p + 1].record(stream)
+        solver._pre_pass("aa_even")
+        solver._call("lbm_aa_even", solver._lattice_code, solver._dtype_code, solver._f.data_ptr(),
+                     solver._kind.data_ptr(), None, None, *solver._step_args())
+        events[2 * p + 2].record(stream)
+    events[-1].synchronize()
+    odd = [events[2 * p].elapsed_time(events[2 * p + 1]) for p in range(pairs)]
+    even = [events[2 * p + 1].elapsed_time(events[2 * p + 2]) for p in range(pairs)]
+    return {"aa_odd": float(np.mean(odd)), "aa_even": float(np.mean(even))}
+
+
+def cpu_baseline(n: int = 128, steps: int = 8) -> dict:
+    """The NumPy oracle on a bounded sample: D3Q19 fp32 TRT + Guo ABC box, n^3, ``steps`` steps."""
+    from oracle import ReferenceSolver
+
+    params = abc_params(n)
+    solver = ReferenceSolver(params)
+    solver.initialize(*abc_fields(n))
+    solver.step(1)
+    start = time.perf_counter()
+    solver.step(steps)
+    seconds = time.perf_counter() - start
+    cores = len(os.sched_getaffinity(0))
+    return {
+        "value": n**3 * steps / seconds / 1e6,
+        "unit": "MLUPS",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"NumPy oracle, D3Q19 fp32 TRT+Guo ABC periodic {n}^3, {steps} steps after 1 warm-up "
+        f"({seconds:.1f} s; NumPy elementwise is single-threaded; {cores} cores visible)",
+    }
+
+
+def traffic_from_profile():
+    if TRAFFIC_FILE.exists():
+        data = json.loads(TRAFFIC_FILE.read_text())
+        return data.get("hbm_bytes_per_launch")
+    return None
+
+
+def main(argv=None) -> int:
+    parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=1000)
+    parser.add_argument("--warmup", type=int, default=20)
+    parser.add_argument("--size", type=int, default=256)
+    parser.add_argument("--variant", default="aa", choices=("aa", "pull"))
+    parser.add_argument("--block", type=int, nargs=2, default=None)
+    parser.add_argument("--no-cpu-baseline", action="store_true")
+    args = parser.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (got --gpus {args.gpus})", file=sys.stderr)
+        return 2
+    if args.steps % 2 or args.warmup % 2:
+        print("steps and warmup must be even (the AA pattern advances in pairs)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from lbm import make_solver
+
+    n = args.size
+    params = abc_params(n)
+    options = {"block": tuple(args.block)} if args.block else {}
+    solver = make_solver(params, backend=f"hip-{args.variant}", **options)
+    solver.initialize(*abc_fields(n))
+    print(f"# {torch.cuda.get_device_name(0)}: D3Q19 fp32 {args.variant} {n}^3, block {solver.block}", flush=True)
+
+    if args.warmup:
+        solver.run(args.warmup, check_every=args.warmup)
+    torch.cuda.synchronize()
+    peak = copy_peak_gbs(torch)
+    print(f"# device-copy peak {peak:.0f} GB/s (2 GiB buffers)", flush=True)
+
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    start.record(stream)
+    solver._advance(args.steps)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    gpu_seconds = start.elapsed_time(stop) / 1e3
+    seconds = wall
+    solver.time_step += args.steps
+    solver._check_health()
+
+    nodes = n**3
+    mlups = nodes * args.steps / seconds / 1e6
+    bytes_per_update = 2 * params.lattice.q * 4
+    gbs = mlups * bytes_per_update / 1e3
+    print(f"# timed {args.steps} steps: host wall {wall * 1e3:.2f} ms (events {gpu_seconds * 1e3:.2f} ms), "
+          f"{mlups:.0f} MLUPS, {gbs:.0f} GB/s = {100 * gbs / peak:.1f}% of copy peak", flush=True)
+
+    if args.variant == "aa":
+        times = kernel_times_ms(torch, solver)
+        dominant = max(times, key=times.get)
+        launch_ms = times[dominant]
+    else:
+        dominant, launch_ms = "pull", seconds * 1e3 / args.steps
+    achieved = nodes * bytes_per_update / (launch_ms / 1e3) / 1e9
+    print(f"# per-kernel ms: {json.dumps({k: round(v, 4) for k, v in (times.items() if args.variant == 'aa' else {})})}",
+          flush=True)
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        baseline = cpu_baseline()
+        print(f"# cpu baseline: {baseline['value']:.2f} MLUPS ({baseline['sample']})", flush=True)
+
+    record = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(seconds * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (ABC flow initial field)",
+        "config": {
+            "workload": f"D3Q19 fp32 TRT+Guo periodic ABC box {n}^3, {args.variant.upper()} pattern",
+            "nodes": nodes,
+            "bytes_per_update": bytes_per_update,
+            "block": list(solver.block),
+            "copy_peak_gbs": round(peak, 1),
+            "fraction_of_copy_peak": round(gbs / peak, 4),
+            "fits_in_infinity_cache": params.lattice.q * nodes * 4 <= 256 * 2**20,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"lbm_{dominant}",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profile(),
+        },
+        "cpu_baseline": baseline,
+    }
+    print(json.dumps(record), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,108 @@
+// Shared host-side plumbing of the C ABI: error reporting, type dispatch, grid description.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+#include "lbm_lattice.h"
+#include "lbm_physics.h"
+#include "../../include/lbm_hip.h"
+
+namespace lbm {
+
+// Last error message of the calling host thread (lbm_last_error()).
+void set_error(const std::string& message);
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void require(bool ok, const std::string& message, int code = LBM_ERR_INVALID)
+{
+    if (!ok) throw Error(code, message);
+}
+
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw Error(LBM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Run an entry point body, turning exceptions into codes + lbm_last_error().
+template <class F>
+int guarded(const char* name, F&& body)
+{
+    try {
+        body();
+        return LBM_OK;
+    } catch (const Error& e) {
+        set_error(std::string(name) + ": " + e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_error(std::string(name) + ": " + e.what());
+        return LBM_ERR_INTERNAL;
+    }
+}
+
+inline void check_launch(const char* what)
+{
+    hip_check(hipGetLastError(), what);
+}
+
+// Call f(L{}, T{}) for the lattice and precision codes of the C ABI.
+template <class F>
+void dispatch(int lattice, int dtype, F&& f)
+{
+    require(lattice == LBM_D2Q9 || lattice == LBM_D3Q19, "unknown lattice code");
+    require(dtype == LBM_F32 || dtype == LBM_F64, "unknown dtype code");
+    if (lattice == LBM_D2Q9) {
+        if (dtype == LBM_F32) f(D2Q9{}, float{});
+        else f(D2Q9{}, double{});
+    } else {
+        if (dtype == LBM_F32) f(D3Q19{}, float{});
+        else f(D3Q19{}, double{});
+    }
+}
+
+template <class F>
+void dispatch_real(int dtype, F&& f)
+{
+    require(dtype == LBM_F32 || dtype == LBM_F64, "unknown dtype code");
+    if (dtype == LBM_F32) f(float{});
+    else f(double{});
+}
+
+// Grid of nx * ny * nz nodes, x fastest; n = nx ny nz < 2^31.
+struct Grid {
+    int nx, ny, nz;
+    uint32_t n;
+};
+
+inline Grid make_grid(int lattice, int nx, int ny, int nz)
+{
+    require(nx > 0 && ny > 0 && nz > 0, "grid sizes must be positive");
+    require(lattice == LBM_D3Q19 || nz == 1, "a D2Q9 grid has nz = 1");
+    const uint64_t n = (uint64_t)nx * ny * nz;
+    require(n < (1ull << 31), "grids are limited to 2^31 nodes");
+    return Grid{nx, ny, nz, (uint32_t)n};
+}
+
+inline dim3 block_of(int bx, int by)
+{
+    require(bx > 0 && by > 0 && bx * by <= 1024, "block must be positive with at most 1024 threads");
+    return dim3(bx, by, 1);
+}
+
+inline dim3 grid_of(const Grid& g, dim3 block)
+{
+    return dim3((g.nx + block.x - 1) / block.x, (g.ny + block.y - 1) / block.y, g.nz);
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace lbm

@@ -1,0 +1,198 @@
+// Sparse affine work on the population buffer: the link-rule pre-pass, affine
+// readouts, and the momentum-exchange force reduction.
+//
+// A compiled affine set stores, term-major, slot[t][r] (int64 flat index into
+// the (q, N) buffer), coeff[t][r] and constant[r], each row padded to m terms
+// with zero coefficients (src/lbm/link_rules.py:54-103, 203-240). A row's
+// value is constant + (coefficient * population) summed over the m terms in
+// NumPy's pairwise order (8 interleaved partial sums for m >= 8), which is how
+// the reference evaluates it (link_rules.py:269-272), so a rule writes the
+// same bits the NumPy oracle computes.
+//
+// Rules are evaluated before any is written: rule values read pre-rule
+// populations only, whatever the geometry (src/lbm/backends/kernels/link_rules.cu:8-11).
+// Up to 4096 rules this is one workgroup with a barrier between the phases;
+// above that, an evaluate launch into scratch followed by a write launch.
+
+#include "lbm_common.h"
+
+namespace lbm {
+
+// NumPy's pairwise summation of term(0..m-1) for m <= 128
+// (numpy/_core/src/umath/loops_utils.h.src, pairwise_sum for n <= PW_BLOCKSIZE).
+template <class T, class Term>
+__device__ __forceinline__ T pairwise_sum(int m, Term term)
+{
+    if (m < 8) {
+        T s = T(0);
+        for (int p = 0; p < m; ++p) s = s + term(p);
+        return s;
+    }
+    T r0 = term(0), r1 = term(1), r2 = term(2), r3 = term(3);
+    T r4 = term(4), r5 = term(5), r6 = term(6), r7 = term(7);
+    const int lim = m - (m % 8);
+    int p = 8;
+    for (; p < lim; p += 8) {
+        r0 = r0 + term(p + 0);
+        r1 = r1 + term(p + 1);
+        r2 = r2 + term(p + 2);
+        r3 = r3 + term(p + 3);
+        r4 = r4 + term(p + 4);
+        r5 = r5 + term(p + 5);
+        r6 = r6 + term(p + 6);
+        r7 = r7 + term(p + 7);
+    }
+    T s = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+    for (; p < m; ++p) s = s + term(p);
+    return s;
+}
+
+template <class T>
+__device__ __forceinline__ T affine_row(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                        const T* __restrict__ coeff, const T* __restrict__ constant, int64_t n,
+                                        int m, int64_t r)
+{
+    const T sum = pairwise_sum<T>(m, [&](int p) {
+        const int64_t e = (int64_t)p * n + r;
+        const T a = coeff[e];
+        return a == T(0) ? T(0) : a * F[slot[e]];
+    });
+    return constant[r] + sum;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_affine_values(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                                       const T* __restrict__ coeff, const T* __restrict__ constant,
+                                                       int64_t n, int m, T* __restrict__ out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = affine_row(F, slot, coeff, constant, n, m, r);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_rules_write(T* __restrict__ F, const int64_t* __restrict__ target,
+                                                     const T* __restrict__ value, int64_t n)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) F[target[r]] = value[r];
+}
+
+constexpr int SMALL_THREADS = 1024;
+constexpr int SMALL_PER_THREAD = 4;
+
+// One workgroup: evaluate up to 4096 rules into registers, barrier, write.
+template <class T>
+__global__ __launch_bounds__(SMALL_THREADS) void k_rules_small(T* __restrict__ F, const int64_t* __restrict__ slot,
+                                                               const T* __restrict__ coeff,
+                                                               const T* __restrict__ constant,
+                                                               const int64_t* __restrict__ target, int64_t n, int m)
+{
+    T v[SMALL_PER_THREAD];
+    static_for<0, SMALL_PER_THREAD>([&](auto K) {
+        const int64_t r = (int64_t)K * SMALL_THREADS + threadIdx.x;
+        v[K] = r < n ? affine_row(F, slot, coeff, constant, n, m, r) : T(0);
+    });
+    __syncthreads();
+    static_for<0, SMALL_PER_THREAD>([&](auto K) {
+        const int64_t r = (int64_t)K * SMALL_THREADS + threadIdx.x;
+        if (r < n) F[target[r]] = v[K];
+    });
+}
+
+constexpr int FORCE_THREADS = 256;
+
+// Force = sum_r weight[r] * value[r] (d components), one workgroup, fixed order.
+template <class T>
+__global__ __launch_bounds__(FORCE_THREADS) void k_affine_force(const T* __restrict__ F,
+                                                                const int64_t* __restrict__ slot,
+                                                                const T* __restrict__ coeff,
+                                                                const T* __restrict__ constant,
+                                                                const double* __restrict__ weight, int64_t n, int m,
+                                                                int d, double* __restrict__ out)
+{
+    __shared__ double part[3][FORCE_THREADS];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t r = threadIdx.x; r < n; r += FORCE_THREADS) {
+        const double v = (double)affine_row(F, slot, coeff, constant, n, m, r);
+        for (int a = 0; a < d; ++a) acc[a] += weight[r * d + a] * v;
+    }
+    for (int a = 0; a < 3; ++a) part[a][threadIdx.x] = acc[a];
+    __syncthreads();
+    for (int width = FORCE_THREADS / 2; width > 0; width /= 2) {
+        if ((int)threadIdx.x < width)
+            for (int a = 0; a < 3; ++a) part[a][threadIdx.x] += part[a][threadIdx.x + width];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < d; ++a) out[a] = part[a][0];
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+LBM_API int lbm_rules_apply(int dtype, void* f, const int64_t* slot, const void* coeff, const void* constant,
+                            const int64_t* target, int64_t n_rules, int m, void* scratch, void* stream)
+{
+    return guarded("lbm_rules_apply", [&] {
+        require(n_rules >= 0 && m >= 1 && m <= 128, "rules need 1 <= m <= 128 terms");
+        if (n_rules == 0) return;
+        require(f && slot && coeff && constant && target, "null pointer");
+        dispatch_real(dtype, [&](auto Tt) {
+            using T = decltype(Tt);
+            const hipStream_t s = as_stream(stream);
+            if (n_rules <= (int64_t)SMALL_THREADS * SMALL_PER_THREAD) {
+                k_rules_small<T><<<1, SMALL_THREADS, 0, s>>>(static_cast<T*>(f), slot, static_cast<const T*>(coeff),
+                                                             static_cast<const T*>(constant), target, n_rules, m);
+            } else {
+                require(scratch != nullptr, "more than 4096 rules need a scratch buffer of n_rules values");
+                const unsigned blocks = (unsigned)((n_rules + 255) / 256);
+                k_affine_values<T><<<blocks, 256, 0, s>>>(static_cast<const T*>(f), slot, static_cast<const T*>(coeff),
+                                                          static_cast<const T*>(constant), n_rules, m,
+                                                          static_cast<T*>(scratch));
+                k_rules_write<T><<<blocks, 256, 0, s>>>(static_cast<T*>(f), target, static_cast<const T*>(scratch),
+                                                        n_rules);
+            }
+            check_launch("rule pre-pass");
+        });
+    });
+}
+
+LBM_API int lbm_affine_values(int dtype, const void* f, const int64_t* slot, const void* coeff, const void* constant,
+                              int64_t n_rows, int m, void* out, void* stream)
+{
+    return guarded("lbm_affine_values", [&] {
+        require(n_rows >= 0 && m >= 1 && m <= 128, "affine rows need 1 <= m <= 128 terms");
+        if (n_rows == 0) return;
+        require(f && slot && coeff && constant && out, "null pointer");
+        dispatch_real(dtype, [&](auto Tt) {
+            using T = decltype(Tt);
+            k_affine_values<T><<<(unsigned)((n_rows + 255) / 256), 256, 0, as_stream(stream)>>>(
+                static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), n_rows,
+                m, static_cast<T*>(out));
+            check_launch("affine values");
+        });
+    });
+}
+
+LBM_API int lbm_affine_force(int dtype, const void* f, const int64_t* slot, const void* coeff, const void* constant,
+                             const double* weight, int64_t n_rows, int m, int d, double* out, void* stream)
+{
+    return guarded("lbm_affine_force", [&] {
+        require(n_rows >= 0 && m >= 1 && m <= 128, "affine rows need 1 <= m <= 128 terms");
+        require(d >= 1 && d <= 3, "d must be 1, 2 or 3");
+        require(out != nullptr, "null output");
+        require(n_rows == 0 || (f && slot && coeff && constant && weight), "null pointer");
+        dispatch_real(dtype, [&](auto Tt) {
+            using T = decltype(Tt);
+            k_affine_force<T><<<1, FORCE_THREADS, 0, as_stream(stream)>>>(
+                static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), weight,
+                n_rows, m, d, out);
+            check_launch("affine force");
+        });
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,589 @@
+// Stream-collide kernels (two-lattice pull, AA even/odd, LDS-tiled), initialisation,
+// macroscopic readout and the divergence watchdog, with their C-ABI entry points.
+//
+// Thread mapping: one thread per node, blockDim (bx, by), grid (ceil(nx/bx),
+// ceil(ny/by), nz). x is the fastest axis of every population plane, so the 64
+// lanes of a wavefront touch 64 consecutive addresses of each plane; with
+// bx = nx (whole rows per workgroup) every cache line of a plane is read or
+// written by exactly one workgroup per step, shifted accesses included.
+// Plane base pointers (f + i * n) are wave-uniform, per-lane offsets 32-bit.
+//
+// Memory traffic per fluid node update is Q loads + Q stores of real_t
+// (152 B for D3Q19 float32) in every variant; the kind byte adds 1 B.
+
+#include "lbm_common.h"
+
+#include <cstring>
+#include <mutex>
+#include <string>
+
+namespace lbm {
+
+// ------------------------------------------------------------------ indexing
+
+__device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+template <class L, int I, int S>
+__device__ __forceinline__ uint32_t neighbour(int x, int y, int z, const Grid& g)
+{
+    using Di = Dir<L, I>;
+    const int xx = Di::x == 0 ? x : wrap(x + S * Di::x, g.nx);
+    const int yy = Di::y == 0 ? y : wrap(y + S * Di::y, g.ny);
+    const int zz = (L::D == 2 || Di::z == 0) ? z : wrap(z + S * Di::z, g.nz);
+    return ((uint32_t)zz * (uint32_t)g.ny + (uint32_t)yy) * (uint32_t)g.nx + (uint32_t)xx;
+}
+
+// Bit i set when the upstream node x - c_i is solid.
+template <class L>
+__device__ __forceinline__ uint32_t blocked_upstream(const uint8_t* __restrict__ kind, int x, int y, int z,
+                                                     const Grid& g)
+{
+    uint32_t m = 0;
+    static_for<1, L::Q>([&](auto I) {
+        if (kind[neighbour<L, I, -1>(x, y, z, g)] & KIND_SOLID) m |= 1u << I;
+    });
+    return m;
+}
+
+// Force of one node for the non-uniform modes.
+template <class L, ForceMode FM, class T>
+__device__ __forceinline__ void node_force(const Relaxation<T>& r, uint8_t k, const T* __restrict__ field,
+                                           const T* __restrict__ body, uint32_t node, uint32_t n, T& Fx,
+                                           T& Fy, T& Fz)
+{
+    Fx = r.force[0];
+    Fy = r.force[1];
+    Fz = r.force[2];
+    if constexpr (FM == ForceMode::DENSE) {
+        Fx = Fx + body[node];
+        Fy = Fy + body[(size_t)n + node];
+        if constexpr (L::D == 3) Fz = Fz + body[2 * (size_t)n + node];
+    }
+    if constexpr (FM != ForceMode::UNIFORM) {
+        if (k & KIND_FORCED) {
+            Fx = Fx + field[node];
+            Fy = Fy + field[(size_t)n + node];
+            if constexpr (L::D == 3) Fz = Fz + field[2 * (size_t)n + node];
+        }
+    }
+}
+
+struct Coord {
+    int x, y, z;
+};
+
+__device__ __forceinline__ bool locate(Coord& c, const Grid& g)
+{
+    c.x = blockIdx.x * blockDim.x + threadIdx.x;
+    c.y = blockIdx.y * blockDim.y + threadIdx.y;
+    c.z = blockIdx.z;
+    return c.x < g.nx && c.y < g.ny;
+}
+
+__device__ __forceinline__ uint32_t flat(const Coord& c, const Grid& g)
+{
+    return ((uint32_t)c.z * (uint32_t)g.ny + (uint32_t)c.y) * (uint32_t)g.nx + (uint32_t)c.x;
+}
+
+// ------------------------------------------------------------------ kernels
+
+// Two-lattice fused pull step.
+template <class L, class T, ForceMode FM, bool MASK>
+__global__ __launch_bounds__(1024) void k_pull(const T* __restrict__ src, T* __restrict__ dst,
+                                               const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
+                                               const T* __restrict__ field, const T* __restrict__ body, Grid g,
+                                               Relaxation<T> r)
+{
+    Coord c;
+    if (!locate(c, g)) return;
+    const uint32_t node = flat(c, g);
+    const uint8_t k = MASK ? kind[node] : 0;
+    if (MASK && (k & KIND_SOLID)) return;
+    const uint32_t blocked = (MASK && (k & KIND_NEAR_SOLID)) ? blocked_upstream<L>(kind, c.x, c.y, c.z, g) : 0u;
+
+    T f[L::Q];
+    static_for<0, L::Q>([&](auto I) {
+        constexpr int i = I;
+        const uint32_t up = neighbour<L, i, -1>(c.x, c.y, c.z, g);
+        if (MASK && ((blocked >> i) & 1u)) {
+            f[i] = src[(size_t)L::opp[i] * g.n + node] + wall_term<L, i>(wall_u, up, g.n);
+        } else {
+            f[i] = src[(size_t)i * g.n + up];
+        }
+    });
+    T Fx = T(0), Fy = T(0), Fz = T(0);
+    if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
+    collide<L, FM>(f, r, Fx, Fy, Fz);
+    static_for<0, L::Q>([&](auto I) { dst[(size_t)I * g.n + node] = f[I]; });
+}
+
+// AA even half-step: node-local, slot reversal.
+template <class L, class T, ForceMode FM, bool MASK>
+__global__ __launch_bounds__(1024) void k_aa_even(T* __restrict__ F, const uint8_t* __restrict__ kind,
+                                                  const T* __restrict__ field, const T* __restrict__ body,
+                                                  Grid g, Relaxation<T> r)
+{
+    Coord c;
+    if (!locate(c, g)) return;
+    const uint32_t node = flat(c, g);
+    const uint8_t k = MASK ? kind[node] : 0;
+    if (MASK && (k & KIND_SOLID)) return;
+    T f[L::Q];
+    static_for<0, L::Q>([&](auto I) { f[I] = F[(size_t)I * g.n + node]; });
+    T Fx = T(0), Fy = T(0), Fz = T(0);
+    if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
+    collide<L, FM>(f, r, Fx, Fy, Fz);
+    static_for<0, L::Q>([&](auto I) { F[(size_t)L::opp[I] * g.n + node] = f[I]; });
+}
+
+// AA odd half-step: pull from x - c_i, push to x + c_i, in place. Slot (k, y) is
+// read and written by the same thread only (src/lbm/backends/kernels/stream_collide_aa.cu:15-31),
+// and every thread loads all Q values before it stores any.
+template <class L, class T, ForceMode FM, bool MASK>
+__global__ __launch_bounds__(1024) void k_aa_odd(T* __restrict__ F, const uint8_t* __restrict__ kind,
+                                                 const T* __restrict__ wall_u, const T* __restrict__ field,
+                                                 const T* __restrict__ body, Grid g, Relaxation<T> r)
+{
+    Coord c;
+    if (!locate(c, g)) return;
+    const uint32_t node = flat(c, g);
+    const uint8_t k = MASK ? kind[node] : 0;
+    if (MASK && (k & KIND_SOLID)) return;
+    const uint32_t blocked = (MASK && (k & KIND_NEAR_SOLID)) ? blocked_upstream<L>(kind, c.x, c.y, c.z, g) : 0u;
+
+    T f[L::Q];
+    static_for<0, L::Q>([&](auto I) {
+        constexpr int i = I;
+        const uint32_t up = neighbour<L, i, -1>(c.x, c.y, c.z, g);
+        if (MASK && ((blocked >> i) & 1u)) {
+            f[i] = F[(size_t)i * g.n + node] + wall_term<L, i>(wall_u, up, g.n);
+        } else {
+            f[i] = F[(size_t)L::opp[i] * g.n + up];
+        }
+    });
+    T Fx = T(0), Fy = T(0), Fz = T(0);
+    if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
+    collide<L, FM>(f, r, Fx, Fy, Fz);
+    static_for<0, L::Q>([&](auto I) {
+        constexpr int i = I;
+        constexpr int o = L::opp[i];
+        const uint32_t down = neighbour<L, i, +1>(c.x, c.y, c.z, g);
+        // x + c_i is solid exactly when x - c_opp(i) is.
+        if (MASK && ((blocked >> o) & 1u)) {
+            F[(size_t)o * g.n + node] = f[i] + wall_term<L, o>(wall_u, down, g.n);
+        } else {
+            F[(size_t)i * g.n + down] = f[i];
+        }
+    });
+}
+
+// LDS-tiled two-lattice step (2D): the block's tile plus a one-node halo of all
+// Q planes and of the kind bytes is staged in LDS, then each thread pulls its
+// neighbours from LDS. Tile = block; out-of-domain threads help load and reach
+// the barrier (src/lbm/backends/kernels/stream_collide_tiled.cu:24-79).
+template <class L, class T, ForceMode FM, bool MASK>
+__global__ __launch_bounds__(1024) void k_lds(const T* __restrict__ src, T* __restrict__ dst,
+                                              const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
+                                              const T* __restrict__ field, const T* __restrict__ body, Grid g,
+                                              Relaxation<T> r)
+{
+    static_assert(L::D == 2, "the LDS-tiled kernel is 2D only");
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int bx = blockDim.x, by = blockDim.y;
+    const int hx = bx + 2, hy = by + 2;
+    const int cells = hx * hy;
+    T* tile = reinterpret_cast<T*>(lds_raw);
+    uint8_t* kind_tile = lds_raw + sizeof(T) * (size_t)L::Q * cells;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x0 = blockIdx.x * bx, y0 = blockIdx.y * by;
+    for (int e = ty * bx + tx; e < cells; e += bx * by) {
+        const int ey = e / hx, ex = e - ey * hx;
+        int gx = (x0 + ex - 1) % g.nx;
+        gx = gx < 0 ? gx + g.nx : gx;
+        int gy = (y0 + ey - 1) % g.ny;
+        gy = gy < 0 ? gy + g.ny : gy;
+        const uint32_t gnode = (uint32_t)gy * (uint32_t)g.nx + (uint32_t)gx;
+        kind_tile[e] = MASK ? kind[gnode] : 0;
+        static_for<0, L::Q>([&](auto I) { tile[I * cells + e] = src[(size_t)I * g.n + gnode]; });
+    }
+    __syncthreads();
+    const int x = x0 + tx, y = y0 + ty;
+    const int centre = (ty + 1) * hx + (tx + 1);
+    const uint8_t k = kind_tile[centre];
+    if (x >= g.nx || y >= g.ny || (MASK && (k & KIND_SOLID))) return;
+    const uint32_t node = (uint32_t)y * (uint32_t)g.nx + (uint32_t)x;
+    T f[L::Q];
+    static_for<0, L::Q>([&](auto I) {
+        constexpr int i = I;
+        using Di = Dir<L, i>;
+        const int e = (ty + 1 - Di::y) * hx + (tx + 1 - Di::x);
+        if (MASK && (kind_tile[e] & KIND_SOLID)) {
+            const uint32_t up = (uint32_t)wrap(y - Di::y, g.ny) * (uint32_t)g.nx + (uint32_t)wrap(x - Di::x, g.nx);
+            f[i] = tile[L::opp[i] * cells + centre] + wall_term<L, i>(wall_u, up, g.n);
+        } else {
+            f[i] = tile[i * cells + e];
+        }
+    });
+    T Fx = T(0), Fy = T(0), Fz = T(0);
+    if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
+    collide<L, FM>(f, r, Fx, Fy, Fz);
+    static_for<0, L::Q>([&](auto I) { dst[(size_t)I * g.n + node] = f[I]; });
+}
+
+template <class L, class T>
+__global__ __launch_bounds__(256) void k_init(T* __restrict__ F, const T* __restrict__ drho,
+                                              const T* __restrict__ ux, const T* __restrict__ uy,
+                                              const T* __restrict__ uz, uint32_t n, int aa)
+{
+    const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= n) return;
+    const T dr = drho[node];
+    const T vx = ux[node], vy = uy[node];
+    const T vz = L::D == 3 ? uz[node] : T(0);
+    const T rho = T(1) + dr;
+    T usum = vx * vx + vy * vy;
+    if constexpr (L::D == 3) usum = usum + vz * vz;
+    const T usq = T(1.5) * usum;
+    static_for<0, L::Q>([&](auto I) {
+        const T feq = shifted_feq<L, I>(dr, rho, vx, vy, vz, usq);
+        const int slot = aa ? L::opp[I] : int(I);
+        F[(size_t)slot * n + node] = feq;
+    });
+}
+
+template <class L, class T, ForceMode FM>
+__global__ __launch_bounds__(256) void k_macro(const T* __restrict__ F, const uint8_t* __restrict__ kind,
+                                               const T* __restrict__ field, uint32_t n, int aa, Relaxation<T> r,
+                                               T* __restrict__ rho_out, T* __restrict__ u_out)
+{
+    const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= n) return;
+    const uint8_t k = kind ? kind[node] : 0;
+    if (k & KIND_SOLID) {
+        rho_out[node] = T(1);
+        static_for<0, L::D>([&](auto A) { u_out[(size_t)A * n + node] = T(0); });
+        return;
+    }
+    T f[L::Q];
+    static_for<0, L::Q>([&](auto I) { f[I] = F[(size_t)(aa ? L::opp[I] : int(I)) * n + node]; });
+    T drho, jx, jy, jz;
+    moments<L>(f, drho, jx, jy, jz);
+    const T rho = T(1) + drho;
+    T h[3] = {r.half_force[0], r.half_force[1], r.half_force[2]};
+    if constexpr (FM != ForceMode::UNIFORM) {
+        T Fa[3] = {r.force[0], r.force[1], r.force[2]};
+        if (k & KIND_FORCED) {
+            static_for<0, L::D>([&](auto A) { Fa[A] = Fa[A] + field[(size_t)A * n + node]; });
+        }
+        static_for<0, 3>([&](auto A) { h[A] = T(0.5) * Fa[A]; });
+    }
+    rho_out[node] = rho;
+    u_out[node] = (jx - h[0]) / rho;
+    u_out[(size_t)n + node] = (jy - h[1]) / rho;
+    if constexpr (L::D == 3) u_out[2 * (size_t)n + node] = (jz - h[2]) / rho;
+}
+
+template <class L, class T>
+__global__ __launch_bounds__(256) void k_health(const T* __restrict__ F, const uint8_t* __restrict__ kind,
+                                                uint32_t n, int aa, uint32_t* __restrict__ out)
+{
+    const uint32_t node = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t bad = 0xffffffffu;
+    uint32_t nonfinite = 0;
+    if (node < n && !(kind && (kind[node] & KIND_SOLID))) {
+        T drho = F[(size_t)(aa ? L::opp[0] : 0) * n + node];
+        static_for<1, L::Q>([&](auto I) { drho = drho + F[(size_t)(aa ? L::opp[I] : int(I)) * n + node]; });
+        const bool finite = isfinite(drho);
+        if (!finite) nonfinite = 1;
+        if (!finite || drho <= T(-1)) bad = node;
+    }
+    // Wavefront reduction first, one atomic per wave.
+    for (int offset = warpSize / 2; offset > 0; offset /= 2) {
+        bad = min(bad, (uint32_t)__shfl_xor((int)bad, offset));
+        nonfinite |= (uint32_t)__shfl_xor((int)nonfinite, offset);
+    }
+    if ((threadIdx.x % warpSize) == 0) {
+        if (bad != 0xffffffffu) atomicMin(out, bad);
+        if (nonfinite) atomicOr(out + 1, 1u);
+    }
+}
+
+template <class L, class T>
+__global__ void k_tables(int* c, int* opp, T* w)
+{
+    const int i = threadIdx.x;
+    if (i >= L::Q) return;
+    c[i] = L::cx[i];
+    c[L::Q + i] = L::cy[i];
+    c[2 * L::Q + i] = L::cz[i];
+    opp[i] = L::opp[i];
+    w[i] = T(L::w[i]);
+}
+
+// ------------------------------------------------------------------ launch helpers
+
+enum class Variant { PULL, EVEN, ODD, LDS };
+
+template <class L, class T, ForceMode FM, bool MASK>
+void launch_step(Variant v, const void* a, void* b, const uint8_t* kind, const void* wall, const void* field,
+                 const void* body, const Grid& g, const Relaxation<T>& r, dim3 block, hipStream_t s)
+{
+    const dim3 grid = grid_of(g, block);
+    const T* wall_t = static_cast<const T*>(wall);
+    const T* field_t = static_cast<const T*>(field);
+    const T* body_t = static_cast<const T*>(body);
+    switch (v) {
+    case Variant::PULL:
+        k_pull<L, T, FM, MASK><<<grid, block, 0, s>>>(static_cast<const T*>(a), static_cast<T*>(b), kind, wall_t,
+                                                       field_t, body_t, g, r);
+        break;
+    case Variant::EVEN:
+        k_aa_even<L, T, FM, MASK><<<grid, block, 0, s>>>(static_cast<T*>(b), kind, field_t, body_t, g, r);
+        break;
+    case Variant::ODD:
+        k_aa_odd<L, T, FM, MASK><<<grid, block, 0, s>>>(static_cast<T*>(b), kind, wall_t, field_t, body_t, g, r);
+        break;
+    case Variant::LDS:
+        if constexpr (L::D == 2) {
+            const size_t lds = sizeof(T) * (size_t)L::Q * (block.x + 2) * (block.y + 2) + (block.x + 2) * (block.y + 2);
+            require(lds <= 160 * 1024, "LDS tile does not fit in 160 KB; use a smaller block");
+            k_lds<L, T, FM, MASK><<<grid, block, lds, s>>>(static_cast<const T*>(a), static_cast<T*>(b), kind,
+                                                           wall_t, field_t, body_t, g, r);
+        } else {
+            require(false, "the LDS-tiled variant is 2D only");
+        }
+        break;
+    }
+    check_launch("stream-collide launch");
+}
+
+int run_step(const char* name, Variant v, int lattice, int dtype, const void* a, void* b, const uint8_t* kind,
+             const void* wall, const void* field, const void* body, int nx, int ny, int nz, double wp,
+             double wm, double fx, double fy, double fz, int force_mode, int mask_mode, int bx, int by,
+             void* stream)
+{
+    return guarded(name, [&] {
+        const Grid g = make_grid(lattice, nx, ny, nz);
+        const dim3 block = block_of(bx, by);
+        require(b != nullptr && (v == Variant::EVEN || v == Variant::ODD || a != nullptr), "null population buffer");
+        require(mask_mode == 0 || kind != nullptr, "mask_mode 1 needs the kind array");
+        require(force_mode >= 0 && force_mode <= 2, "unknown force mode");
+        require(force_mode == 0 || mask_mode == 1 || force_mode == 2, "the IBM field needs the kind array");
+        require(force_mode != 2 || body != nullptr, "dense force mode needs a body field");
+        require(force_mode != 1 || field != nullptr, "field force mode needs a force field");
+        require(mask_mode == 0 || v == Variant::EVEN || wall != nullptr, "wall velocity array missing");
+        const double force[3] = {fx, fy, fz};
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            const Relaxation<T> r = make_relaxation<L, T>(wp, wm, force);
+            auto go = [&](auto fm, auto mk) {
+                launch_step<L, T, decltype(fm)::value, decltype(mk)::value>(v, a, b, kind, wall, field, body, g, r,
+                                                                             block, as_stream(stream));
+            };
+            using U = std::integral_constant<ForceMode, ForceMode::UNIFORM>;
+            using Fi = std::integral_constant<ForceMode, ForceMode::FIELD>;
+            using De = std::integral_constant<ForceMode, ForceMode::DENSE>;
+            using M0 = std::integral_constant<bool, false>;
+            using M1 = std::integral_constant<bool, true>;
+            if (force_mode == 0) {
+                if (mask_mode) go(U{}, M1{}); else go(U{}, M0{});
+            } else if (force_mode == 1) {
+                go(Fi{}, M1{});
+            } else {
+                if (mask_mode) go(De{}, M1{}); else go(De{}, M0{});
+            }
+        });
+    });
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void lbm::set_error(const std::string& message) { g_last_error = message; }
+
+extern "C" {
+
+LBM_API const char* lbm_last_error(void) { return g_last_error.c_str(); }
+
+LBM_API const char* lbm_version(void) { return "lbm_hip 0.1.0 (gfx950)"; }
+
+LBM_API int lbm_lattice_tables(int lattice, int* q, int* c, int* opp, double* w)
+{
+    return guarded("lbm_lattice_tables", [&] {
+        dispatch(lattice, LBM_F64, [&](auto Lt, auto) {
+            using L = decltype(Lt);
+            *q = L::Q;
+            for (int i = 0; i < L::Q; ++i) {
+                c[i] = L::cx[i];
+                c[L::Q + i] = L::cy[i];
+                c[2 * L::Q + i] = L::cz[i];
+                opp[i] = L::opp[i];
+                w[i] = L::w[i];
+            }
+        });
+    });
+}
+
+LBM_API int lbm_lattice_tables_device(int lattice, int dtype, int* c, int* opp, void* w, void* stream)
+{
+    return guarded("lbm_lattice_tables_device", [&] {
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            k_tables<L, T><<<1, 64, 0, as_stream(stream)>>>(c, opp, static_cast<T*>(w));
+            check_launch("lattice table kernel");
+        });
+    });
+}
+
+LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst, const uint8_t* kind,
+                          const void* wall_u, const void* force_field, const void* body_field, int nx, int ny,
+                          int nz, double omega_plus, double omega_minus, double fx, double fy, double fz,
+                          int force_mode, int mask_mode, int bx, int by, void* stream)
+{
+    return run_step("lbm_pull_step", Variant::PULL, lattice, dtype, f_src, f_dst, kind, wall_u, force_field,
+                    body_field, nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by,
+                    stream);
+}
+
+LBM_API int lbm_aa_even(int lattice, int dtype, void* f, const uint8_t* kind, const void* force_field,
+                        const void* body_field, int nx, int ny, int nz, double omega_plus, double omega_minus,
+                        double fx, double fy, double fz, int force_mode, int mask_mode, int bx, int by,
+                        void* stream)
+{
+    return run_step("lbm_aa_even", Variant::EVEN, lattice, dtype, nullptr, f, kind, nullptr, force_field,
+                    body_field, nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by,
+                    stream);
+}
+
+LBM_API int lbm_aa_odd(int lattice, int dtype, void* f, const uint8_t* kind, const void* wall_u,
+                       const void* force_field, const void* body_field, int nx, int ny, int nz, double omega_plus,
+                       double omega_minus, double fx, double fy, double fz, int force_mode, int mask_mode, int bx,
+                       int by, void* stream)
+{
+    return run_step("lbm_aa_odd", Variant::ODD, lattice, dtype, nullptr, f, kind, wall_u, force_field, body_field,
+                    nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by, stream);
+}
+
+LBM_API int lbm_lds_step(int lattice, int dtype, const void* f_src, void* f_dst, const uint8_t* kind,
+                         const void* wall_u, const void* force_field, const void* body_field, int nx, int ny,
+                         int nz, double omega_plus, double omega_minus, double fx, double fy, double fz,
+                         int force_mode, int mask_mode, int bx, int by, void* stream)
+{
+    if (lattice != LBM_D2Q9) {
+        set_error("lbm_lds_step: the LDS-tiled variant is 2D only");
+        return LBM_ERR_INVALID;
+    }
+    return run_step("lbm_lds_step", Variant::LDS, lattice, dtype, f_src, f_dst, kind, wall_u, force_field,
+                    body_field, nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by,
+                    stream);
+}
+
+LBM_API int lbm_init_equilibrium(int lattice, int dtype, void* f, const void* drho, const void* ux, const void* uy,
+                                 const void* uz, int64_t n, int aa, void* stream)
+{
+    return guarded("lbm_init_equilibrium", [&] {
+        require(n > 0 && n < (1ll << 31), "node count out of range");
+        require(f && drho && ux && uy, "null pointer");
+        require(lattice != LBM_D3Q19 || uz != nullptr, "3D initialisation needs uz");
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            const uint32_t nn = (uint32_t)n;
+            k_init<L, T><<<(nn + 255) / 256, 256, 0, as_stream(stream)>>>(
+                static_cast<T*>(f), static_cast<const T*>(drho), static_cast<const T*>(ux),
+                static_cast<const T*>(uy), static_cast<const T*>(uz), nn, aa);
+            check_launch("init kernel");
+        });
+    });
+}
+
+LBM_API int lbm_macroscopic(int lattice, int dtype, const void* f, const uint8_t* kind, const void* force_field,
+                            int64_t n, int aa, double fx, double fy, double fz, int force_mode, void* rho, void* u,
+                            void* stream)
+{
+    return guarded("lbm_macroscopic", [&] {
+        require(n > 0 && n < (1ll << 31), "node count out of range");
+        require(force_mode == 0 || (force_field && kind), "the IBM field needs the kind array");
+        const double force[3] = {fx, fy, fz};
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            const Relaxation<T> r = make_relaxation<L, T>(1.0, 1.0, force);
+            const uint32_t nn = (uint32_t)n;
+            const dim3 grid((nn + 255) / 256);
+            if (force_mode == 0)
+                k_macro<L, T, ForceMode::UNIFORM><<<grid, 256, 0, as_stream(stream)>>>(
+                    static_cast<const T*>(f), kind, nullptr, nn, aa, r, static_cast<T*>(rho), static_cast<T*>(u));
+            else
+                k_macro<L, T, ForceMode::FIELD><<<grid, 256, 0, as_stream(stream)>>>(
+                    static_cast<const T*>(f), kind, static_cast<const T*>(force_field), nn, aa, r,
+                    static_cast<T*>(rho), static_cast<T*>(u));
+            check_launch("macroscopic kernel");
+        });
+    });
+}
+
+LBM_API int lbm_health(int lattice, int dtype, const void* f, const uint8_t* kind, int64_t n, int aa, uint32_t* out,
+                       void* stream)
+{
+    return guarded("lbm_health", [&] {
+        require(n > 0 && n < (1ll << 31), "node count out of range");
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            const uint32_t nn = (uint32_t)n;
+            k_health<L, T><<<(nn + 255) / 256, 256, 0, as_stream(stream)>>>(static_cast<const T*>(f), kind, nn, aa,
+                                                                            out);
+            check_launch("health kernel");
+        });
+    });
+}
+
+LBM_API int lbm_kernel_resources(int lattice, int dtype, int* values, int max_kernels, char* names, int names_len,
+                                 int* count)
+{
+    return guarded("lbm_kernel_resources", [&] {
+        std::string joined;
+        int k = 0;
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            auto add = [&](const char* name, const void* fn) {
+                hipFuncAttributes attr{};
+                hip_check(hipFuncGetAttributes(&attr, fn), "hipFuncGetAttributes");
+                if (k < max_kernels) {
+                    values[4 * k + 0] = attr.numRegs;
+                    values[4 * k + 1] = (int)attr.localSizeBytes;
+                    values[4 * k + 2] = (int)attr.sharedSizeBytes;
+                    values[4 * k + 3] = attr.maxThreadsPerBlock;
+                }
+                joined += (k ? ";" : "");
+                joined += name;
+                ++k;
+            };
+            add("pull", reinterpret_cast<const void*>(&k_pull<L, T, ForceMode::UNIFORM, true>));
+            add("pull_all_fluid", reinterpret_cast<const void*>(&k_pull<L, T, ForceMode::UNIFORM, false>));
+            add("aa_even", reinterpret_cast<const void*>(&k_aa_even<L, T, ForceMode::UNIFORM, true>));
+            add("aa_odd", reinterpret_cast<const void*>(&k_aa_odd<L, T, ForceMode::UNIFORM, true>));
+            add("aa_even_all_fluid", reinterpret_cast<const void*>(&k_aa_even<L, T, ForceMode::UNIFORM, false>));
+            add("aa_odd_all_fluid", reinterpret_cast<const void*>(&k_aa_odd<L, T, ForceMode::UNIFORM, false>));
+            add("aa_odd_field", reinterpret_cast<const void*>(&k_aa_odd<L, T, ForceMode::FIELD, true>));
+            add("aa_odd_dense", reinterpret_cast<const void*>(&k_aa_odd<L, T, ForceMode::DENSE, true>));
+            if constexpr (L::D == 2) add("lds", reinterpret_cast<const void*>(&k_lds<L, T, ForceMode::UNIFORM, true>));
+        });
+        *count = k;
+        if (names && names_len > 0) {
+            std::strncpy(names, joined.c_str(), (size_t)names_len - 1);
+            names[names_len - 1] = '\0';
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,391 @@
+"""HIP backend: fused stream-collide kernels on MI355X, driven through the C ABI.
+
+Replaces ``CudaSolver`` of the reference (``src/lbm/backends/cuda_solver.py:179-325``)
+behind the same ``Solver`` contract. Device memory is allocated by PyTorch and
+handed to ``liblbm_hip.so`` as raw pointers together with the current torch
+stream; no launch synchronises.
+
+Per time step (``pull``/``lds``) or pair of half-steps (``aa``):
+
+1. pre-pass for the layout the next kernel reads (``pull``, ``aa_odd``,
+   ``aa_even``): link rules (``lbm_rules_apply``), then the fused IBM force
+   (``lbm_ibm_force``);
+2. the stream-collide kernel (``lbm_pull_step``, ``lbm_aa_odd`` /
+   ``lbm_aa_even``, ``lbm_lds_step``).
+
+Between ``run`` calls the AA buffer is in its post-even layout: slot opp(i)
+holds f*_i (``state_layout = "aa_odd"``).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from lbm.backends import BackendUnavailable, _require_device, load_library
+from lbm.backends._native import DTYPE_CODES, FORCE_DENSE, FORCE_FIELD, FORCE_UNIFORM, pointer
+from lbm.diagnostics import MacroscopicFields
+from lbm.geometry import neighbour_index
+from lbm.lattice import VelocitySet
+from lbm.link_rules import CompiledAffine, compile_rules
+from lbm.solver import SimulationDiverged, Solver
+
+VARIANTS = ("pull", "aa", "lds")
+TWO_D_ONLY = ("lds",)
+KIND_SOLID, KIND_FORCED, KIND_NEAR_SOLID = 1, 2, 4
+
+
+def check_variant(variant: str, d: int) -> None:
+    if variant not in VARIANTS:
+        raise ValueError(f"unknown HIP variant {variant!r}; choose from {VARIANTS}")
+    if variant in TWO_D_ONLY and d != 2:
+        raise ValueError(f"HIP variant {variant!r} is 2D only; use 'pull' or 'aa' for a {d}D grid")
+
+
+def check_block(block) -> tuple[int, int]:
+    bx, by = (int(b) for b in block)
+    if bx <= 0 or by <= 0 or bx * by > 1024:
+        raise ValueError(f"block must be positive with at most 1024 threads; got {tuple(block)}")
+    return bx, by
+
+
+def default_block(params, variant: str) -> tuple[int, int]:
+    """Whole rows per workgroup where possible: bx covers nx (a multiple of the
+    64-lane wavefront, at most 256), by fills the workgroup to 256 threads."""
+    if variant == "lds":
+        return (64, 4)
+    bx = min(256, 64 * math.ceil(params.nx / 64))
+    return bx, max(1, min(params.ny, 256 // bx))
+
+
+def node_kinds(lattice: VelocitySet, solid: np.ndarray, forced_nodes=None) -> np.ndarray:
+    """Kind byte per node: solid bit, IBM-forced bit, near-solid bit (a solid node at x - c_i)."""
+    shape = solid.shape
+    flat_solid = solid.ravel()
+    kinds = flat_solid.astype(np.uint8) * KIND_SOLID
+    if forced_nodes is not None:
+        kinds[forced_nodes] |= KIND_FORCED
+    if flat_solid.any():
+        near = np.zeros(flat_solid.size, dtype=bool)
+        nodes = np.arange(flat_solid.size)
+        for c in lattice.c:
+            if any(c):
+                near |= flat_solid[neighbour_index(nodes, [-a for a in c], shape)]
+        kinds[near & ~flat_solid] |= KIND_NEAR_SOLID
+    return kinds.reshape(shape)
+
+
+class _AffinePack:
+    """A compiled affine set on the device: term-major slots / coefficients, constants."""
+
+    def __init__(self, torch, compiled: CompiledAffine, dtype, device):
+        coeff = np.asarray(compiled.coeff)
+        self.n = coeff.shape[0]
+        self.m = max(1, coeff.shape[1]) if coeff.ndim == 2 else 1
+        slots = np.zeros((self.m, self.n), dtype=np.int64)
+        coeffs = np.zeros((self.m, self.n))
+        if self.n:
+            slots[: coeff.shape[1]] = np.asarray(compiled.slots, dtype=np.int64).T
+            coeffs[: coeff.shape[1]] = coeff.T
+        self.slot = torch.as_tensor(np.ascontiguousarray(slots), device=device)
+        self.coeff = torch.as_tensor(np.ascontiguousarray(coeffs.astype(dtype)), device=device)
+        self.constant = torch.as_tensor(np.asarray(compiled.constant, dtype=dtype), device=device)
+        self.source = compiled
+
+
+class HipSolver(Solver):
+    """Fused stream-bounce-collide on MI355X.
+
+    Args:
+        params: Simulation description.
+        variant: ``"pull"``, ``"aa"`` or ``"lds"`` (2D only).
+        block: Threads per block (bx, by), at most 1024; default from ``default_block``.
+        autotune: Time a few block shapes on this grid and keep the fastest.
+    """
+
+    def __init__(self, params, variant: str = "pull", block=None, autotune: bool = False):
+        check_variant(variant, params.d)
+        block = default_block(params, variant) if block is None else check_block(block)
+        super().__init__(params)
+        torch = _require_device()
+        self._lib = load_library()
+        self._torch = torch
+        self.device = torch.device("cuda")
+        self.variant = variant
+        self.step_multiple = 2 if variant == "aa" else 1
+        self.block = block
+        p = params
+        self._np_dtype = np.dtype(p.dtype)
+        self._torch_dtype = {"float32": torch.float32, "float64": torch.float64}[p.dtype]
+        self._lattice_code = p.lattice.code
+        self._dtype_code = DTYPE_CODES[p.dtype]
+        self._n = p.n_nodes
+        if self._n >= 2**31:
+            raise ValueError("grids are limited to 2^31 nodes")
+        self._nz = p.nz or 1
+        self._rates = p.relaxation_rates
+        self._force3 = tuple(p.force) + (0.0,) * (3 - p.d)
+        self._body = None
+
+        self._ibm = None
+        forced = None
+        if p.immersed is not None:
+            from lbm.ibm import ImmersedForcing
+
+            self._ibm = ImmersedForcing(p)
+            forced = self._ibm.nodes
+        kinds = node_kinds(p.lattice, p.solid, forced)
+        self._mask_mode = 1 if (p.solid.any() or forced is not None) else 0
+        self._kind = torch.as_tensor(kinds.ravel(), device=self.device)
+        self._wall = torch.as_tensor(
+            np.ascontiguousarray(p.wall_velocity.reshape(p.d, -1).astype(self._np_dtype)), device=self.device
+        )
+        self._force_mode = FORCE_FIELD if self._ibm is not None else FORCE_UNIFORM
+        self._field = None
+        if self._ibm is not None:
+            self._setup_ibm()
+
+        self._rule_passes = {}
+        if p.link_rules is not None and len(p.link_rules):
+            layouts = ("aa_odd", "aa_even") if variant == "aa" else ("pull",)
+            for layout in layouts:
+                compiled = compile_rules(p.lattice, p.link_rules, layout, p.solid, p.wall_velocity)
+                pack = _AffinePack(torch, compiled.values, self._np_dtype, self.device)
+                pack.target = torch.as_tensor(np.asarray(compiled.target, dtype=np.int64), device=self.device)
+                pack.scratch = torch.empty(max(1, pack.n), dtype=self._torch_dtype, device=self.device)
+                self._rule_passes[layout] = pack
+        self._affine_cache: dict[int, _AffinePack] = {}
+        self._f = torch.empty((p.lattice.q, self._n), dtype=self._torch_dtype, device=self.device)
+        self._f_next = None if variant == "aa" else torch.empty_like(self._f)
+        self.initialize()
+        if autotune:
+            from lbm.backends.autotune import tune_block
+
+            self.block = tune_block(self)
+
+    # ------------------------------------------------------------------ plumbing
+
+    @property
+    def _stream(self) -> int:
+        return self._torch.cuda.current_stream().cuda_stream
+
+    def _call(self, name: str, *args) -> None:
+        self._lib.call(name, *args)
+
+    def _setup_ibm(self) -> None:
+        torch, ibm, p = self._torch, self._ibm, self.params
+        dev, dt = self.device, self._np_dtype
+        self._field = torch.zeros((p.d, self._n), dtype=self._torch_dtype, device=dev)
+        self._body_force = torch.zeros(p.d, dtype=self._torch_dtype, device=dev)
+        self._ibm_nodes = torch.as_tensor(ibm.nodes, device=dev)
+        self._ibm_inverse = torch.as_tensor(ibm.inverse.astype(np.int32), device=dev)
+        self._ibm_weights = torch.as_tensor(np.ascontiguousarray(ibm.weights.astype(dt)), device=dev)
+        self._ibm_areas = torch.as_tensor(np.asarray(ibm.body.areas, dtype=dt), device=dev)
+        self._ibm_start = torch.as_tensor(ibm.spread_start, device=dev)
+        self._ibm_entry = torch.as_tensor(ibm.spread_entry, device=dev)
+        import ctypes
+
+        size = ctypes.c_int64()
+        self._lib.raw("lbm_ibm_scratch_size")(
+            self._lattice_code, self._dtype_code, ibm.n_nodes, ibm.n_markers, ctypes.byref(size)
+        )
+        self._ibm_scratch = torch.empty(size.value // dt.itemsize, dtype=self._torch_dtype, device=dev)
+        self._ibm_packs = {}
+
+    def _ibm_pack(self, layout: str) -> _AffinePack:
+        if layout not in self._ibm_packs:
+            self._ibm_packs[layout] = _AffinePack(self._torch, self._ibm.compiled(layout), self._np_dtype, self.device)
+        return self._ibm_packs[layout]
+
+    def set_body_force_field(self, field) -> None:
+        """Dense per-node body force (d, *shape) added to the uniform force (``None`` clears it)."""
+        if field is None:
+            self._body = None
+            self._force_mode = FORCE_FIELD if self._ibm is not None else FORCE_UNIFORM
+            return
+        t = self._torch.as_tensor(np.asarray(field), dtype=self._torch_dtype, device=self.device)
+        self._body = t.reshape(self.params.d, self._n).contiguous()
+        self._force_mode = FORCE_DENSE
+
+    # ------------------------------------------------------------------ Solver API
+
+    @property
+    def lattice_bytes(self) -> int:
+        p = self.params
+        return (1 if self.variant == "aa" else 2) * p.lattice.q * self._n * self._np_dtype.itemsize
+
+    @property
+    def state_layout(self) -> str:
+        return "aa_odd" if self.variant == "aa" else "pull"
+
+    def initialize(self, rho=1.0, ux=0.0, uy=0.0, uz=0.0) -> None:
+        drho, u = self._initial_fields(rho, ux, uy, uz)
+        torch = self._torch
+        dev = self.device
+        dr = torch.as_tensor(np.ascontiguousarray(drho).ravel(), device=dev)
+        comps = [torch.as_tensor(np.ascontiguousarray(a).ravel(), device=dev) for a in u]
+        uz_t = comps[2] if len(comps) == 3 else None
+        self._call(
+            "lbm_init_equilibrium", self._lattice_code, self._dtype_code, pointer(self._f), pointer(dr),
+            pointer(comps[0]), pointer(comps[1]), pointer(uz_t), self._n, int(self.variant == "aa"), self._stream,
+        )
+        if self._f_next is not None:
+            self._f_next.copy_(self._f)
+        if self._field is not None:
+            self._field.zero_()
+            self._body_force.zero_()
+        self.time_step = 0
+
+    def _step_args(self):
+        p = self.params
+        wp, wm = self._rates
+        fx, fy, fz = self._force3
+        bx, by = self.block
+        return (p.nx, p.ny, self._nz, wp, wm, fx, fy, fz, self._force_mode, self._mask_mode, bx, by, self._stream)
+
+    def _advance(self, steps: int) -> None:
+        L, D = self._lattice_code, self._dtype_code
+        kind = pointer(self._kind)
+        field = pointer(self._field)
+        body = pointer(self._body)
+        args = self._step_args()
+        if self.variant == "aa":
+            for _ in range(steps // 2):
+                self._pre_pass("aa_odd")
+                self._call("lbm_aa_odd", L, D, pointer(self._f), kind, pointer(self._wall), field, body, *args)
+                self._pre_pass("aa_even")
+                self._call("lbm_aa_even", L, D, pointer(self._f), kind, field, body, *args)
+            return
+        name = "lbm_lds_step" if self.variant == "lds" else "lbm_pull_step"
+        for _ in range(steps):
+            self._pre_pass("pull")
+            self._call(name, L, D, pointer(self._f), pointer(self._f_next), kind, pointer(self._wall), field, body, *args)
+            self._f, self._f_next = self._f_next, self._f
+
+    def _pre_pass(self, layout: str) -> None:
+        """Link rules, then the IBM force field, for the layout the next kernel reads."""
+        pack = self._rule_passes.get(layout)
+        if pack is not None:
+            self._call(
+                "lbm_rules_apply", self._dtype_code, pointer(self._f), pointer(pack.slot), pointer(pack.coeff),
+                pointer(pack.constant), pointer(pack.target), pack.n, pack.m, pointer(pack.scratch), self._stream,
+            )
+        if self._ibm is not None:
+            ibm = self._ibm
+            mp = self._ibm_pack(layout)
+            fx, fy, fz = self._force3
+            self._call(
+                "lbm_ibm_force", self._lattice_code, self._dtype_code, pointer(self._f), pointer(mp.slot),
+                pointer(mp.coeff), pointer(mp.constant), mp.m, pointer(self._ibm_nodes), ibm.n_nodes,
+                pointer(self._ibm_inverse), pointer(self._ibm_weights), pointer(self._ibm_areas), ibm.n_markers,
+                pointer(self._ibm_start), pointer(self._ibm_entry), ibm.iterations, fx, fy, fz, self._n,
+                pointer(self._field), pointer(self._body_force), pointer(self._ibm_scratch), self._stream,
+            )
+
+    def synchronize(self) -> None:
+        self._torch.cuda.synchronize()
+
+    def _check_health(self) -> None:
+        torch = self._torch
+        out = torch.tensor([-1, 0], dtype=torch.int32, device=self.device)
+        self._call(
+            "lbm_health", self._lattice_code, self._dtype_code, pointer(self._f), pointer(self._kind), self._n,
+            int(self.variant == "aa"), pointer(out), self._stream,
+        )
+        first, nonfinite = out.cpu().numpy().view(np.uint32).tolist()
+        if first == 0xFFFFFFFF:
+            return
+        location = tuple(int(i) for i in np.unravel_index(first, self.params.shape))
+        reason = "non-finite populations" if nonfinite else "non-positive density"
+        raise SimulationDiverged(self.time_step, location, reason)
+
+    def device_populations(self):
+        """Post-collision populations in natural order as a (q, N) device tensor (a copy for AA)."""
+        if self.variant == "aa":
+            return self._f[list(self.params.lattice.opposite)]
+        return self._f
+
+    def populations(self) -> np.ndarray:
+        p = self.params
+        f = self._f.cpu().numpy()
+        if self.variant == "aa":
+            f = f[list(p.lattice.opposite)]
+        return np.ascontiguousarray(f).reshape(p.lattice.q, *p.shape)
+
+    def _state_buffer(self):
+        return self._f
+
+    def evaluate_affine(self, compiled: CompiledAffine) -> np.ndarray:
+        pack = self._affine_pack(compiled)
+        out = self._torch.empty(max(1, pack.n), dtype=self._torch_dtype, device=self.device)
+        if pack.n:
+            self._call(
+                "lbm_affine_values", self._dtype_code, pointer(self._f), pointer(pack.slot), pointer(pack.coeff),
+                pointer(pack.constant), pack.n, pack.m, pointer(out), self._stream,
+            )
+        return out[: pack.n].cpu().numpy().astype(np.float64)
+
+    def _affine_pack(self, compiled: CompiledAffine) -> _AffinePack:
+        key = id(compiled)
+        pack = self._affine_cache.get(key)
+        if pack is None or pack.source is not compiled:
+            pack = _AffinePack(self._torch, compiled, self._np_dtype, self.device)
+            self._affine_cache[key] = pack
+        return pack
+
+    def force_on_device(self, probe, out) -> None:
+        """Momentum-exchange force of ``probe`` into the float64 device tensor ``out`` (d,), no host sync."""
+        compiled = probe.compiled(self.state_layout)
+        pack = self._affine_pack(compiled)
+        weight = getattr(pack, "weight", None)
+        if weight is None:
+            weight = self._torch.as_tensor(np.ascontiguousarray(probe.weight, dtype=np.float64), device=self.device)
+            pack.weight = weight
+        self._call(
+            "lbm_affine_force", self._dtype_code, pointer(self._f), pointer(pack.slot), pointer(pack.coeff),
+            pointer(pack.constant), pointer(weight), pack.n, pack.m, self.params.d, pointer(out), self._stream,
+        )
+
+    @property
+    def immersed_force(self) -> np.ndarray:
+        if self._ibm is None:
+            raise AttributeError("this simulation has no immersed body")
+        return self._body_force.cpu().numpy().astype(np.float64)
+
+    def macroscopic(self) -> MacroscopicFields:
+        p = self.params
+        torch = self._torch
+        rho = torch.empty(self._n, dtype=self._torch_dtype, device=self.device)
+        u = torch.empty((p.d, self._n), dtype=self._torch_dtype, device=self.device)
+        fx, fy, fz = self._force3
+        self._call(
+            "lbm_macroscopic", self._lattice_code, self._dtype_code, pointer(self._f), pointer(self._kind),
+            pointer(self._field), self._n, int(self.variant == "aa"), fx, fy, fz,
+            FORCE_FIELD if self._field is not None else FORCE_UNIFORM, pointer(rho), pointer(u), self._stream,
+        )
+        rho_h = rho.cpu().numpy().reshape(p.shape)
+        u_h = u.cpu().numpy().reshape(p.d, *p.shape)
+        return MacroscopicFields(rho=rho_h, u=tuple(u_h[a].copy() for a in range(p.d)), solid=np.array(p.solid))
+
+    def kernel_resources(self) -> dict[str, dict[str, int]]:
+        """Registers, scratch bytes per lane, static LDS bytes and max threads of each step kernel."""
+        import ctypes
+
+        values = (ctypes.c_int * (4 * 16))()
+        names = ctypes.create_string_buffer(1024)
+        count = ctypes.c_int()
+        self._call("lbm_kernel_resources", self._lattice_code, self._dtype_code, values, 16, names, 1024,
+                   ctypes.byref(count))
+        out = {}
+        for k, name in enumerate(names.value.decode().split(";")[: count.value]):
+            out[name] = {
+                "registers": values[4 * k],
+                "local_bytes": values[4 * k + 1],
+                "shared_bytes": values[4 * k + 2],
+                "max_threads": values[4 * k + 3],
+            }
+        return out
+
+
+__all__ = ["HipSolver", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block"]

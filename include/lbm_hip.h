@@ -1,0 +1,178 @@
+/*
+ * lbm_hip.h - C ABI of the MI355X (gfx950) lattice Boltzmann library liblbm_hip.so.
+ *
+ * Every entry point takes plain device pointers (allocated by the caller),
+ * sizes and scalars, launches on the given HIP stream (NULL = default stream)
+ * without synchronising, and returns LBM_OK or a negative LBM_ERR_* code;
+ * lbm_last_error() then describes the failure. The library never frees
+ * caller memory.
+ *
+ * Populations are stored structure-of-arrays, f[q][z][y][x] (x fastest,
+ * 2D grids have nz = 1), shifted by the lattice weights (f~ = f - w), in
+ * float (LBM_F32) or double (LBM_F64). kind[] holds one byte per node:
+ * bit 0 solid, bit 1 IBM-forced fluid node, bit 2 fluid node with a solid
+ * neighbour at some x - c_i (set by the host, lets interior wavefronts skip
+ * the bounce-back checks). wall_u and force fields are [d][z][y][x] arrays
+ * of the population precision.
+ *
+ * Each declaration names the reference interface it replaces
+ * (ShauryaAttreya37/High-Performance-LBM-On-GPU-, paths relative to its root).
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_API __attribute__((visibility("default")))
+
+enum { LBM_D2Q9 = 0, LBM_D3Q19 = 1 };
+enum { LBM_F32 = 0, LBM_F64 = 1 };
+/* Force handling: uniform F0; F0 plus a sparse IBM field at forced nodes;
+ * F0 plus a dense per-node body force (plus the IBM field at forced nodes). */
+enum { LBM_FORCE_UNIFORM = 0, LBM_FORCE_FIELD = 1, LBM_FORCE_DENSE = 2 };
+enum {
+    LBM_OK = 0,
+    LBM_ERR_INVALID = -1,  /* bad argument */
+    LBM_ERR_HIP = -2,      /* HIP runtime error */
+    LBM_ERR_INTERNAL = -3
+};
+
+/* Message of the last failed call on this thread ("" if none). */
+LBM_API const char* lbm_last_error(void);
+/* Library version string. */
+LBM_API const char* lbm_version(void);
+
+/* Host-side copy of the compiled lattice tables: c[3*Q] (x, y, z rows), opp[Q], w[Q].
+ * Replaces lattice.cuda_preamble (src/lbm/lattice.py:174-214); needs no GPU. */
+LBM_API int lbm_lattice_tables(int lattice, int* q, int* c, int* opp, double* w);
+
+/* Same tables read back from device code (one tiny kernel) into device buffers
+ * c[3*Q] (int32), opp[Q] (int32), w[Q] (dtype).
+ * Replaces the on-device constant check of tests/test_cuda_backend.py:160-183. */
+LBM_API int lbm_lattice_tables_device(int lattice, int dtype, int* c, int* opp, void* w, void* stream);
+
+/* Fused pull stream + halfway bounce-back (Ladd) + TRT/Guo collision, two lattices.
+ * Replaces stream_collide_naive (src/lbm/backends/kernels/stream_collide_naive.cu:15-49),
+ * launched from CudaSolver._advance (src/lbm/backends/cuda_solver.py:279-284).
+ * mask_mode 0: no solid/forced nodes anywhere (kind ignored, may be NULL); 1: use kind.
+ * force_field: IBM field (FIELD/DENSE modes, read at forced nodes); body_field:
+ * dense per-node force (DENSE mode). bx * by <= 1024 threads per block. */
+LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst,
+                          const uint8_t* kind, const void* wall_u, const void* force_field,
+                          const void* body_field, int nx, int ny, int nz, double omega_plus,
+                          double omega_minus, double fx, double fy, double fz, int force_mode,
+                          int mask_mode, int bx, int by, void* stream);
+
+/* AA pattern, even half-step: read F[i](x), collide, write F[opp i](x).
+ * Replaces aa_even (src/lbm/backends/kernels/stream_collide_aa.cu:33-57). */
+LBM_API int lbm_aa_even(int lattice, int dtype, void* f, const uint8_t* kind,
+                        const void* force_field, const void* body_field, int nx, int ny, int nz,
+                        double omega_plus, double omega_minus, double fx, double fy, double fz,
+                        int force_mode, int mask_mode, int bx, int by, void* stream);
+
+/* AA pattern, odd half-step: pull F[opp i](x - c_i), collide, push to F[i](x + c_i);
+ * solid targets are reflected into F[opp i](x) with the wall term.
+ * Replaces aa_odd (src/lbm/backends/kernels/stream_collide_aa.cu:59-98). */
+LBM_API int lbm_aa_odd(int lattice, int dtype, void* f, const uint8_t* kind, const void* wall_u,
+                       const void* force_field, const void* body_field, int nx, int ny, int nz,
+                       double omega_plus, double omega_minus, double fx, double fy, double fz,
+                       int force_mode, int mask_mode, int bx, int by, void* stream);
+
+/* Two-lattice step with each block's populations plus a one-node halo staged in LDS
+ * (2D only; tile = block). Replaces stream_collide_tiled
+ * (src/lbm/backends/kernels/stream_collide_tiled.cu:24-79). */
+LBM_API int lbm_lds_step(int lattice, int dtype, const void* f_src, void* f_dst,
+                         const uint8_t* kind, const void* wall_u, const void* force_field,
+                         const void* body_field, int nx, int ny, int nz, double omega_plus,
+                         double omega_minus, double fx, double fy, double fz, int force_mode,
+                         int mask_mode, int bx, int by, void* stream);
+
+/* Shifted equilibrium from drho and u (device arrays of dtype, n nodes each; uz
+ * ignored in 2D) into f; aa != 0 stores direction i in slot opp(i) (post-even layout).
+ * Replaces Solver._initial_populations + CudaSolver.initialize
+ * (src/lbm/solver.py:101-117, src/lbm/backends/cuda_solver.py:258-267). */
+LBM_API int lbm_init_equilibrium(int lattice, int dtype, void* f, const void* drho, const void* ux,
+                                 const void* uy, const void* uz, int64_t n, int aa, void* stream);
+
+/* rho and u (rho[n], u[d][n], dtype) from stored post-collision populations; solid
+ * nodes give rho = 1, u = 0. aa != 0 reads the post-even AA layout directly.
+ * force_mode / force_field as for the step kernels (the field enters at forced nodes).
+ * Replaces Solver.macroscopic (src/lbm/solver.py:209-221, diagnostics.py:72-80). */
+LBM_API int lbm_macroscopic(int lattice, int dtype, const void* f, const uint8_t* kind,
+                            const void* force_field, int64_t n, int aa, double fx, double fy,
+                            double fz, int force_mode, void* rho, void* u, void* stream);
+
+/* Divergence watchdog: out[0] = smallest flat index of a fluid node with a
+ * non-finite drho or drho <= -1 (unchanged if none), out[1] |= 1 if any fluid
+ * drho is non-finite. The caller initialises out = {UINT32_MAX, 0}.
+ * Replaces Solver._check_health (src/lbm/solver.py:162-172). */
+LBM_API int lbm_health(int lattice, int dtype, const void* f, const uint8_t* kind, int64_t n,
+                       int aa, uint32_t* out, void* stream);
+
+/* Link-rule pre-pass: value[r] = constant[r] + sum_t coeff[t][r] f[slot[t][r]] for all
+ * rules first (pairwise sum over the m padded terms, as NumPy's row sum), then
+ * f[target[r]] = value[r]. slot/coeff are term-major (m x n_rules); scratch holds
+ * n_rules values of dtype (unused when n_rules <= 4096: one fused launch).
+ * Replaces rules_evaluate / rules_write (src/lbm/backends/kernels/link_rules.cu:15-43)
+ * and _RulePass.__call__ (src/lbm/backends/cuda_solver.py:157-176). */
+LBM_API int lbm_rules_apply(int dtype, void* f, const int64_t* slot, const void* coeff,
+                            const void* constant, const int64_t* target, int64_t n_rules, int m,
+                            void* scratch, void* stream);
+
+/* Values of compiled affine expressions on the state: out[r] (dtype), same
+ * arithmetic as the rule evaluation. Replaces Solver.evaluate_affine
+ * (src/lbm/solver.py:178-188) and link_rules.evaluate_affine (src/lbm/link_rules.py:269-272). */
+LBM_API int lbm_affine_values(int dtype, const void* f, const int64_t* slot, const void* coeff,
+                              const void* constant, int64_t n_rows, int m, void* out, void* stream);
+
+/* Momentum-exchange force: out[a] (double) += sum_r weight[r][a] * value[r] over the
+ * affine rows (value rounded to dtype, then widened), summed in row order by one
+ * workgroup; d components. Replaces ForceProbe.evaluate's host matmul
+ * (src/lbm/forces.py:98-103) so a per-step drag history stays on the device. */
+LBM_API int lbm_affine_force(int dtype, const void* f, const int64_t* slot, const void* coeff,
+                             const void* constant, const double* weight, int64_t n_rows, int m,
+                             int d, double* out, void* stream);
+
+/* Bytes of device scratch lbm_ibm_force needs for n_nodes unique stencil nodes
+ * and n_markers markers. Host only. */
+LBM_API int lbm_ibm_scratch_size(int lattice, int dtype, int n_nodes, int n_markers, int64_t* bytes);
+
+/* Immersed-boundary direct forcing, all multi-direct iterations in one launch
+ * (one workgroup). Moment rows ((d+1) * n_nodes affine rows, term-major m-padded,
+ * compiled for the current layout) give drho and rho u at the unique stencil nodes
+ * nodes[n_nodes]; inverse[n_markers][3^d] and weights[n_markers][3^d] describe the
+ * Roma stencils, areas[n_markers] the surface elements; spread_start[n_nodes + 1] /
+ * spread_entry[] list, per node, the flat (marker, stencil) entries touching it in
+ * increasing order. Writes field[a][nodes[n]] (d x n_total, dtype) and
+ * body_force[d] (dtype) = -(sum of marker forces over all iterations).
+ * Replaces ImmersedForcing.force_field (src/lbm/ibm.py:189-214) and the field
+ * scatter of CudaSolver._pre_pass (src/lbm/backends/cuda_solver.py:286-292). */
+LBM_API int lbm_ibm_force(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
+                          const void* constant, int m, const int64_t* nodes, int n_nodes, const int* inverse,
+                          const void* weights, const void* areas, int n_markers, const int* spread_start,
+                          const int* spread_entry, int iterations, double fx, double fy, double fz,
+                          int64_t n_total, void* field, void* body_force, void* scratch, void* stream);
+
+/* Grid-stride copy of `bytes` (multiple of 16, 16-byte aligned buffers) with
+ * 16-byte accesses: the device-copy bandwidth probe. Replaces the cp.copyto peak
+ * measurement of benchmarks/bench_mlups.py:66-79. */
+LBM_API int lbm_device_copy(void* dst, const void* src, int64_t bytes, void* stream);
+
+/* Per-kernel resources of the compiled step kernels for (lattice, dtype):
+ * fills up to max_kernels entries of {registers, scratch (private) bytes per lane,
+ * static LDS bytes, max threads per block} and writes their names (';'-separated)
+ * into names. Needs a device.
+ * Replaces CudaSolver.kernel_resources (src/lbm/backends/cuda_solver.py:311-325). */
+LBM_API int lbm_kernel_resources(int lattice, int dtype, int* values, int max_kernels,
+                                 char* names, int names_len, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

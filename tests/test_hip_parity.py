@@ -1,0 +1,136 @@
+"""HIP kernels against the NumPy oracle on geometries with walls, a moving lid, an obstacle and
+a body force (tests/test_cuda_backend.py:186-272 of the reference).
+
+The kernels evaluate every node update in the oracle's operation order with no fused
+multiply-add, so populations agree to the last bit where the arithmetic path is the same
+(pull and AA layouts). The asserted bounds are the reference's (< 1e-15 float64, < 1e-7
+float32 after 40 steps); bit-identity is asserted separately where it is expected.
+"""
+
+import numpy as np
+import pytest
+
+from lbm import SimulationParams, make_solver
+from lbm.boundaries import cavity_walls
+from tests.helpers import fluid_diff, gpu_cases, mixed_params, run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("collision", ["bgk", "trt"])
+@pytest.mark.parametrize(("backend", "d"), gpu_cases())
+def test_float64_matches_oracle_to_round_off(backend, d, collision):
+    params = mixed_params("float64", collision, d)
+    reference, gpu = run_pair(params, backend)
+    assert fluid_diff(params, reference, gpu) < 1e-15
+
+
+@pytest.mark.parametrize(("backend", "d"), gpu_cases())
+def test_float32_matches_float32_oracle(backend, d):
+    params = mixed_params("float32", "trt", d)
+    reference, gpu = run_pair(params, backend)
+    assert fluid_diff(params, reference, gpu) < 1e-7
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize(("backend", "d"), gpu_cases())
+def test_populations_are_bit_identical_to_the_oracle(backend, d, dtype):
+    """Same operation order, no contraction: the stored states are equal bit for bit."""
+    params = mixed_params(dtype, "trt", d)
+    reference, gpu = run_pair(params, backend)
+    fluid = params.fluid
+    np.testing.assert_array_equal(gpu.populations()[:, fluid], reference.populations()[:, fluid])
+
+
+@pytest.mark.parametrize(("backend", "d"), gpu_cases())
+def test_macroscopic_fields_match_oracle(backend, d):
+    params = mixed_params("float64", "trt", d)
+    reference, gpu = run_pair(params, backend)
+    ref, out = reference.macroscopic(), gpu.macroscopic()
+    np.testing.assert_allclose(out.rho, ref.rho, atol=1e-14)
+    for axis in range(d):
+        np.testing.assert_allclose(out.u[axis], ref.u[axis], atol=1e-14)
+
+
+@pytest.mark.parametrize("backend", ["hip-pull", "hip-aa"])
+def test_closed_3d_box_with_moving_lid_conserves_mass(backend):
+    solid, wall = cavity_walls(n=14, lid_velocity=0.05, dimensions=3)
+    params = SimulationParams(nx=16, ny=16, nz=16, tau=0.7, solid=solid, wall_velocity=wall, dtype="float64")
+    solver = make_solver(params, backend=backend)
+    mass0 = solver.macroscopic().total_mass
+    solver.run(400)
+    assert solver.macroscopic().total_mass == pytest.approx(mass0, rel=1e-13)
+
+
+def test_aa_backend_advances_in_pairs_only():
+    solver = make_solver(SimulationParams(nx=8, ny=8, tau=0.8), backend="hip-aa")
+    with pytest.raises(ValueError, match="multiples of 2"):
+        solver.run(3)
+    solver.run(4)
+    assert solver.time_step == 4
+
+
+def test_aa_backend_uses_half_the_lattice_memory():
+    params = SimulationParams(nx=64, ny=32, tau=0.8)
+    pull = make_solver(params, backend="hip-pull")
+    aa = make_solver(params, backend="hip-aa")
+    assert aa.lattice_bytes * 2 == pull.lattice_bytes == 2 * 9 * 64 * 32 * 4
+
+
+def test_3d_lattice_memory_counts_nineteen_populations_per_node():
+    params = SimulationParams(nx=16, ny=12, nz=8, tau=0.8)
+    assert make_solver(params, backend="hip-aa").lattice_bytes == 19 * 16 * 12 * 8 * 4
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("d", [2, 3])
+def test_kernels_keep_populations_in_registers(d, dtype):
+    sizes = {"nx": 8, "ny": 8} | ({"nz": 8} if d == 3 else {})
+    solver = make_solver(SimulationParams(**sizes, tau=0.8, dtype=dtype), backend="hip-pull")
+    resources = solver.kernel_resources()
+    assert resources
+    for name, res in resources.items():
+        assert res["local_bytes"] == 0, f"{name} spills to scratch"
+
+
+def test_invalid_block_is_rejected():
+    with pytest.raises(ValueError, match="block"):
+        make_solver(SimulationParams(nx=8, ny=8, tau=0.8), backend="hip-pull", block=(64, 32))
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_aa_and_pull_agree_bit_for_bit_over_many_steps(d):
+    """Race freedom of the in-place AA update: it must equal the two-lattice result exactly."""
+    params = mixed_params("float32", "trt", d)
+    _, pull = run_pair(params, "hip-pull", steps=200)
+    _, aa = run_pair(params, "hip-aa", steps=200)
+    np.testing.assert_array_equal(aa.populations()[:, params.fluid], pull.populations()[:, params.fluid])
+
+
+@pytest.mark.parametrize("block", [(32, 8), (64, 4), (128, 2), (256, 1), (16, 16)])
+def test_results_do_not_depend_on_the_block_shape(block):
+    params = mixed_params("float64", "trt", 3)
+    _, ref = run_pair(params, "hip-aa", steps=10, block=(64, 4))
+    _, other = run_pair(params, "hip-aa", steps=10, block=block)
+    np.testing.assert_array_equal(other.populations(), ref.populations())
+
+
+def test_device_lattice_tables_match_the_host_tables():
+    import torch
+
+    from lbm.backends._native import library
+    from lbm.lattice import LATTICES
+
+    lib = library()
+    for code, vs in enumerate(LATTICES.values()):
+        for dtype, tdt in ((0, torch.float32), (1, torch.float64)):
+            c = torch.zeros(3 * vs.q, dtype=torch.int32, device="cuda")
+            opp = torch.zeros(vs.q, dtype=torch.int32, device="cuda")
+            w = torch.zeros(vs.q, dtype=tdt, device="cuda")
+            lib.call("lbm_lattice_tables_device", code, dtype, c.data_ptr(), opp.data_ptr(), w.data_ptr(), None)
+            torch.cuda.synchronize()
+            ci = c.cpu().numpy().reshape(3, vs.q)
+            for axis in range(3):
+                np.testing.assert_array_equal(ci[axis], vs.component(axis))
+            np.testing.assert_array_equal(opp.cpu().numpy(), vs.opposite)
+            np.testing.assert_array_equal(w.cpu().numpy(), np.asarray(vs.w, dtype=w.cpu().numpy().dtype))

@@ -14,11 +14,15 @@
 //   P5  total += sum_l F_l;  force(x) += sum over entries touching x of w_ls F_l
 //   end: field[:, nodes] = force, body_force = -total
 //
-// Summation orders are NumPy's: pairwise over the stencil and over markers
-// (row sums), and for the spread, float64 accumulation in flat (marker,
-// stencil) order from 0 then added to the float64-widened force, as
-// numpy.bincount does; each node walks its own precomputed entry list, so the
-// result is deterministic without atomics.
+// Summation orders are the ones NumPy uses for the reference's expressions:
+// pairwise over the padded terms of a moment row and over the stencil for
+// rho_l (contiguous row sums), but sequential over the stencil for U_l and over
+// the markers for the total, because u[:, inverse] is laid out with the
+// component axis innermost and NumPy then reduces those axes as outer loops;
+// the spread accumulates in float64 in flat (marker, stencil) order from 0 and
+// is added to the float64-widened force, as numpy.bincount does. Each node
+// walks its own precomputed entry list, so the result is deterministic
+// without atomics.
 
 #include "lbm_common.h"
 
@@ -51,23 +55,17 @@ __device__ __forceinline__ T pairwise_block(int m, Term term)
     return s;
 }
 
-// NumPy pairwise sum of a contiguous array of any length up to 128 * 2^DEPTH:
-// blocks of <= 128 elements, halves split at a multiple of 8.
-template <class T, int DEPTH>
-__device__ __noinline__ T pairwise_array(const T* a, int n)
+// Sequential sum term(0) + term(1) + ..., the order NumPy uses when the reduced
+// axis is an outer loop of the reduction.
+template <class T, class Term>
+__device__ __forceinline__ T sequential_sum(int n, Term term)
 {
-    if (n <= 128) return pairwise_block<T>(n, [&](int p) { return a[p]; });
-    if constexpr (DEPTH == 0) {
-        return T(0);  // unreachable: the host bounds n
-    } else {
-        int n2 = n / 2;
-        n2 -= n2 % 8;
-        return pairwise_array<T, DEPTH - 1>(a, n2) + pairwise_array<T, DEPTH - 1>(a + n2, n - n2);
-    }
+    T s = term(0);
+    for (int p = 1; p < n; ++p) s = s + term(p);
+    return s;
 }
 
 constexpr int IBM_THREADS = 1024;
-constexpr int IBM_DEPTH = 14;  // up to 2^21 markers
 
 template <class T>
 struct IbmScratch {
@@ -158,13 +156,16 @@ __global__ __launch_bounds__(IBM_THREADS) void k_ibm(const T* __restrict__ F, co
             const T two_rho_m = T(2) * s.rho_m[l];
             for (int a = 0; a < D; ++a) {
                 const T* ua = s.u + (size_t)a * nn;
-                const T um = pairwise_block<T>(S, [&](int p) { return w[p] * ua[inv[p]]; });
+                const T um = sequential_sum<T>(S, [&](int p) { return w[p] * ua[inv[p]]; });
                 s.f_m[(size_t)a * nm + l] = (two_rho_m * (T(0) - um)) * areas[l];
             }
         }
         __syncthreads();
         // P5: marker totals (one thread per component) and the deterministic spread.
-        if (tid < D) s.total[tid] = s.total[tid] + pairwise_array<T, IBM_DEPTH>(s.f_m + (size_t)tid * nm, nm);
+        if (tid < D) {
+            const T* fa = s.f_m + (size_t)tid * nm;
+            s.total[tid] = s.total[tid] + sequential_sum<T>(nm, [&](int l) { return fa[l]; });
+        }
         for (int n = tid; n < nn; n += IBM_THREADS) {
             const int begin = spread_start[n], end = spread_start[n + 1];
             for (int a = 0; a < D; ++a) {
@@ -211,7 +212,6 @@ LBM_API int lbm_ibm_force(int lattice, int dtype, const void* f, const int64_t* 
 {
     return guarded("lbm_ibm_force", [&] {
         require(n_nodes > 0 && n_markers > 0, "the immersed body needs markers and stencil nodes");
-        require(n_markers <= (128 << IBM_DEPTH), "too many markers");
         require(iterations >= 1, "iterations must be >= 1");
         require(m >= 1 && m <= 128, "moment rows need 1 <= m <= 128 terms");
         require(n_total > 0 && n_total < (1ll << 31), "node count out of range");

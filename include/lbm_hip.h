@@ -158,7 +158,7 @@ LBM_API int lbm_ibm_force(int lattice, int dtype, const void* f, const int64_t* 
                           const int* spread_entry, int iterations, double fx, double fy, double fz,
                           int64_t n_total, void* field, void* body_force, void* scratch, void* stream);
 
-/* Grid-stride copy of `bytes` (multiple of 16, 16-byte aligned buffers) with
+/* Copy of `bytes` (multiple of 16, 16-byte aligned buffers) with
  * 16-byte accesses: the device-copy bandwidth probe. Replaces the cp.copyto peak
  * measurement of benchmarks/bench_mlups.py:66-79. */
 LBM_API int lbm_device_copy(void* dst, const void* src, int64_t bytes, void* stream);

@@ -1,0 +1,98 @@
+"""Link rules (Bouzidi, Zou-He), the momentum-exchange probe and the fused IBM kernel on the GPU,
+against the NumPy oracle (tests/test_geometry_link_rules.py:121-246, test_open_boundaries.py:63-72
+and test_ibm.py:59-101 of the reference). Each case exercises the rule compiler for the layout the
+backend reads (pull; AA odd and even)."""
+
+import numpy as np
+import pytest
+
+from lbm import make_solver
+from lbm.forces import ForceProbe
+from tests.helpers import fluid_diff, random_state
+from tests.test_golden import bouzidi_case, ibm_case, open_case
+
+pytestmark = pytest.mark.gpu
+BACKENDS = ["hip-pull", "hip-aa", "hip-lds"]
+
+
+def both(params, backend, seed=0, steps=40):
+    from oracle import ReferenceSolver
+
+    ref = ReferenceSolver(params)
+    gpu = make_solver(params, backend=backend)
+    for s in (ref, gpu):
+        s.initialize(*random_state(params, seed))
+        s.run(steps)
+    return ref, gpu
+
+
+def backends_for(d):
+    return BACKENDS if d == 2 else BACKENDS[:2]
+
+
+CASES = [(b, d) for d in (2, 3) for b in backends_for(d)]
+
+
+@pytest.mark.parametrize(("backend", "d"), CASES)
+@pytest.mark.parametrize(("dtype", "tol"), [("float64", 1e-15), ("float32", 1e-7)])
+def test_bouzidi_matches_the_oracle(backend, d, dtype, tol):
+    params, _, _ = bouzidi_case(d, dtype)
+    ref, gpu = both(params, backend)
+    assert fluid_diff(params, ref, gpu) < tol
+
+
+@pytest.mark.parametrize(("backend", "d"), CASES)
+def test_force_probe_reads_every_layout_the_same(backend, d):
+    params, links, _ = bouzidi_case(d, "float64")
+    probe = ForceProbe.build(params, links)
+    ref, gpu = both(params, backend)
+    np.testing.assert_allclose(probe.evaluate(gpu), probe.evaluate(ref), rtol=1e-12, atol=1e-16)
+
+
+@pytest.mark.parametrize(("backend", "d"), CASES)
+def test_force_on_device_matches_the_host_evaluation(backend, d):
+    import torch
+
+    params, links, _ = bouzidi_case(d, "float64")
+    probe = ForceProbe.build(params, links)
+    _, gpu = both(params, backend)
+    out = torch.zeros(d, dtype=torch.float64, device="cuda")
+    gpu.force_on_device(probe, out)
+    np.testing.assert_allclose(out.cpu().numpy(), probe.evaluate(gpu), rtol=1e-12, atol=1e-18)
+
+
+@pytest.mark.parametrize(("backend", "d"), CASES)
+def test_open_boundaries_match_the_oracle(backend, d):
+    params, _ = open_case(d)
+    ref, gpu = both(params, backend, seed=1)
+    assert fluid_diff(params, ref, gpu) < 1e-15
+
+
+@pytest.mark.parametrize(("backend", "d"), CASES)
+@pytest.mark.parametrize(("dtype", "tol"), [("float64", 1e-15), ("float32", 1e-7)])
+def test_ibm_matches_the_oracle(backend, d, dtype, tol):
+    params = ibm_case(d, dtype)
+    ref, gpu = both(params, backend)
+    assert fluid_diff(params, ref, gpu) < tol
+    np.testing.assert_allclose(gpu.immersed_force, ref.immersed_force, rtol=10 * tol, atol=1e-18)
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_ibm_pull_is_bit_identical_to_the_oracle(d):
+    """Pull layout: same moment rows, same pairwise and bincount orders -> same bits."""
+    params = ibm_case(d, "float64")
+    ref, gpu = both(params, "hip-pull")
+    np.testing.assert_array_equal(gpu.populations()[:, params.fluid], ref.populations()[:, params.fluid])
+    np.testing.assert_array_equal(gpu.immersed_force, ref.immersed_force)
+
+
+def test_body_at_rest_in_fluid_at_rest_feels_no_force():
+    from lbm import SimulationParams
+    from lbm.geometry import circle
+    from lbm.ibm import ImmersedBoundary
+
+    params = SimulationParams(nx=24, ny=24, tau=0.8, dtype="float64",
+                              immersed=ImmersedBoundary.from_ball(circle(11.5, 11.5, 4.0)))
+    solver = make_solver(params, backend="hip-aa")
+    solver.run(10)
+    np.testing.assert_allclose(solver.immersed_force, 0.0, atol=1e-16)

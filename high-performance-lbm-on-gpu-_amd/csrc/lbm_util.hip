@@ -9,24 +9,16 @@
 namespace lbm {
 
 constexpr int COPY_THREADS = 256;
-constexpr int COPY_UNROLL = 4;
 
-// Each thread moves COPY_UNROLL 16-byte words, all loads issued before any store;
-// consecutive lanes touch consecutive words.
-__global__ __launch_bounds__(COPY_THREADS) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// One 16-byte word per lane, nontemporal load and store (the data is not reused):
+// the fastest of the copy shapes measured on MI355X (one, four or eight words per
+// lane, temporal or nontemporal), so the reported peak is the device's, not the probe's.
+__global__ __launch_bounds__(COPY_THREADS) void k_copy(const u32x4* __restrict__ src, u32x4* __restrict__ dst, size_t n)
 {
-    const size_t base = (size_t)blockIdx.x * COPY_THREADS * COPY_UNROLL + threadIdx.x;
-    uint4 v[COPY_UNROLL];
-#pragma unroll
-    for (int k = 0; k < COPY_UNROLL; ++k) {
-        const size_t i = base + (size_t)k * COPY_THREADS;
-        if (i < n) v[k] = src[i];
-    }
-#pragma unroll
-    for (int k = 0; k < COPY_UNROLL; ++k) {
-        const size_t i = base + (size_t)k * COPY_THREADS;
-        if (i < n) dst[i] = v[k];
-    }
+    const size_t i = (size_t)blockIdx.x * COPY_THREADS + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 }  // namespace lbm
@@ -43,10 +35,9 @@ LBM_API int lbm_device_copy(void* dst, const void* src, int64_t bytes, void* str
                 "copy buffers must be 16-byte aligned");
         const size_t n = (size_t)bytes / 16;
         if (n == 0) return;
-        const size_t per_block = (size_t)COPY_THREADS * COPY_UNROLL;
-        const size_t blocks = (n + per_block - 1) / per_block;
+        const size_t blocks = (n + COPY_THREADS - 1) / COPY_THREADS;
         require(blocks < (1ull << 31), "copy too large");
-        k_copy<<<(unsigned)blocks, COPY_THREADS, 0, as_stream(stream)>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+        k_copy<<<(unsigned)blocks, COPY_THREADS, 0, as_stream(stream)>>>(static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), n);
         check_launch("copy kernel");
     });
 }

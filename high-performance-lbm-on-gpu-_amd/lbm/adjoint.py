@@ -1,0 +1,349 @@
+"""Gradients of the time-averaged drag of an immersed body by a hand-written discrete adjoint.
+
+No counterpart in the reference (which has no autograd or checkpointing). The forward
+model is the HIP pull step of ``HipSolver`` with a dense per-node body force g(x) added
+to the uniform force:
+
+    per step t:  h^t, D^t = IBM(f^t; X)              (D^t: x component of the body force)
+                 f_s = stream + bounce-back + rules(f^t)
+                 f^{t+1} = collide(f_s, F0 + g + h^t)  at fluid nodes
+    loss L = mean of D^t over the last ``window`` steps.
+
+``DragAdjoint.backward`` returns dL/dg (d, *shape) and dL/dX (n_markers, d), both float64.
+The adjoint runs backwards over segments of ``checkpoint_every`` steps: each segment is
+recomputed forward from its checkpoint (states kept in HBM), then swept backwards with
+``lbm_adj_pull_step`` (transposed collision Jacobian + push), ``lbm_adj_rules`` (transposed
+link rules) and ``lbm_ibm_adjoint`` (reverse multi-direct forcing). dL/dX follows from the
+accumulated dL/dw and the derivative of the Roma kernel.
+
+Segments are captured once into HIP graphs (through ``torch.cuda.CUDAGraph``) and replayed,
+so the Python loop does not set the step rate. ``DragLoss`` wraps the whole thing as a
+``torch.autograd.Function``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from lbm.backends._native import FORCE_DENSE, pointer
+from lbm.backends.hip_solver import HipSolver
+from lbm.ibm import STENCIL, roma, roma_derivative
+from lbm.link_rules import compile_rules
+
+
+def transpose_rows(slots: np.ndarray, coeff: np.ndarray):
+    """CSR of the transposed affine rows: distinct slots, start offsets, (row, coeff) entries."""
+    rows, terms = np.nonzero(coeff)
+    s = slots[rows, terms]
+    order = np.lexsort((terms, rows, s))
+    s, rows, c = s[order], rows[order], coeff[rows[order], terms[order]]
+    unique, start = np.unique(s, return_index=True)
+    start = np.append(start, len(s)).astype(np.int32)
+    return unique.astype(np.int64), start, rows.astype(np.int32), c
+
+
+def marker_gradient(markers: np.ndarray, stencil_base: np.ndarray, grad_w: np.ndarray) -> np.ndarray:
+    """dL/dX from dL/dw, w_ls = prod_a phi(p_ls,a - X_l,a): dw/dX_l,a = -phi'(r_a) prod_{b!=a} phi(r_b)."""
+    n, d = markers.shape
+    offsets = np.stack(np.meshgrid(*(np.arange(STENCIL),) * d, indexing="ij"), axis=-1).reshape(-1, d)
+    r = (stencil_base[:, None, :] + offsets[None, :, :]) - markers[:, None, :]
+    phi = roma(r)
+    dphi = roma_derivative(r)
+    grad = np.zeros((n, d))
+    for a in range(d):
+        others = np.prod(np.delete(phi, a, axis=-1), axis=-1)
+        grad[:, a] = np.sum(grad_w * (-dphi[..., a] * others), axis=1)
+    return grad
+
+
+class DragAdjoint:
+    """Checkpointed forward/adjoint of the mean IBM drag (see module docstring).
+
+    Args:
+        params: ``SimulationParams`` with an immersed body (drag = its x body force).
+        steps: time steps of the forward run.
+        window: the loss averages the drag of steps ``steps - window`` .. ``steps - 1``.
+        checkpoint_every: segment length k (states of one segment are recomputed).
+        init: initial macroscopic fields (rho, ux, uy[, uz]); default rest.
+        graphs: capture segments into HIP graphs (needs ``steps % k == 0`` and even k).
+    """
+
+    def __init__(self, params, steps: int, window: int, checkpoint_every: int = 100, init=None,
+                 block=None, graphs: bool = True):
+        if params.immersed is None:
+            raise ValueError("the drag functional needs an immersed body (params.immersed)")
+        if not 0 < window <= steps:
+            raise ValueError("need 0 < window <= steps")
+        k = int(checkpoint_every)
+        if k <= 0:
+            raise ValueError("checkpoint_every must be positive")
+        solver = HipSolver(params, variant="pull", block=block)
+        self.solver = s = solver
+        self.params = p = params
+        self.steps, self.window, self.k = steps, window, k
+        if k % 2:
+            raise ValueError("checkpoint_every must be even (the adjoint buffers alternate per step)")
+        self.graphs = graphs and steps % k == 0
+        torch = s._torch
+        self._torch = torch
+        dev, tdt = s.device, s._torch_dtype
+        d, q, n = p.d, p.lattice.q, s._n
+        self.init = init if init is not None else (1.0,)
+        s.set_body_force_field(np.zeros((d, *p.shape)))
+        s._force_mode = FORCE_DENSE
+        self.n_segments = -(-steps // k)
+        self._ckpt = torch.zeros((self.n_segments, q, n), dtype=tdt, device=dev)
+        self._seg = torch.zeros((k + 1, q, n), dtype=tdt, device=dev)
+        self._field_hist = torch.zeros((k, d, n), dtype=tdt, device=dev)
+        self._start = torch.zeros((q, n), dtype=tdt, device=dev)
+        self._drag = torch.zeros((steps, d), dtype=tdt, device=dev)
+        self._drag_stage = torch.zeros((k, d), dtype=tdt, device=dev)
+        self._bf_scratch = torch.zeros(d, dtype=tdt, device=dev)
+        weights = np.zeros((steps, d))
+        weights[steps - window:, 0] = 1.0 / window
+        self._body_weight = torch.as_tensor(weights, device=dev)
+        self._weight_stage = torch.zeros((k, d), dtype=torch.float64, device=dev)
+        self._lam = torch.zeros((2, q, n), dtype=tdt, device=dev)
+        self._grad_body = torch.zeros((d, n), dtype=torch.float64, device=dev)
+        self._nu = torch.zeros((d, n), dtype=tdt, device=dev)
+        self._setup_ibm_adjoint()
+        self._setup_rule_adjoint()
+        self._graphs = {}
+        self.set_initial_state(*self.init)
+
+    # ------------------------------------------------------------------ set-up
+
+    def _setup_ibm_adjoint(self) -> None:
+        import ctypes
+
+        s, torch, dev = self.solver, self._torch, self.solver.device
+        ibm = s._ibm
+        mp = s._ibm_pack("pull")
+        compiled = ibm.compiled("pull")
+        tslot, tstart, trow, tcoeff = transpose_rows(np.asarray(compiled.slots), np.asarray(compiled.coeff))
+        if s.params.link_rules is not None and len(s.params.link_rules):
+            targets = compile_rules(s.params.lattice, s.params.link_rules, "pull", s.params.solid,
+                                    s.params.wall_velocity).target
+            if np.intersect1d(targets, tslot).size:
+                raise NotImplementedError("IBM moment rows read a link-rule target slot; move the body away")
+        self._mp = mp
+        self._tslot = torch.as_tensor(tslot, device=dev)
+        self._tstart = torch.as_tensor(tstart, device=dev)
+        self._trow = torch.as_tensor(trow, device=dev)
+        self._tcoeff = torch.as_tensor(tcoeff.astype(s._np_dtype), device=dev)
+        self._grad_w = torch.zeros((ibm.n_markers, STENCIL**s.params.d), dtype=torch.float64, device=dev)
+        size = ctypes.c_int64()
+        s._lib.call("lbm_ibm_adjoint_scratch_size", s._lattice_code, s._dtype_code, ibm.n_nodes, ibm.n_markers,
+                    ibm.iterations, ctypes.byref(size))
+        self._ibm_adj_scratch = torch.zeros(size.value // s._np_dtype.itemsize + 1, dtype=s._torch_dtype, device=dev)
+
+    def _setup_rule_adjoint(self) -> None:
+        s, torch, dev = self.solver, self._torch, self.solver.device
+        pack = s._rule_passes.get("pull")
+        self._rules = None
+        if pack is None:
+            return
+        compiled = pack.source
+        uslot, ustart, urule, ucoeff = transpose_rows(np.asarray(compiled.slots), np.asarray(compiled.coeff))
+        self._rules = {
+            "target": pack.target,
+            "n": pack.n,
+            "uslot": torch.as_tensor(uslot, device=dev),
+            "ustart": torch.as_tensor(ustart, device=dev),
+            "urule": torch.as_tensor(urule, device=dev),
+            "ucoeff": torch.as_tensor(ucoeff.astype(s._np_dtype), device=dev),
+            "nu": len(uslot),
+            "scratch": torch.zeros(max(1, pack.n), dtype=s._torch_dtype, device=dev),
+        }
+
+    # ------------------------------------------------------------------ controls
+
+    def set_initial_state(self, *fields) -> None:
+        self.solver.initialize(*fields)
+        self._start.copy_(self.solver._f)
+
+    def set_body_force(self, field) -> None:
+        """Dense body force g, shape (d, *shape) (NumPy or torch, any device)."""
+        t = self._torch.as_tensor(field, dtype=self.solver._torch_dtype)
+        self.solver._body.copy_(t.reshape(self.solver._body.shape))
+
+    def set_markers(self, markers) -> None:
+        """New marker positions (same count); the stencil node set must stay the same."""
+        s = self.solver
+        body = s.params.immersed.with_markers(np.asarray(markers, dtype=np.float64))
+        old = s._ibm
+        stencil, weights = body.stencil(s.params.shape)
+        nodes, inverse = np.unique(stencil, return_inverse=True)
+        if not (np.array_equal(nodes, old.nodes) and np.array_equal(inverse.reshape(stencil.shape), old.inverse)):
+            raise ValueError("the markers moved across a grid line: the stencil changed")
+        old.body = body
+        old.weights = weights
+        s._ibm_weights.copy_(self._torch.as_tensor(weights.astype(s._np_dtype)))
+
+    @property
+    def markers(self) -> np.ndarray:
+        return np.array(self.solver._ibm.body.markers)
+
+    # ------------------------------------------------------------------ launches
+
+    def _ibm_forward(self, f, field, body_out) -> None:
+        s, ibm, mp = self.solver, self.solver._ibm, self._mp
+        fx, fy, fz = s._force3
+        s._call("lbm_ibm_force", s._lattice_code, s._dtype_code, pointer(f), pointer(mp.slot), pointer(mp.coeff),
+                pointer(mp.constant), mp.m, pointer(s._ibm_nodes), ibm.n_nodes, pointer(s._ibm_inverse),
+                pointer(s._ibm_weights), pointer(s._ibm_areas), ibm.n_markers, pointer(s._ibm_start),
+                pointer(s._ibm_entry), ibm.iterations, fx, fy, fz, s._n, pointer(field), body_out,
+                pointer(s._ibm_scratch), s._stream)
+
+    def _rules_forward(self, f) -> None:
+        s = self.solver
+        pack = s._rule_passes.get("pull")
+        if pack is not None:
+            s._call("lbm_rules_apply", s._dtype_code, pointer(f), pointer(pack.slot), pointer(pack.coeff),
+                    pointer(pack.constant), pointer(pack.target), pack.n, pack.m, pointer(pack.scratch), s._stream)
+
+    def _pull(self, src, dst, field) -> None:
+        s = self.solver
+        s._call("lbm_pull_step", s._lattice_code, s._dtype_code, pointer(src), pointer(dst), pointer(s._kind),
+                pointer(s._wall), pointer(field), pointer(s._body), *s._step_args())
+
+    def _forward_core(self, length: int, drag) -> None:
+        """``length`` steps from solver._f (ending there: length is even or a copy closes it)."""
+        s = self.solver
+        a, b = s._f, s._f_next
+        for j in range(length):
+            cur, nxt = (a, b) if j % 2 == 0 else (b, a)
+            self._ibm_forward(cur, s._field, drag[j].data_ptr())
+            self._rules_forward(cur)
+            self._pull(cur, nxt, s._field)
+        if length % 2:
+            a.copy_(b)
+
+    def _backward_core(self, length: int, weights) -> None:
+        """Recompute ``length`` steps from seg[0], then sweep them backwards.
+
+        The adjoint buffers alternate with the parity of the step inside the segment;
+        segments start at even steps (k is even), so the parity is the global one.
+        """
+        s = self.solver
+        L, D = s._lattice_code, s._dtype_code
+        for j in range(length):
+            cur = self._seg[j]
+            self._ibm_forward(cur, self._field_hist[j], self._bf_scratch.data_ptr())
+            self._rules_forward(cur)
+            self._pull(cur, self._seg[j + 1], self._field_hist[j])
+        ibm, mp = s._ibm, self._mp
+        fx, fy, fz = s._force3
+        wp, wm = s._rates
+        bx, by = self.adjoint_block
+        for j in reversed(range(length)):
+            lam_next, lam_out = self._lam[(j + 1) % 2], self._lam[j % 2]
+            s._call("lbm_adj_pull_step", L, D, pointer(lam_next), pointer(lam_out), pointer(self._seg[j]),
+                    pointer(s._kind), pointer(s._wall), pointer(self._field_hist[j]), pointer(s._body),
+                    s.params.nx, s.params.ny, s._nz, wp, wm, fx, fy, fz, FORCE_DENSE, s._mask_mode,
+                    pointer(self._grad_body), pointer(self._nu), bx, by, s._stream)
+            if self._rules is not None:
+                r = self._rules
+                s._call("lbm_adj_rules", D, pointer(lam_out), pointer(r["target"]), r["n"], pointer(r["uslot"]),
+                        pointer(r["ustart"]), pointer(r["urule"]), pointer(r["ucoeff"]), r["nu"],
+                        pointer(r["scratch"]), s._stream)
+            s._call("lbm_ibm_adjoint", L, D, pointer(self._seg[j]), pointer(mp.slot), pointer(mp.coeff),
+                    pointer(mp.constant), mp.m, pointer(s._ibm_nodes), ibm.n_nodes, pointer(s._ibm_inverse),
+                    pointer(s._ibm_weights), pointer(s._ibm_areas), ibm.n_markers, pointer(s._ibm_start),
+                    pointer(s._ibm_entry), ibm.iterations, fx, fy, fz, s._n, pointer(self._nu),
+                    weights[j].data_ptr(), pointer(lam_out), pointer(self._tslot), pointer(self._tstart),
+                    pointer(self._trow), pointer(self._tcoeff), len(self._tslot), pointer(self._grad_w),
+                    pointer(self._ibm_adj_scratch), s._stream)
+
+    @property
+    def adjoint_block(self) -> tuple[int, int]:
+        bx = min(256, 64 * -(-self.params.nx // 64))
+        return bx, max(1, min(self.params.ny, 256 // bx))
+
+    # ------------------------------------------------------------------ graphs
+
+    def _replay(self, key, launch) -> None:
+        """Run ``launch`` eagerly the first time, then capture it and replay the graph."""
+        if not self.graphs:
+            launch()
+            return
+        torch = self._torch
+        graph = self._graphs.get(key)
+        if graph is None:
+            launch()  # eager warm-up: loads every kernel before capture
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                launch()
+            self._graphs[key] = graph
+            return
+        graph.replay()
+
+    # ------------------------------------------------------------------ public API
+
+    def run_forward(self, store: bool = True) -> None:
+        """The forward run from the initial state: drag history and (if ``store``) checkpoints.
+
+        Checkpoint copies and the drag history go through fixed staging buffers outside the
+        captured segment, so one graph serves every segment."""
+        self.solver._f.copy_(self._start)
+        for seg in range(self.n_segments):
+            t0 = seg * self.k
+            length = min(self.k, self.steps - t0)
+            if store:
+                self._ckpt[seg].copy_(self.solver._f)
+            self._replay(("fwd", length), lambda: self._forward_core(length, self._drag_stage))
+            self._drag[t0 : t0 + length].copy_(self._drag_stage[:length])
+
+    def forward(self) -> float:
+        """Run forward with checkpoints; returns the loss (host float64)."""
+        self.run_forward(store=True)
+        drag = self._drag[self.steps - self.window :, 0].double()
+        return float(drag.mean().item())
+
+    @property
+    def drag_history(self) -> np.ndarray:
+        return self._drag[:, 0].double().cpu().numpy()
+
+    def backward(self):
+        """dL/dg (d, *shape) and dL/dX (n_markers, d), float64 host arrays.
+
+        Needs the checkpoints of the last ``forward()``."""
+        self._lam.zero_()
+        self._grad_body.zero_()
+        self._grad_w.zero_()
+        self._nu.zero_()
+        for seg in reversed(range(self.n_segments)):
+            t0 = seg * self.k
+            length = min(self.k, self.steps - t0)
+            self._seg[0].copy_(self._ckpt[seg])
+            self._weight_stage[:length].copy_(self._body_weight[t0 : t0 + length])
+            if length % 2 and self.n_segments > 1:
+                raise ValueError("segments after the first must have an even length")
+            self._replay(("bwd", length), lambda: self._backward_core(length, self._weight_stage))
+        self._torch.cuda.synchronize()
+        p = self.params
+        grad_body = self._grad_body.cpu().numpy().reshape(p.d, *p.shape)
+        ibm = self.solver._ibm
+        grad_x = marker_gradient(ibm.body.markers, ibm.body.stencil_base(), self._grad_w.cpu().numpy())
+        return grad_body, grad_x
+
+
+def drag_loss(problem: DragAdjoint, body_field, markers):
+    """Differentiable mean drag: a ``torch.autograd.Function`` over (body_field, markers)."""
+    import torch
+
+    class DragLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, g, x):
+            problem.set_body_force(g.detach())
+            problem.set_markers(x.detach().cpu().numpy())
+            return torch.tensor(problem.forward(), dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            gb, gx = problem.backward()
+            scale = grad_out.detach().cpu().double()
+            return (torch.as_tensor(gb) * scale).to(body_field.device), (torch.as_tensor(gx) * scale).to(markers.device)
+
+    return DragLoss.apply(body_field, markers)

@@ -163,6 +163,43 @@ LBM_API int lbm_ibm_force(int lattice, int dtype, const void* f, const int64_t* 
  * measurement of benchmarks/bench_mlups.py:66-79. */
 LBM_API int lbm_device_copy(void* dst, const void* src, int64_t bytes, void* stream);
 
+/* Adjoint of one pull step (no counterpart in the reference, which has no adjoint):
+ * lam_out = (d f^{t+1} / d f^t)^T lam_next for the collision and streaming part. f is the
+ * state the forward pull step read (link-rule values already in place), force arguments as
+ * for lbm_pull_step. The adjoint populations are pushed to x - c_i (bounce-back links to the
+ * node's own opposite slot); lam_out must differ from lam_next and be zero at solid nodes.
+ * grad_body[d][n] (double, optional) += dL/dF; nu_forced (dtype, optional) receives dL/dF at
+ * IBM-forced nodes. bx * by <= 256. */
+LBM_API int lbm_adj_pull_step(int lattice, int dtype, const void* lam_next, void* lam_out, const void* f,
+                              const uint8_t* kind, const void* wall_u, const void* force_field,
+                              const void* body_field, int nx, int ny, int nz, double omega_plus,
+                              double omega_minus, double fx, double fy, double fz, int force_mode,
+                              int mask_mode, double* grad_body, void* nu_forced, int bx, int by, void* stream);
+
+/* Transposed link-rule pre-pass (no counterpart in the reference): a_r = lam[target_r],
+ * lam[target_r] = 0, then lam[uslot[u]] += sum_{e in [ustart[u], ustart[u+1])} ucoeff[e] a[urule[e]]
+ * over the distinct term slots. One workgroup; scratch holds n_rules values of dtype. */
+LBM_API int lbm_adj_rules(int dtype, void* lam, const int64_t* target, int n_rules, const int64_t* uslot,
+                          const int* ustart, const int* urule, const void* ucoeff, int n_unique,
+                          void* scratch, void* stream);
+
+/* Scratch bytes of lbm_ibm_adjoint. Host only. */
+LBM_API int lbm_ibm_adjoint_scratch_size(int lattice, int dtype, int n_nodes, int n_markers, int iterations,
+                                         int64_t* bytes);
+
+/* Adjoint of lbm_ibm_force (no counterpart in the reference): recomputes the multi-direct
+ * iterations from f and runs them backwards with seeds nu (dL/dh at forced nodes, d x n_total)
+ * and body_weight[d] (dL/d body force, double). Adds the transposed moment rows into lam
+ * (distinct slots tslot[n_tslots] with (trow, tcoeff) lists in tstart order) and accumulates
+ * dL/dw into grad_w[n_markers][3^d] (double). One workgroup. */
+LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
+                            const void* constant, int m, const int64_t* nodes, int n_nodes, const int* inverse,
+                            const void* weights, const void* areas, int n_markers, const int* spread_start,
+                            const int* spread_entry, int iterations, double fx, double fy, double fz,
+                            int64_t n_total, const void* nu, const double* body_weight, void* lam,
+                            const int64_t* tslot, const int* tstart, const int* trow, const void* tcoeff,
+                            int n_tslots, double* grad_w, void* scratch, void* stream);
+
 /* Per-kernel resources of the compiled step kernels for (lattice, dtype):
  * fills up to max_kernels entries of {registers, scratch (private) bytes per lane,
  * static LDS bytes, max threads per block} and writes their names (';'-separated)

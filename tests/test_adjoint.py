@@ -1,0 +1,217 @@
+"""The hand-written adjoint: gradients of the mean IBM drag w.r.t. a dense body-force field and
+the marker positions, checked against float64 torch autograd through a torch re-expression of
+the oracle step and against float64 central finite differences of the forward run.
+
+The reference has no adjoint; the checks pin it to the mathematics (autograd of the same
+step, finite differences) rather than to reference outputs.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from lbm import SimulationParams
+from lbm.boundaries import channel_walls
+from lbm.geometry import circle
+from lbm.ibm import ImmersedBoundary, roma, roma_derivative
+from lbm.lattice import D2Q9
+from lbm.open_boundaries import open_channel
+from oracle import ReferenceSolver
+from oracle.torch_step import TorchDragModel
+
+STEPS, WINDOW, K = 60, 20, 20
+
+
+def case(dtype="float64", rules=False, nx=64, ny=32):
+    solid = channel_walls(nx, ny)
+    link_rules = None
+    force = (1e-5, 0.0)
+    if rules:
+        y = np.arange(ny) - 0.5
+        h = ny - 2
+        inflow = np.zeros((2, ny))
+        inflow[0] = np.where((y > 0) & (y < h), 4 * 0.04 * y * (h - y) / h**2, 0.0)
+        solid, link_rules = open_channel(D2Q9, solid, inflow)
+        force = (0.0, 0.0)
+    body = ImmersedBoundary.from_ball(circle(0.3 * nx + 0.27, ny / 2 + 0.31, 4.0), spacing=1.0, iterations=3)
+    return SimulationParams(nx=nx, ny=ny, tau=0.6, collision="trt", force=force, solid=solid, dtype=dtype,
+                            immersed=body, link_rules=link_rules)
+
+
+def init_fields(params):
+    return (1.0, np.full(params.shape, 0.03), np.zeros(params.shape))
+
+
+def oracle_initial_state(params):
+    s = ReferenceSolver(params)
+    s.initialize(*init_fields(params))
+    return s.populations()
+
+
+# ---------------------------------------------------------------- CPU: the checker itself
+
+
+def test_roma_derivative_matches_finite_differences():
+    r = np.linspace(-1.7, 1.7, 341)
+    # phi is C1: its second derivative jumps at |r| = 1/2 and 3/2, where central
+    # differences are only first-order accurate, so those points are left out.
+    r = r[(np.abs(np.abs(r) - 0.5) > 1e-3) & (np.abs(np.abs(r) - 1.5) > 1e-3)]
+    eps = 1e-6
+    fd = (roma(r + eps) - roma(r - eps)) / (2 * eps)
+    np.testing.assert_allclose(roma_derivative(r), fd, atol=1e-7)
+
+
+def test_marker_gradient_is_the_chain_rule_of_the_roma_weights():
+    from lbm.adjoint import marker_gradient
+
+    body = ImmersedBoundary.from_ball(circle(10.2, 9.7, 4.0), spacing=1.0)
+    grad_w = np.random.default_rng(0).standard_normal((len(body.markers), 9))
+    base = body.stencil_base()
+    g = marker_gradient(body.markers, base, grad_w)
+    eps = 1e-6
+    for l, a in [(0, 0), (3, 1), (7, 0)]:
+        plus, minus = body.markers.copy(), body.markers.copy()
+        plus[l, a] += eps
+        minus[l, a] -= eps
+        wp = body.with_markers(plus).stencil((20, 21))[1]
+        wm = body.with_markers(minus).stencil((20, 21))[1]
+        fd = np.sum(grad_w * (wp - wm)) / (2 * eps)
+        assert g[l, a] == pytest.approx(fd, rel=1e-6, abs=1e-10)
+
+
+def test_transposed_rows_are_the_transpose():
+    from lbm.adjoint import transpose_rows
+
+    rng = np.random.default_rng(1)
+    slots = rng.integers(0, 30, size=(12, 4))
+    coeff = rng.standard_normal((12, 4)) * (rng.random((12, 4)) > 0.3)
+    uslot, start, row, c = transpose_rows(slots, coeff)
+    dense = np.zeros((12, 30))
+    for r in range(12):
+        for t in range(4):
+            dense[r, slots[r, t]] += coeff[r, t]
+    lam = rng.standard_normal(12)
+    expect = dense.T @ lam
+    got = np.zeros(30)
+    for u, s in enumerate(uslot):
+        got[s] = sum(c[e] * lam[row[e]] for e in range(start[u], start[u + 1]))
+    np.testing.assert_allclose(got, expect, atol=1e-14)
+
+
+@pytest.mark.parametrize("rules", [False, True], ids=["walls", "zou-he"])
+def test_torch_model_reproduces_the_oracle_drag(rules):
+    params = case(rules=rules)
+    model = TorchDragModel(params, steps=30, window=30, init_state=oracle_initial_state(params))
+    g = torch.zeros((2, *params.shape), dtype=torch.float64)
+    x = torch.as_tensor(params.immersed.markers)
+    loss = float(model.loss(g, x))
+    ref = ReferenceSolver(params)
+    ref.initialize(*init_fields(params))
+    drags = []
+    for _ in range(30):
+        ref.step(1)
+        drags.append(ref.immersed_force[0])
+    assert loss == pytest.approx(np.mean(drags), rel=1e-12)
+
+
+# ---------------------------------------------------------------- GPU
+
+
+def make_problem(params, graphs=True):
+    from lbm.adjoint import DragAdjoint
+
+    return DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params),
+                       graphs=graphs)
+
+
+def random_body_field(params, seed=3, scale=1e-6):
+    return np.random.default_rng(seed).standard_normal((params.d, *params.shape)) * scale * params.fluid
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rules", [False, True], ids=["walls", "zou-he"])
+def test_gpu_forward_drag_matches_the_oracle(rules):
+    params = case(rules=rules)
+    problem = make_problem(params)
+    problem.forward()
+    ref = ReferenceSolver(params)
+    ref.initialize(*init_fields(params))
+    drags = []
+    for _ in range(STEPS):
+        ref.step(1)
+        drags.append(ref.immersed_force[0])
+    np.testing.assert_allclose(problem.drag_history, drags, rtol=1e-13, atol=1e-18)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rules", [False, True], ids=["walls", "zou-he"])
+def test_adjoint_matches_float64_torch_autograd(rules):
+    params = case(rules=rules)
+    g0 = random_body_field(params)
+    problem = make_problem(params)
+    problem.set_body_force(g0)
+    loss = problem.forward()
+    grad_g, grad_x = problem.backward()
+    model = TorchDragModel(params, STEPS, WINDOW, oracle_initial_state(params))
+    g = torch.as_tensor(g0).requires_grad_(True)
+    x = torch.as_tensor(params.immersed.markers.copy()).requires_grad_(True)
+    ref_loss = model.loss(g, x)
+    ref_loss.backward()
+    assert loss == pytest.approx(float(ref_loss), rel=1e-12)
+    scale_g = np.abs(g.grad.numpy()).max()
+    np.testing.assert_allclose(grad_g, g.grad.numpy(), rtol=0, atol=1e-9 * scale_g)
+    np.testing.assert_allclose(grad_x, x.grad.numpy(), rtol=1e-8, atol=1e-9 * np.abs(x.grad.numpy()).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_adjoint_matches_central_finite_differences(dtype):
+    """Directional derivatives along a random g-direction and single marker coordinates,
+    against float64 central differences of the forward run."""
+    params = case(dtype=dtype)
+    params64 = case(dtype="float64")
+    problem = make_problem(params)
+    fd_problem = make_problem(params64)
+    g0 = random_body_field(params)
+    problem.set_body_force(g0)
+    problem.forward()
+    grad_g, grad_x = problem.backward()
+    direction = random_body_field(params, seed=11, scale=1.0)
+    eps = 1e-6
+
+    def loss_at(g, markers):
+        fd_problem.set_body_force(g)
+        fd_problem.set_markers(markers)
+        return fd_problem.forward()
+
+    x0 = params.immersed.markers.copy()
+    fd = (loss_at(g0 + eps * direction, x0) - loss_at(g0 - eps * direction, x0)) / (2 * eps)
+    adj = float(np.sum(grad_g * direction))
+    assert adj == pytest.approx(fd, rel=1e-4)
+    for l, a in [(0, 0), (5, 1), (11, 0)]:
+        h = 1e-5
+        xp, xm = x0.copy(), x0.copy()
+        xp[l, a] += h
+        xm[l, a] -= h
+        fd_x = (loss_at(g0, xp) - loss_at(g0, xm)) / (2 * h)
+        assert grad_x[l, a] == pytest.approx(fd_x, rel=1e-4, abs=1e-4 * np.abs(grad_x).max())
+
+
+@pytest.mark.gpu
+def test_graph_replay_equals_eager_launches():
+    params = case()
+    a, b = make_problem(params, graphs=True), make_problem(params, graphs=False)
+    g0 = random_body_field(params)
+    results = []
+    for problem in (a, b):
+        problem.set_body_force(g0)
+        loss = problem.forward()
+        results.append((loss, *problem.backward()))
+    # the captured path replays the same kernels: identical bits
+    assert results[0][0] == results[1][0]
+    np.testing.assert_array_equal(results[0][1], results[1][1])
+    np.testing.assert_array_equal(results[0][2], results[1][2])
+    # and a second replay of the captured graphs reproduces the first
+    a.set_body_force(g0)
+    assert a.forward() == results[0][0]
+    np.testing.assert_array_equal(a.backward()[0], results[0][1])

@@ -96,7 +96,7 @@ def kernel_times_ms(torch, solver, pairs: int = 20) -> dict[str, float]:
     return {"aa_odd": float(np.mean(odd)), "aa_even": float(np.mean(even))}
 
 
-def cpu_baseline(n: int = 128, steps: int = 8) -> dict:
+def cpu_baseline(n: int = 128, steps: int = 32) -> dict:
     """The NumPy oracle on a bounded sample: D3Q19 fp32 TRT + Guo ABC box, n^3, ``steps`` steps."""
     from oracle import ReferenceSolver
 
@@ -118,6 +118,43 @@ def cpu_baseline(n: int = 128, steps: int = 8) -> dict:
     }
 
 
+def adjoint_ratio(torch, steps: int = 4000, window: int = 2000, k: int = 100) -> dict:
+    """Forward-only vs checkpointed forward+adjoint rate on the differentiable configuration:
+    D2Q9 fp32 channel 512x128, Zou-He inlet/outlet, IBM cylinder R = 12 (markers 1 lattice
+    unit apart, 3 multi-direct passes), Re = 100, body-force field N(0, 1e-7), loss = mean
+    drag over the last ``window`` of ``steps`` steps, checkpoints every ``k`` steps."""
+    from lbm.adjoint import DragAdjoint
+    from lbm.cases.ibm_channel import ibm_channel_params, initial_fields
+
+    params = ibm_channel_params()
+    problem = DragAdjoint(params, steps=steps, window=window, checkpoint_every=k, init=initial_fields(params))
+    g = np.random.default_rng(0).standard_normal((2, *params.shape)) * 1e-7 * params.fluid
+    problem.set_body_force(g)
+    problem.forward()
+    problem.backward()  # first calls capture the graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    problem.run_forward(store=False)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = problem.forward()
+    grad_g, grad_x = problem.backward()
+    torch.cuda.synchronize()
+    t_both = time.perf_counter() - t0
+    nodes = params.nx * params.ny
+    return {
+        "workload": "D2Q9 fp32 IBM-cylinder channel 512x128, Re 100, 4000 steps, loss = mean drag of the "
+        "last 2000, checkpoints every 100 steps",
+        "forward_mlups": round(nodes * steps / t_fwd / 1e6, 1),
+        "forward_adjoint_mlups": round(nodes * steps / t_both / 1e6, 1),
+        "ratio": round(t_fwd / t_both, 4),
+        "loss": loss,
+        "grad_body_norm": float(np.linalg.norm(grad_g)),
+        "grad_markers_norm": float(np.linalg.norm(grad_x)),
+    }
+
+
 def traffic_from_profile():
     if TRAFFIC_FILE.exists():
         data = json.loads(TRAFFIC_FILE.read_text())
@@ -134,6 +171,7 @@ def main(argv=None) -> int:
     parser.add_argument("--variant", default="aa", choices=("aa", "pull"))
     parser.add_argument("--block", type=int, nargs=2, default=None)
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-adjoint", action="store_true")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (got --gpus {args.gpus})", file=sys.stderr)
@@ -190,6 +228,11 @@ def main(argv=None) -> int:
     print(f"# per-kernel ms: {json.dumps({k: round(v, 4) for k, v in (times.items() if args.variant == 'aa' else {})})}",
           flush=True)
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_ratio(torch)
+        print(f"# adjoint: {json.dumps(adjoint)}", flush=True)
+
     baseline = None
     if not args.no_cpu_baseline:
         baseline = cpu_baseline()
@@ -227,6 +270,7 @@ def main(argv=None) -> int:
             "traffic": traffic_from_profile(),
         },
         "cpu_baseline": baseline,
+        "adjoint": adjoint,
     }
     print(json.dumps(record), flush=True)
     return 0

@@ -248,12 +248,28 @@ __global__ __launch_bounds__(IBM_ADJ_THREADS) void k_ibm_adj(
     const int* __restrict__ spread_entry, int iters, T hfx, T hfy, T hfz, uint32_t n_total,
     const T* __restrict__ nu, const double* __restrict__ body_weight, T* __restrict__ lam,
     const int64_t* __restrict__ tslot, const int* __restrict__ tstart, const int* __restrict__ trow,
-    const T* __restrict__ tcoeff, int n_tslots, double* __restrict__ grad_w, T* __restrict__ scratch)
+    const T* __restrict__ tcoeff, int n_tslots, double* __restrict__ grad_w, T* __restrict__ scratch,
+    const int64_t* __restrict__ rtarget, int n_rules, const int64_t* __restrict__ ruslot,
+    const int* __restrict__ rustart, const int* __restrict__ rurule, const T* __restrict__ rucoeff, int n_runique,
+    T* __restrict__ rule_a)
 {
     constexpr int S = D == 2 ? 9 : 27;
     IbmAdjScratch<T> s;
     ibm_adj_layout<T>(scratch, D, nn, nm, iters, &s);
     const int tid = threadIdx.x;
+    // Transposed link rules first (they act on slots the IBM rows never read).
+    if (n_rules > 0) {
+        for (int r = tid; r < n_rules; r += IBM_ADJ_THREADS) rule_a[r] = lam[rtarget[r]];
+        __syncthreads();
+        for (int r = tid; r < n_rules; r += IBM_ADJ_THREADS) lam[rtarget[r]] = T(0);
+        __syncthreads();
+        for (int u = tid; u < n_runique; u += IBM_ADJ_THREADS) {
+            T acc = T(0);
+            for (int e = rustart[u]; e < rustart[u + 1]; ++e) acc += rucoeff[e] * rule_a[rurule[e]];
+            lam[ruslot[u]] += acc;
+        }
+        __syncthreads();
+    }
     const int rows = (D + 1) * nn;
     const T hf[3] = {hfx, hfy, hfz};
     // ---- forward recompute (same expressions as k_ibm; summation order is irrelevant here)
@@ -482,9 +498,13 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
                             const int* spread_entry, int iterations, double fx, double fy, double fz, int64_t n_total,
                             const void* nu, const double* body_weight, void* lam, const int64_t* tslot,
                             const int* tstart, const int* trow, const void* tcoeff, int n_tslots, double* grad_w,
-                            void* scratch, void* stream)
+                            void* scratch, const int64_t* rule_target, int n_rules, const int64_t* rule_uslot,
+                            const int* rule_ustart, const int* rule_urule, const void* rule_ucoeff, int n_rule_unique,
+                            void* rule_scratch, void* stream)
 {
     return guarded("lbm_ibm_adjoint", [&] {
+        require(n_rules == 0 || (rule_target && rule_uslot && rule_ustart && rule_urule && rule_ucoeff && rule_scratch),
+                "null rule pointer");
         require(n_nodes > 0 && n_markers > 0 && iterations >= 1 && m >= 1, "bad IBM sizes");
         require(n_total > 0 && n_total < (1ll << 31), "node count out of range");
         require(f && slot && coeff && constant && nodes && inverse && weights && areas && spread_start &&
@@ -498,7 +518,9 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
                 nodes, n_nodes, inverse, static_cast<const T*>(weights), static_cast<const T*>(areas), n_markers,
                 spread_start, spread_entry, iterations, T(0.5 * fx), T(0.5 * fy), T(0.5 * fz), (uint32_t)n_total,
                 static_cast<const T*>(nu), body_weight, static_cast<T*>(lam), tslot, tstart, trow,
-                static_cast<const T*>(tcoeff), n_tslots, grad_w, static_cast<T*>(scratch));
+                static_cast<const T*>(tcoeff), n_tslots, grad_w, static_cast<T*>(scratch), rule_target, n_rules,
+                rule_uslot, rule_ustart, rule_urule, static_cast<const T*>(rule_ucoeff), n_rule_unique,
+                static_cast<T*>(rule_scratch));
             check_launch("IBM adjoint");
         });
     });

@@ -66,10 +66,11 @@ class DragAdjoint:
         checkpoint_every: segment length k (states of one segment are recomputed).
         init: initial macroscopic fields (rho, ux, uy[, uz]); default rest.
         graphs: capture segments into HIP graphs (needs ``steps % k == 0`` and even k).
+        field_store_bytes: HBM budget for the per-step IBM fields of the forward run.
     """
 
     def __init__(self, params, steps: int, window: int, checkpoint_every: int = 100, init=None,
-                 block=None, graphs: bool = True):
+                 block=None, graphs: bool = True, field_store_bytes: int = 16 * 2**30):
         if params.immersed is None:
             raise ValueError("the drag functional needs an immersed body (params.immersed)")
         if not 0 < window <= steps:
@@ -94,7 +95,12 @@ class DragAdjoint:
         self.n_segments = -(-steps // k)
         self._ckpt = torch.zeros((self.n_segments, q, n), dtype=tdt, device=dev)
         self._seg = torch.zeros((k + 1, q, n), dtype=tdt, device=dev)
-        self._field_hist = torch.zeros((k, d, n), dtype=tdt, device=dev)
+        # Each forward step records its IBM force field into its own pre-zeroed dense buffer
+        # (the IBM kernel writes only the stencil nodes), so the segment recompute reuses it
+        # instead of re-running the IBM; above ``field_store_bytes`` the recompute re-runs it.
+        field_bytes = steps * d * n * s._np_dtype.itemsize
+        self.store_fields = field_bytes <= field_store_bytes
+        self._field_hist = torch.zeros((steps if self.store_fields else k, d, n), dtype=tdt, device=dev)
         self._start = torch.zeros((q, n), dtype=tdt, device=dev)
         self._drag = torch.zeros((steps, d), dtype=tdt, device=dev)
         self._drag_stage = torch.zeros((k, d), dtype=tdt, device=dev)
@@ -207,31 +213,38 @@ class DragAdjoint:
         s._call("lbm_pull_step", s._lattice_code, s._dtype_code, pointer(src), pointer(dst), pointer(s._kind),
                 pointer(s._wall), pointer(field), pointer(s._body), *s._step_args())
 
-    def _forward_core(self, length: int, drag) -> None:
-        """``length`` steps from solver._f (ending there: length is even or a copy closes it)."""
+    def _forward_core(self, length: int, drag, fields) -> None:
+        """``length`` steps from solver._f (ending there: length is even or a copy closes it).
+
+        ``fields`` is the per-step field buffer (j-th step writes fields[j]) or None to use
+        the solver's single field buffer."""
         s = self.solver
         a, b = s._f, s._f_next
         for j in range(length):
             cur, nxt = (a, b) if j % 2 == 0 else (b, a)
-            self._ibm_forward(cur, s._field, drag[j].data_ptr())
+            field = s._field if fields is None else fields[j]
+            self._ibm_forward(cur, field, drag[j].data_ptr())
             self._rules_forward(cur)
-            self._pull(cur, nxt, s._field)
+            self._pull(cur, nxt, field)
         if length % 2:
             a.copy_(b)
 
-    def _backward_core(self, length: int, weights) -> None:
+    def _backward_core(self, length: int, weights, fields) -> None:
         """Recompute ``length`` steps from seg[0], then sweep them backwards.
 
-        The adjoint buffers alternate with the parity of the step inside the segment;
-        segments start at even steps (k is even), so the parity is the global one.
+        ``fields[j]`` is the IBM field of the j-th step of the segment; with recorded
+        fields the recompute skips the IBM. The adjoint buffers alternate with the parity
+        of the step inside the segment; segments start at even steps (k is even), so the
+        parity is the global one.
         """
         s = self.solver
         L, D = s._lattice_code, s._dtype_code
         for j in range(length):
             cur = self._seg[j]
-            self._ibm_forward(cur, self._field_hist[j], self._bf_scratch.data_ptr())
+            if not self.store_fields:
+                self._ibm_forward(cur, fields[j], self._bf_scratch.data_ptr())
             self._rules_forward(cur)
-            self._pull(cur, self._seg[j + 1], self._field_hist[j])
+            self._pull(cur, self._seg[j + 1], fields[j])
         ibm, mp = s._ibm, self._mp
         fx, fy, fz = s._force3
         wp, wm = s._rates
@@ -239,21 +252,20 @@ class DragAdjoint:
         for j in reversed(range(length)):
             lam_next, lam_out = self._lam[(j + 1) % 2], self._lam[j % 2]
             s._call("lbm_adj_pull_step", L, D, pointer(lam_next), pointer(lam_out), pointer(self._seg[j]),
-                    pointer(s._kind), pointer(s._wall), pointer(self._field_hist[j]), pointer(s._body),
+                    pointer(s._kind), pointer(s._wall), pointer(fields[j]), pointer(s._body),
                     s.params.nx, s.params.ny, s._nz, wp, wm, fx, fy, fz, FORCE_DENSE, s._mask_mode,
                     pointer(self._grad_body), pointer(self._nu), bx, by, s._stream)
-            if self._rules is not None:
-                r = self._rules
-                s._call("lbm_adj_rules", D, pointer(lam_out), pointer(r["target"]), r["n"], pointer(r["uslot"]),
-                        pointer(r["ustart"]), pointer(r["urule"]), pointer(r["ucoeff"]), r["nu"],
-                        pointer(r["scratch"]), s._stream)
+            r = self._rules or {"target": None, "n": 0, "uslot": None, "ustart": None, "urule": None,
+                                "ucoeff": None, "nu": 0, "scratch": None}
             s._call("lbm_ibm_adjoint", L, D, pointer(self._seg[j]), pointer(mp.slot), pointer(mp.coeff),
                     pointer(mp.constant), mp.m, pointer(s._ibm_nodes), ibm.n_nodes, pointer(s._ibm_inverse),
                     pointer(s._ibm_weights), pointer(s._ibm_areas), ibm.n_markers, pointer(s._ibm_start),
                     pointer(s._ibm_entry), ibm.iterations, fx, fy, fz, s._n, pointer(self._nu),
                     weights[j].data_ptr(), pointer(lam_out), pointer(self._tslot), pointer(self._tstart),
                     pointer(self._trow), pointer(self._tcoeff), len(self._tslot), pointer(self._grad_w),
-                    pointer(self._ibm_adj_scratch), s._stream)
+                    pointer(self._ibm_adj_scratch), pointer(r["target"]), r["n"], pointer(r["uslot"]),
+                    pointer(r["ustart"]), pointer(r["urule"]), pointer(r["ucoeff"]), r["nu"], pointer(r["scratch"]),
+                    s._stream)
 
     @property
     def adjoint_block(self) -> tuple[int, int]:
@@ -292,7 +304,9 @@ class DragAdjoint:
             length = min(self.k, self.steps - t0)
             if store:
                 self._ckpt[seg].copy_(self.solver._f)
-            self._replay(("fwd", length), lambda: self._forward_core(length, self._drag_stage))
+            fields = self._field_hist[t0 : t0 + length] if (store and self.store_fields) else None
+            self._replay(("fwd", length, store, t0 if fields is not None else 0),
+                         lambda: self._forward_core(length, self._drag_stage, fields))
             self._drag[t0 : t0 + length].copy_(self._drag_stage[:length])
 
     def forward(self) -> float:
@@ -320,7 +334,9 @@ class DragAdjoint:
             self._weight_stage[:length].copy_(self._body_weight[t0 : t0 + length])
             if length % 2 and self.n_segments > 1:
                 raise ValueError("segments after the first must have an even length")
-            self._replay(("bwd", length), lambda: self._backward_core(length, self._weight_stage))
+            fields = self._field_hist[t0 : t0 + length] if self.store_fields else self._field_hist
+            self._replay(("bwd", length, t0 if self.store_fields else 0),
+                         lambda: self._backward_core(length, self._weight_stage, fields))
         self._torch.cuda.synchronize()
         p = self.params
         grad_body = self._grad_body.cpu().numpy().reshape(p.d, *p.shape)

@@ -191,14 +191,19 @@ LBM_API int lbm_ibm_adjoint_scratch_size(int lattice, int dtype, int n_nodes, in
  * iterations from f and runs them backwards with seeds nu (dL/dh at forced nodes, d x n_total)
  * and body_weight[d] (dL/d body force, double). Adds the transposed moment rows into lam
  * (distinct slots tslot[n_tslots] with (trow, tcoeff) lists in tstart order) and accumulates
- * dL/dw into grad_w[n_markers][3^d] (double). One workgroup. */
+ * dL/dw into grad_w[n_markers][3^d] (double). When n_rules > 0 it first applies the
+ * transposed link rules exactly as lbm_adj_rules does (same argument meaning), so one
+ * launch covers both sparse adjoints of a step. One workgroup. */
 LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
                             const void* constant, int m, const int64_t* nodes, int n_nodes, const int* inverse,
                             const void* weights, const void* areas, int n_markers, const int* spread_start,
                             const int* spread_entry, int iterations, double fx, double fy, double fz,
                             int64_t n_total, const void* nu, const double* body_weight, void* lam,
                             const int64_t* tslot, const int* tstart, const int* trow, const void* tcoeff,
-                            int n_tslots, double* grad_w, void* scratch, void* stream);
+                            int n_tslots, double* grad_w, void* scratch, const int64_t* rule_target,
+                            int n_rules, const int64_t* rule_uslot, const int* rule_ustart,
+                            const int* rule_urule, const void* rule_ucoeff, int n_rule_unique,
+                            void* rule_scratch, void* stream);
 
 /* Per-kernel resources of the compiled step kernels for (lattice, dtype):
  * fills up to max_kernels entries of {registers, scratch (private) bytes per lane,

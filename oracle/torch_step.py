@@ -20,7 +20,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from oracle.reference_solver import Lattice, bounce_back_links, equilibrium, neighbour
+from oracle.reference_solver import Lattice, bounce_back_links, dot, neighbour
 
 
 def roma_t(r):
@@ -29,6 +29,17 @@ def roma_t(r):
     outer = (5.0 - 3.0 * a - torch.sqrt(torch.clamp(1.0 - 3.0 * (1.0 - a) ** 2, min=1e-30))) / 6.0
     zero = torch.zeros_like(a)
     return torch.where(a <= 0.5, inner, torch.where(a <= 1.5, outer, zero))
+
+
+def equilibrium_t(lat: Lattice, delta_rho, u):
+    """oracle.reference_solver.equilibrium with torch.stack (same expression order)."""
+    rho = 1.0 + delta_rho
+    usq = 1.5 * sum(u_a * u_a for u_a in u)
+    terms = []
+    for c_i, w_i in zip(lat.c, lat.w, strict=True):
+        cu = 3.0 * dot(c_i, u)
+        terms.append(w_i * (delta_rho + rho * (cu + 0.5 * cu * cu - usq)))
+    return torch.stack([t if torch.is_tensor(t) else torch.full_like(delta_rho, t) for t in terms])
 
 
 class TorchDragModel:
@@ -108,7 +119,7 @@ class TorchDragModel:
         rho = 1.0 + drho
         j = [sum(c[a] * f[i] for i, c in enumerate(lat.c) if c[a]) for a in range(lat.d)]
         u = [(j[a] + 0.5 * F[a]) / rho for a in range(lat.d)]
-        feq = equilibrium(lat, drho, u)
+        feq = equilibrium_t(lat, drho, u)
         neq = f - feq
         neq_o = neq[self.opp]
         post = f - 0.5 * self.wp * (neq + neq_o) - 0.5 * self.wm * (neq - neq_o)

@@ -157,7 +157,7 @@ def test_adjoint_matches_float64_torch_autograd(rules):
     x = torch.as_tensor(params.immersed.markers.copy()).requires_grad_(True)
     ref_loss = model.loss(g, x)
     ref_loss.backward()
-    assert loss == pytest.approx(float(ref_loss), rel=1e-12)
+    assert loss == pytest.approx(float(ref_loss.detach()), rel=1e-12)
     scale_g = np.abs(g.grad.numpy()).max()
     np.testing.assert_allclose(grad_g, g.grad.numpy(), rtol=0, atol=1e-9 * scale_g)
     np.testing.assert_allclose(grad_x, x.grad.numpy(), rtol=1e-8, atol=1e-9 * np.abs(x.grad.numpy()).max())

@@ -155,6 +155,51 @@ def adjoint_ratio(torch, steps: int = 4000, window: int = 2000, k: int = 100) ->
     }
 
 
+def batched_measure(torch, count: int = 64, steps: int = 100) -> dict:
+    """Aggregate MLUPS of 64 seeded D2Q9 fp32 Bouzidi-cylinder channels (H in [64, 512],
+    L = 4H, Re in [20, 200], Zou-He inlet/outlet) advanced by one persistent launch per
+    step (graph-captured with the rule pre-pass, autotuned tiles), against the same
+    lattices each run alone (graph-captured pull solver, default tiles)."""
+    from lbm import make_solver
+    from lbm.batched import BatchedSolver
+    from lbm.cases.batched_channels import batched_channels, initial_fields
+
+    t_build = time.perf_counter()
+    params = batched_channels(count=count, seed=0)
+    t_build = time.perf_counter() - t_build
+    batch = BatchedSolver(params, autotune=True)
+    for i, p in enumerate(params):
+        batch.initialize(i, *initial_fields(p))
+    batch.run(4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.run(steps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    total = batch.total_nodes
+    t_alone = 0.0
+    for p in params:
+        solver = make_solver(p, backend="hip-pull", graphs=True)
+        solver.initialize(*initial_fields(p))
+        solver.run(4, check_every=4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver._advance(steps)
+        torch.cuda.synchronize()
+        t_alone += time.perf_counter() - t0
+        del solver
+    return {
+        "workload": f"{count} D2Q9 fp32 Bouzidi-cylinder channels (seed 0), {total} nodes, {steps} steps",
+        "nodes": total,
+        "batched_mlups": round(total * steps / t_batch / 1e6, 1),
+        "alone_mlups": round(total * steps / t_alone / 1e6, 1),
+        "speedup": round(t_alone / t_batch, 3),
+        "block": list(batch.block),
+        "groups": batch.groups,
+        "host_build_seconds": round(t_build, 1),
+    }
+
+
 def traffic_from_profile():
     if TRAFFIC_FILE.exists():
         data = json.loads(TRAFFIC_FILE.read_text())
@@ -172,6 +217,7 @@ def main(argv=None) -> int:
     parser.add_argument("--block", type=int, nargs=2, default=None)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-adjoint", action="store_true")
+    parser.add_argument("--no-batched", action="store_true")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (got --gpus {args.gpus})", file=sys.stderr)
@@ -233,6 +279,11 @@ def main(argv=None) -> int:
         adjoint = adjoint_ratio(torch)
         print(f"# adjoint: {json.dumps(adjoint)}", flush=True)
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_measure(torch)
+        print(f"# batched: {json.dumps(batched)}", flush=True)
+
     baseline = None
     if not args.no_cpu_baseline:
         baseline = cpu_baseline()
@@ -271,6 +322,7 @@ def main(argv=None) -> int:
         },
         "cpu_baseline": baseline,
         "adjoint": adjoint,
+        "batched": batched,
     }
     print(json.dumps(record), flush=True)
     return 0

@@ -1,0 +1,120 @@
+// Many independent lattices of uneven sizes advanced by one persistent launch per step.
+//
+// No counterpart in the reference (one lattice per solver there). All lattices keep their
+// populations in one pool (lattice l at element offset f_offset[l], its own SoA planes),
+// their kind bytes and wall velocities in two more pools. A work table lists every
+// (lattice, tile) of bx x by nodes; a fixed grid of workgroups walks it with a static
+// stride (workgroup w takes tiles w, w + G, w + 2G, ...), so the assignment is
+// deterministic and no atomics are involved. Each node runs pull_update, the code of the
+// single-lattice pull kernel, with its lattice's own grid and relaxation constants, so a
+// batched step produces the same bits as running every lattice alone.
+
+#include "lbm_common.h"
+#include "lbm_step.h"
+
+namespace lbm {
+
+template <class T>
+struct LatticeDesc {
+    Grid g;
+    int pad;
+    uint64_t f_offset;     // elements into the population pools (q * n per lattice)
+    uint64_t kind_offset;  // bytes into the kind pool (n per lattice)
+    uint64_t wall_offset;  // elements into the wall pool (d * n per lattice)
+    Relaxation<T> r;
+};
+
+struct TileDesc {
+    int lattice, x0, y0, z;
+};
+
+template <class L, class T, bool MASK>
+__global__ __launch_bounds__(1024) void k_batched_pull(const T* __restrict__ src, T* __restrict__ dst,
+                                                       const uint8_t* __restrict__ kind, const T* __restrict__ wall,
+                                                       const LatticeDesc<T>* __restrict__ lattices,
+                                                       const TileDesc* __restrict__ tiles, int n_tiles)
+{
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const TileDesc td = tiles[t];
+        const LatticeDesc<T>& d = lattices[td.lattice];
+        const Grid g = d.g;
+        Coord c{td.x0 + (int)threadIdx.x, td.y0 + (int)threadIdx.y, td.z};
+        if (c.x < g.nx && c.y < g.ny)
+            pull_update<L, T, ForceMode::UNIFORM, MASK>(src + d.f_offset, dst + d.f_offset, kind + d.kind_offset,
+                                                       wall + d.wall_offset, nullptr, nullptr, g, d.r, c);
+    }
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+LBM_API int lbm_batched_desc_size(int dtype, int64_t* bytes)
+{
+    return guarded("lbm_batched_desc_size", [&] {
+        dispatch_real(dtype, [&](auto Tt) {
+            using T = decltype(Tt);
+            *bytes = (int64_t)sizeof(LatticeDesc<T>);
+        });
+    });
+}
+
+LBM_API int lbm_batched_describe(int lattice, int dtype, int n_lattices, const int* nx, const int* ny,
+                                 const int* nz, const double* omega_plus, const double* omega_minus,
+                                 const double* force, void* host_desc)
+{
+    return guarded("lbm_batched_describe", [&] {
+        require(n_lattices > 0 && nx && ny && nz && omega_plus && omega_minus && force && host_desc, "bad arguments");
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            auto* out = static_cast<LatticeDesc<T>*>(host_desc);
+            uint64_t f = 0, k = 0, w = 0;
+            for (int l = 0; l < n_lattices; ++l) {
+                LatticeDesc<T> d{};
+                d.g = make_grid(lattice, nx[l], ny[l], nz[l]);
+                d.f_offset = f;
+                d.kind_offset = k;
+                d.wall_offset = w;
+                d.r = make_relaxation<L, T>(omega_plus[l], omega_minus[l], force + 3 * l);
+                f += (uint64_t)L::Q * d.g.n;
+                k += d.g.n;
+                w += (uint64_t)L::D * d.g.n;
+                out[l] = d;
+            }
+        });
+    });
+}
+
+LBM_API int lbm_batched_pull_step(int lattice, int dtype, const void* src_pool, void* dst_pool,
+                                  const uint8_t* kind_pool, const void* wall_pool, const void* desc,
+                                  const int* tiles, int n_tiles, int mask_mode, int bx, int by, int groups,
+                                  void* stream)
+{
+    return guarded("lbm_batched_pull_step", [&] {
+        require(n_tiles > 0 && src_pool && dst_pool && desc && tiles, "bad arguments");
+        require(mask_mode == 0 || (kind_pool && wall_pool), "mask_mode 1 needs kind and wall pools");
+        require(groups > 0, "groups must be positive");
+        const dim3 block = block_of(bx, by);
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            const unsigned grid = (unsigned)std::min(groups, n_tiles);
+            auto* td = reinterpret_cast<const TileDesc*>(tiles);
+            auto* ld = static_cast<const LatticeDesc<T>*>(desc);
+            if (mask_mode)
+                k_batched_pull<L, T, true><<<grid, block, 0, as_stream(stream)>>>(
+                    static_cast<const T*>(src_pool), static_cast<T*>(dst_pool), kind_pool,
+                    static_cast<const T*>(wall_pool), ld, td, n_tiles);
+            else
+                k_batched_pull<L, T, false><<<grid, block, 0, as_stream(stream)>>>(
+                    static_cast<const T*>(src_pool), static_cast<T*>(dst_pool), kind_pool,
+                    static_cast<const T*>(wall_pool), ld, td, n_tiles);
+            check_launch("batched pull step");
+        });
+    });
+}
+
+}  // extern "C"

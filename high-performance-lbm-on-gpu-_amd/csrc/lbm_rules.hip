@@ -69,6 +69,19 @@ __global__ __launch_bounds__(256) void k_affine_values(const T* __restrict__ F, 
     if (r < n) out[r] = affine_row(F, slot, coeff, constant, n, m, r);
 }
 
+// Rows of a concatenation of rule sets: row r sums only its own set's padded width
+// width[r] (the term arrays are padded to the widest set), so each row rounds exactly as
+// it does when its set is evaluated alone.
+template <class T>
+__global__ __launch_bounds__(256) void k_affine_values_w(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                                         const T* __restrict__ coeff, const T* __restrict__ constant,
+                                                         const int* __restrict__ width, int64_t n,
+                                                         T* __restrict__ out)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = affine_row(F, slot, coeff, constant, n, width[r], r);
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_rules_write(T* __restrict__ F, const int64_t* __restrict__ target,
                                                      const T* __restrict__ value, int64_t n)
@@ -156,6 +169,27 @@ LBM_API int lbm_rules_apply(int dtype, void* f, const int64_t* slot, const void*
                                                         n_rules);
             }
             check_launch("rule pre-pass");
+        });
+    });
+}
+
+LBM_API int lbm_rules_apply_widths(int dtype, void* f, const int64_t* slot, const void* coeff, const void* constant,
+                                   const int64_t* target, const int* width, int64_t n_rules, int m, void* scratch,
+                                   void* stream)
+{
+    return guarded("lbm_rules_apply_widths", [&] {
+        require(n_rules >= 0 && m >= 1 && m <= 128, "rules need 1 <= m <= 128 terms");
+        if (n_rules == 0) return;
+        require(f && slot && coeff && constant && target && width && scratch, "null pointer");
+        dispatch_real(dtype, [&](auto Tt) {
+            using T = decltype(Tt);
+            const hipStream_t s = as_stream(stream);
+            const unsigned blocks = (unsigned)((n_rules + 255) / 256);
+            k_affine_values_w<T><<<blocks, 256, 0, s>>>(static_cast<const T*>(f), slot, static_cast<const T*>(coeff),
+                                                        static_cast<const T*>(constant), width, n_rules,
+                                                        static_cast<T*>(scratch));
+            k_rules_write<T><<<blocks, 256, 0, s>>>(static_cast<T*>(f), target, static_cast<const T*>(scratch), n_rules);
+            check_launch("batched rule pre-pass");
         });
     });
 }

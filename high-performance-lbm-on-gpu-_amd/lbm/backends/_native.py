@@ -36,6 +36,10 @@ SIGNATURES: dict[str, list] = {
     "lbm_affine_force": [_i, _p, _p, _p, _p, _p, _i64, _i, _i, _p, _p],
     "lbm_kernel_resources": [_i, _i, POINTER(c_int), _i, c_char_p, _i, POINTER(c_int)],
     "lbm_device_copy": [_p, _p, _i64, _p],
+    "lbm_rules_apply_widths": [_i, _p, _p, _p, _p, _p, _p, _i64, _i, _p, _p],
+    "lbm_batched_desc_size": [_i, POINTER(c_int64)],
+    "lbm_batched_describe": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p],
+    "lbm_batched_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _p],
     "lbm_adj_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _p, _p, _i, _i, _p],
     "lbm_adj_rules": [_i, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p],
     "lbm_ibm_adjoint_scratch_size": [_i, _i, _i, _i, _i, POINTER(c_int64)],

@@ -103,9 +103,10 @@ class HipSolver(Solver):
         variant: ``"pull"``, ``"aa"`` or ``"lds"`` (2D only).
         block: Threads per block (bx, by), at most 1024; default from ``default_block``.
         autotune: Time a few block shapes on this grid and keep the fastest.
+        graphs: Capture two steps (pre-passes and kernels) into a HIP graph and replay it.
     """
 
-    def __init__(self, params, variant: str = "pull", block=None, autotune: bool = False):
+    def __init__(self, params, variant: str = "pull", block=None, autotune: bool = False, graphs: bool = False):
         check_variant(variant, params.d)
         block = default_block(params, variant) if block is None else check_block(block)
         super().__init__(params)
@@ -115,7 +116,9 @@ class HipSolver(Solver):
         self.device = torch.device("cuda")
         self.variant = variant
         self.step_multiple = 2 if variant == "aa" else 1
-        self.block = block
+        self._block = block
+        self.graphs = graphs
+        self._graph = None
         p = params
         self._np_dtype = np.dtype(p.dtype)
         self._torch_dtype = {"float32": torch.float32, "float64": torch.float64}[p.dtype]
@@ -165,6 +168,15 @@ class HipSolver(Solver):
 
             self.block = tune_block(self)
 
+    @property
+    def block(self) -> tuple[int, int]:
+        return self._block
+
+    @block.setter
+    def block(self, value) -> None:
+        self._block = check_block(value)
+        self._graph = None
+
     # ------------------------------------------------------------------ plumbing
 
     @property
@@ -204,10 +216,12 @@ class HipSolver(Solver):
         if field is None:
             self._body = None
             self._force_mode = FORCE_FIELD if self._ibm is not None else FORCE_UNIFORM
+            self._graph = None
             return
         t = self._torch.as_tensor(np.asarray(field), dtype=self._torch_dtype, device=self.device)
         self._body = t.reshape(self.params.d, self._n).contiguous()
         self._force_mode = FORCE_DENSE
+        self._graph = None
 
     # ------------------------------------------------------------------ Solver API
 
@@ -233,6 +247,7 @@ class HipSolver(Solver):
         )
         if self._f_next is not None:
             self._f_next.copy_(self._f)
+        self._graph = None
         if self._field is not None:
             self._field.zero_()
             self._body_force.zero_()
@@ -246,6 +261,29 @@ class HipSolver(Solver):
         return (p.nx, p.ny, self._nz, wp, wm, fx, fy, fz, self._force_mode, self._mask_mode, bx, by, self._stream)
 
     def _advance(self, steps: int) -> None:
+        if self.graphs and steps >= 2:
+            pairs, rest = divmod(steps, 2)
+            for _ in range(pairs):
+                self._advance_pair_graphed()
+            if rest:
+                self._advance_eager(rest)
+            return
+        self._advance_eager(steps)
+
+    def _advance_pair_graphed(self) -> None:
+        """Two steps from one captured HIP graph (buffers return to their places)."""
+        torch = self._torch
+        if self._graph is None:
+            self._advance_eager(2)
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                self._advance_eager(2)
+            self._graph = graph
+        else:
+            self._graph.replay()
+
+    def _advance_eager(self, steps: int) -> None:
         L, D = self._lattice_code, self._dtype_code
         kind = pointer(self._kind)
         field = pointer(self._field)

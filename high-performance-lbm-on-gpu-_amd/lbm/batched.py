@@ -1,0 +1,223 @@
+"""Many independent lattices of uneven sizes advanced together on one GPU.
+
+No counterpart in the reference (one lattice per solver). Every lattice is a pull-layout
+simulation (``SimulationParams``: walls, link rules, uniform force; no immersed body)
+of the same velocity set and precision. All lattices share three device pools
+(populations x2, kind bytes, wall velocities); a step is
+
+1. one rule pre-pass over the concatenated rules of every lattice
+   (``lbm_rules_apply_widths``, rows keep their own padded width so they round as alone);
+2. one persistent launch of ``lbm_batched_pull_step`` over the (lattice, tile) work table.
+
+Two steps (the pool ping-pong) are captured in a HIP graph and replayed. Each lattice's
+populations equal, bit for bit, what ``HipSolver(params, "pull")`` computes for it alone
+(``tests/test_batched.py``). Block shape and the number of persistent workgroups come from
+``lbm.backends.autotune.tune_batched`` when ``autotune=True``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from lbm.backends import _require_device, load_library
+from lbm.backends._native import DTYPE_CODES, pointer
+from lbm.backends.hip_solver import check_block, node_kinds
+from lbm.link_rules import compile_rules
+
+DEFAULT_GROUPS_PER_CU = 8
+COMPUTE_UNITS = 256
+
+
+class BatchedSolver:
+    """Pull-layout lattices ``params_list`` stepped by one persistent launch per step.
+
+    Args:
+        params_list: ``SimulationParams`` of every lattice (same lattice and dtype, no IBM).
+        block: (bx, by) tile = workgroup shape; default (256, 4).
+        groups: persistent workgroups; default 8 per compute unit.
+        graphs: capture the two-step cycle into a HIP graph.
+        autotune: pick block and groups by timing (``lbm.backends.autotune``).
+    """
+
+    def __init__(self, params_list, block=(256, 4), groups=None, graphs: bool = True, autotune: bool = False):
+        params_list = list(params_list)
+        if not params_list:
+            raise ValueError("need at least one lattice")
+        first = params_list[0]
+        for p in params_list:
+            if p.lattice.name != first.lattice.name or p.dtype != first.dtype:
+                raise ValueError("batched lattices must share the velocity set and the dtype")
+            if p.immersed is not None:
+                raise ValueError("batched lattices do not support immersed bodies")
+        torch = _require_device()
+        self._torch = torch
+        self._lib = load_library()
+        self.params_list = params_list
+        self.lattice = first.lattice
+        self.dtype = first.dtype
+        self.device = torch.device("cuda")
+        self._np_dtype = np.dtype(first.dtype)
+        self._tdt = {"float32": torch.float32, "float64": torch.float64}[first.dtype]
+        self._lcode, self._dcode = first.lattice.code, DTYPE_CODES[first.dtype]
+        q, d = first.lattice.q, first.d
+        self.nodes = [p.n_nodes for p in params_list]
+        self.total_nodes = int(sum(self.nodes))
+        self.block = check_block(block)
+        self.groups = int(groups) if groups else DEFAULT_GROUPS_PER_CU * COMPUTE_UNITS
+        self.graphs = graphs
+        self.time_step = 0
+
+        n_l = len(params_list)
+        size = ctypes.c_int64()
+        self._lib.call("lbm_batched_desc_size", self._dcode, ctypes.byref(size))
+        host = np.zeros(n_l * size.value, dtype=np.uint8)
+        ints = lambda vals: np.ascontiguousarray(vals, dtype=np.int32)  # noqa: E731
+        nx = ints([p.nx for p in params_list])
+        ny = ints([p.ny for p in params_list])
+        nz = ints([p.nz or 1 for p in params_list])
+        rates = np.array([p.relaxation_rates for p in params_list], dtype=np.float64)
+        wp, wm = np.ascontiguousarray(rates[:, 0]), np.ascontiguousarray(rates[:, 1])
+        force = np.ascontiguousarray(np.array([tuple(p.force) + (0.0,) * (3 - d) for p in params_list]))
+        self._lib.call("lbm_batched_describe", self._lcode, self._dcode, n_l, nx.ctypes.data, ny.ctypes.data,
+                       nz.ctypes.data, wp.ctypes.data, wm.ctypes.data, force.ctypes.data, host.ctypes.data)
+        self._desc = torch.as_tensor(host, device=self.device)
+        self.f_offset = np.concatenate([[0], np.cumsum([q * n for n in self.nodes])]).astype(np.int64)
+        node_offset = np.concatenate([[0], np.cumsum(self.nodes)]).astype(np.int64)
+        self.node_offset = node_offset
+
+        kinds = np.concatenate([node_kinds(p.lattice, p.solid).ravel() for p in params_list])
+        walls = np.concatenate([p.wall_velocity.reshape(-1).astype(self._np_dtype) for p in params_list])
+        self._kind = torch.as_tensor(kinds, device=self.device)
+        self._wall = torch.as_tensor(walls, device=self.device)
+        self._mask_mode = 1 if any(p.solid.any() for p in params_list) else 0
+        self._fa = torch.zeros(int(self.f_offset[-1]), dtype=self._tdt, device=self.device)
+        self._fb = torch.zeros_like(self._fa)
+        self._build_rules()
+        self._build_tiles()
+        self._graph = None
+        for i in range(n_l):
+            self.initialize(i)
+        if autotune:
+            from lbm.backends.autotune import tune_batched
+
+            self.block, self.groups = tune_batched(self)
+            self._build_tiles()
+
+    # ------------------------------------------------------------------ set-up
+
+    def _build_rules(self) -> None:
+        slots, coeffs, consts, targets, widths = [], [], [], [], []
+        for i, p in enumerate(self.params_list):
+            if p.link_rules is None or not len(p.link_rules):
+                continue
+            c = compile_rules(p.lattice, p.link_rules, "pull", p.solid, p.wall_velocity)
+            base = int(self.f_offset[i])
+            slots.append(np.asarray(c.values.slots) + base)
+            coeffs.append(np.asarray(c.values.coeff))
+            consts.append(np.asarray(c.values.constant))
+            targets.append(np.asarray(c.target) + base)
+            widths.append(np.full(len(c.target), c.values.coeff.shape[1], dtype=np.int32))
+        self._n_rules = int(sum(len(t) for t in targets))
+        if not self._n_rules:
+            return
+        m = max(s.shape[1] for s in slots)
+        pad = lambda a: np.pad(a, ((0, 0), (0, m - a.shape[1])))  # noqa: E731
+        slot = np.concatenate([pad(s) for s in slots]).T
+        coeff = np.concatenate([pad(c) for c in coeffs]).T.astype(self._np_dtype)
+        torch, dev = self._torch, self.device
+        self._rule_m = m
+        self._rule_slot = torch.as_tensor(np.ascontiguousarray(slot), device=dev)
+        self._rule_coeff = torch.as_tensor(np.ascontiguousarray(coeff), device=dev)
+        self._rule_const = torch.as_tensor(np.concatenate(consts).astype(self._np_dtype), device=dev)
+        self._rule_target = torch.as_tensor(np.concatenate(targets), device=dev)
+        self._rule_width = torch.as_tensor(np.concatenate(widths), device=dev)
+        self._rule_scratch = torch.empty(self._n_rules, dtype=self._tdt, device=dev)
+
+    def _build_tiles(self) -> None:
+        bx, by = self.block
+        tiles = []
+        for i, p in enumerate(self.params_list):
+            for z in range(p.nz or 1):
+                for y0 in range(0, p.ny, by):
+                    for x0 in range(0, p.nx, bx):
+                        tiles.append((i, x0, y0, z))
+        self._tiles = self._torch.as_tensor(np.asarray(tiles, dtype=np.int32).ravel(), device=self.device)
+        self._n_tiles = len(tiles)
+        self._graph = None
+
+    # ------------------------------------------------------------------ state
+
+    def view(self, i: int, pool=None):
+        """Populations of lattice i as a (q, N_i) view of the current pool."""
+        pool = self._fa if pool is None else pool
+        q = self.lattice.q
+        return pool[int(self.f_offset[i]) : int(self.f_offset[i + 1])].view(q, self.nodes[i])
+
+    def initialize(self, i: int, rho=1.0, ux=0.0, uy=0.0, uz=0.0) -> None:
+        """Equilibrium of lattice i (same construction as ``Solver.initialize``)."""
+        from lbm.solver import initial_fields
+
+        p = self.params_list[i]
+        drho, u = initial_fields(p, rho, ux, uy, uz)
+        torch, dev = self._torch, self.device
+        dr = torch.as_tensor(np.ascontiguousarray(drho).ravel(), device=dev)
+        comps = [torch.as_tensor(np.ascontiguousarray(a).ravel(), device=dev) for a in u]
+        f = self.view(i)
+        self._lib.call("lbm_init_equilibrium", self._lcode, self._dcode, pointer(f), pointer(dr), pointer(comps[0]),
+                       pointer(comps[1]), pointer(comps[2] if len(comps) == 3 else None), p.n_nodes, 0,
+                       torch.cuda.current_stream().cuda_stream)
+        self.view(i, self._fb).copy_(f)
+
+    def _step(self, src, dst) -> None:
+        stream = self._torch.cuda.current_stream().cuda_stream
+        if self._n_rules:
+            self._lib.call("lbm_rules_apply_widths", self._dcode, pointer(src), pointer(self._rule_slot),
+                           pointer(self._rule_coeff), pointer(self._rule_const), pointer(self._rule_target),
+                           pointer(self._rule_width), self._n_rules, self._rule_m, pointer(self._rule_scratch), stream)
+        bx, by = self.block
+        self._lib.call("lbm_batched_pull_step", self._lcode, self._dcode, pointer(src), pointer(dst),
+                       pointer(self._kind), pointer(self._wall), pointer(self._desc), pointer(self._tiles),
+                       self._n_tiles, self._mask_mode, bx, by, self.groups, stream)
+
+    def _pair(self) -> None:
+        self._step(self._fa, self._fb)
+        self._step(self._fb, self._fa)
+
+    def run(self, steps: int) -> None:
+        """Advance every lattice by ``steps`` steps (pairs replayed from one captured graph)."""
+        torch = self._torch
+        pairs, odd = divmod(int(steps), 2)
+        for _ in range(pairs):
+            if not self.graphs:
+                self._pair()
+                continue
+            if self._graph is None:
+                self._pair()
+                torch.cuda.synchronize()
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    self._pair()
+                self._graph = graph
+            else:
+                self._graph.replay()
+        if odd:
+            self._step(self._fa, self._fb)
+            self._fa, self._fb = self._fb, self._fa
+            self._graph = None
+        self.time_step += int(steps)
+
+    def synchronize(self) -> None:
+        self._torch.cuda.synchronize()
+
+    def populations(self, i: int) -> np.ndarray:
+        p = self.params_list[i]
+        return self.view(i).cpu().numpy().reshape(p.lattice.q, *p.shape)
+
+    def snapshot(self):
+        return self._fa.clone()
+
+    def restore(self, state) -> None:
+        self._fa.copy_(state)
+        self._fb.copy_(state)

@@ -124,6 +124,14 @@ LBM_API int lbm_rules_apply(int dtype, void* f, const int64_t* slot, const void*
                             const void* constant, const int64_t* target, int64_t n_rules, int m,
                             void* scratch, void* stream);
 
+/* lbm_rules_apply for a concatenation of rule sets (no counterpart in the reference): row r
+ * sums only its first width[r] padded terms (its own set's width; m is the padded width of
+ * the arrays), so every row rounds as when its set is applied alone. Always two launches
+ * (evaluate into scratch[n_rules], then write). Used by the batched multi-lattice step. */
+LBM_API int lbm_rules_apply_widths(int dtype, void* f, const int64_t* slot, const void* coeff,
+                                   const void* constant, const int64_t* target, const int* width,
+                                   int64_t n_rules, int m, void* scratch, void* stream);
+
 /* Values of compiled affine expressions on the state: out[r] (dtype), same
  * arithmetic as the rule evaluation. Replaces Solver.evaluate_affine
  * (src/lbm/solver.py:178-188) and link_rules.evaluate_affine (src/lbm/link_rules.py:269-272). */
@@ -204,6 +212,26 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
                             int n_rules, const int64_t* rule_uslot, const int* rule_ustart,
                             const int* rule_urule, const void* rule_ucoeff, int n_rule_unique,
                             void* rule_scratch, void* stream);
+
+/* Size in bytes of one lattice descriptor of the batched step (no counterpart in the
+ * reference, which runs one lattice per solver). Host only. */
+LBM_API int lbm_batched_desc_size(int dtype, int64_t* bytes);
+
+/* Fill n_lattices descriptors (host memory, lbm_batched_desc_size bytes each) for lattices
+ * of sizes nx/ny/nz[l], rates omega_plus/minus[l] and uniform force force[3 l .. 3 l + 2],
+ * laid out back to back in the pools: populations q n_l, kind bytes n_l, wall d n_l. */
+LBM_API int lbm_batched_describe(int lattice, int dtype, int n_lattices, const int* nx, const int* ny,
+                                 const int* nz, const double* omega_plus, const double* omega_minus,
+                                 const double* force, void* host_desc);
+
+/* One two-lattice pull step of every lattice of a batch in one launch: `groups` workgroups
+ * of bx x by threads walk the n_tiles work items tiles[4 t .. 4 t + 3] = (lattice, x0, y0, z)
+ * with a static stride; desc is the device copy of the descriptors. Same per-node
+ * arithmetic as lbm_pull_step (uniform force mode). No counterpart in the reference. */
+LBM_API int lbm_batched_pull_step(int lattice, int dtype, const void* src_pool, void* dst_pool,
+                                  const uint8_t* kind_pool, const void* wall_pool, const void* desc,
+                                  const int* tiles, int n_tiles, int mask_mode, int bx, int by, int groups,
+                                  void* stream);
 
 /* Per-kernel resources of the compiled step kernels for (lattice, dtype):
  * fills up to max_kernels entries of {registers, scratch (private) bytes per lane,

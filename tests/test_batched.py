@@ -1,0 +1,80 @@
+"""Batched multi-lattice stepping: every lattice of a batch must equal, bit for bit, the same
+lattice run alone through ``HipSolver(params, "pull")`` (no counterpart in the reference)."""
+
+import numpy as np
+import pytest
+
+from lbm import SimulationParams, make_solver
+from lbm.cases.batched_channels import batched_channels, initial_fields
+
+pytestmark = pytest.mark.gpu
+
+
+def small_batch(dtype="float32"):
+    params = batched_channels(count=5, seed=7, dtype=dtype, h_range=(24, 72))
+    params.append(SimulationParams(nx=40, ny=30, tau=0.8, collision="bgk", force=(1e-5, 0.0), dtype=dtype))
+    return params
+
+
+def alone(params, steps, graphs=False):
+    solver = make_solver(params, backend="hip-pull", graphs=graphs)
+    solver.initialize(*initial_fields(params))
+    solver.run(steps)
+    return solver.populations()
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("steps", [20, 21])
+def test_batched_equals_each_lattice_alone(dtype, steps):
+    from lbm.batched import BatchedSolver
+
+    params = small_batch(dtype)
+    batch = BatchedSolver(params)
+    for i, p in enumerate(params):
+        batch.initialize(i, *initial_fields(p))
+    batch.run(steps)
+    for i, p in enumerate(params):
+        np.testing.assert_array_equal(batch.populations(i), alone(p, steps))
+
+
+@pytest.mark.parametrize(("block", "groups"), [((64, 4), 3), ((256, 1), 1000), ((128, 8), 64)])
+def test_batched_results_do_not_depend_on_tiles_or_grid(block, groups):
+    from lbm.batched import BatchedSolver
+
+    params = small_batch()
+    ref = BatchedSolver(params)
+    other = BatchedSolver(params, block=block, groups=groups, graphs=False)
+    for b in (ref, other):
+        for i, p in enumerate(params):
+            b.initialize(i, *initial_fields(p))
+        b.run(10)
+    for i in range(len(params)):
+        np.testing.assert_array_equal(other.populations(i), ref.populations(i))
+
+
+def test_autotuned_batch_still_matches():
+    from lbm.batched import BatchedSolver
+
+    params = small_batch()
+    batch = BatchedSolver(params, autotune=True)
+    for i, p in enumerate(params):
+        batch.initialize(i, *initial_fields(p))
+    batch.run(8)
+    np.testing.assert_array_equal(batch.populations(0), alone(params[0], 8))
+
+
+def test_graphed_single_solver_equals_eager():
+    p = small_batch()[1]
+    np.testing.assert_array_equal(alone(p, 13, graphs=True), alone(p, 13, graphs=False))
+
+
+def test_autotuned_solver_keeps_its_state():
+    from tests.helpers import mixed_params, random_state
+
+    params = mixed_params("float64", "trt", 3)
+    a = make_solver(params, backend="hip-aa")
+    b = make_solver(params, backend="hip-aa", autotune=True)
+    for s in (a, b):
+        s.initialize(*random_state(params))
+        s.run(10)
+    np.testing.assert_array_equal(a.populations(), b.populations())

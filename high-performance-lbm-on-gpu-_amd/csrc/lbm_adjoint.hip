@@ -12,7 +12,9 @@
 // AA odd step, so the push needs no atomics; solid-node slots are never written and
 // stay zero. dL/dg accumulates nu in float64.
 
+#include "lbm_affine.h"
 #include "lbm_common.h"
+#include "lbm_ibm_core.h"
 
 namespace lbm {
 
@@ -175,226 +177,339 @@ __global__ __launch_bounds__(256) void k_adj_pull(const T* __restrict__ lam_next
     }
 }
 
+// lam[slot[t]] += sum_{e in [start[t], start[t+1])} coeff[e] * x[row[e]] for every t < n.
+// The slots are distinct, so no two threads touch one. A thread takes four slots at a
+// time and issues their loads together, the first four entries of each list included;
+// longer lists finish in a loop.
+template <class T>
+__device__ __forceinline__ void scatter_rows(T* __restrict__ lam, const int64_t* __restrict__ slot,
+                                             const int* __restrict__ start, const int* __restrict__ row,
+                                             const T* __restrict__ coeff, int n, const T* x, int tid, int nthreads)
+{
+    constexpr int K = 4, E = 4;
+    for (int t0 = tid; t0 < n; t0 += K * nthreads) {
+        int64_t sl[K];
+        int b[K], end[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int t = t0 + k * nthreads;
+            if (t < n) {
+                sl[k] = slot[t];
+                b[k] = start[t];
+                end[k] = start[t + 1];
+            }
+        }
+        T old[K];
+        int rr[K][E];
+        T cc[K][E];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (t0 + k * nthreads < n) {
+                old[k] = lam[sl[k]];
+#pragma unroll
+                for (int j = 0; j < E; ++j)
+                    if (b[k] + j < end[k]) {
+                        rr[k][j] = row[b[k] + j];
+                        cc[k][j] = coeff[b[k] + j];
+                    }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (t0 + k * nthreads < n) {
+                T acc = T(0);
+#pragma unroll
+                for (int j = 0; j < E; ++j)
+                    if (b[k] + j < end[k]) acc += cc[k][j] * x[rr[k][j]];
+                for (int e = b[k] + E; e < end[k]; ++e) acc += coeff[e] * x[row[e]];
+                lam[sl[k]] = old[k] + acc;
+            }
+        }
+    }
+}
+
 // Transposed link rules: a_r = lam[target_r]; lam[target_r] = 0; lam[slot] += sum coeff a_r
 // over the (rule, coefficient) entries of each distinct term slot (host-built, fixed order).
+// Rule targets are distinct, so each target is read and cleared by its own thread.
 template <class T>
 __global__ __launch_bounds__(1024) void k_adj_rules(T* __restrict__ lam, const int64_t* __restrict__ target,
                                                     int n_rules, const int64_t* __restrict__ uslot,
                                                     const int* __restrict__ ustart, const int* __restrict__ urule,
                                                     const T* __restrict__ ucoeff, int n_unique, T* __restrict__ a)
 {
-    for (int r = threadIdx.x; r < n_rules; r += blockDim.x) a[r] = lam[target[r]];
-    __syncthreads();
-    for (int r = threadIdx.x; r < n_rules; r += blockDim.x) lam[target[r]] = T(0);
-    __syncthreads();
-    for (int u = threadIdx.x; u < n_unique; u += blockDim.x) {
-        T s = T(0);
-        for (int e = ustart[u]; e < ustart[u + 1]; ++e) s += ucoeff[e] * a[urule[e]];
-        lam[uslot[u]] += s;
+    for (int r = threadIdx.x; r < n_rules; r += blockDim.x) {
+        const int64_t t = target[r];
+        a[r] = lam[t];
+        lam[t] = T(0);
     }
+    __syncthreads();
+    scatter_rows(lam, uslot, ustart, urule, ucoeff, n_unique, a, threadIdx.x, blockDim.x);
 }
 
 // ------------------------------------------------------------------ IBM adjoint
 
-constexpr int IBM_ADJ_THREADS = 1024;
+// Workgroup bound as for the forward kernel: 1024 lanes in 2D, 512 in 3D.
+constexpr int ibm_adj_max_threads(int d) { return d == 2 ? 1024 : 512; }
 
+// Forward intermediates (lbm_ibm_core.h layout, every iteration kept) followed by the
+// reverse-sweep arrays.
 template <class T>
-struct IbmAdjScratch {
-    T *moments, *rho, *ustar, *rho_m;     // (d+1) nn, nn, d nn, nm
-    T *force;                              // (iters + 1) d nn
-    T *u;                                  // iters d nn
-    T *U, *Fm;                             // iters d nm each
-    T *force_bar, *u_bar, *ustar_bar, *rho_bar, *rho_m_bar, *Fm_bar, *U_bar, *m_bar;
+struct IbmAdjScratch : IbmForward<T> {
+    T *force_bar, *ustar_bar, *rho_bar, *rho_m_bar, *Fm_bar, *U_bar, *m_bar;
 };
 
 template <class T>
 __host__ __device__ inline size_t ibm_adj_layout(T* base, int d, int nn, int nm, int iters, IbmAdjScratch<T>* s)
 {
-    size_t o = 0;
+    size_t o = ibm_forward_layout<T>(base, d, nn, nm, iters, s);
+    IbmAdjScratch<T> dummy;
+    IbmAdjScratch<T>& v = s ? *s : dummy;
     auto take = [&](T** p, size_t count) {
-        if (s) *p = base + o;
+        *p = base + o;
         o += count;
     };
-    T* dummy = nullptr;
-    take(s ? &s->moments : &dummy, (size_t)(d + 1) * nn);
-    take(s ? &s->rho : &dummy, nn);
-    take(s ? &s->ustar : &dummy, (size_t)d * nn);
-    take(s ? &s->rho_m : &dummy, nm);
-    take(s ? &s->force : &dummy, (size_t)(iters + 1) * d * nn);
-    take(s ? &s->u : &dummy, (size_t)iters * d * nn);
-    take(s ? &s->U : &dummy, (size_t)iters * d * nm);
-    take(s ? &s->Fm : &dummy, (size_t)iters * d * nm);
-    take(s ? &s->force_bar : &dummy, (size_t)d * nn);
-    take(s ? &s->u_bar : &dummy, (size_t)d * nn);
-    take(s ? &s->ustar_bar : &dummy, (size_t)d * nn);
-    take(s ? &s->rho_bar : &dummy, nn);
-    take(s ? &s->rho_m_bar : &dummy, nm);
-    take(s ? &s->Fm_bar : &dummy, (size_t)d * nm);
-    take(s ? &s->U_bar : &dummy, (size_t)d * nm);
-    take(s ? &s->m_bar : &dummy, (size_t)(d + 1) * nn);
+    take(&v.force_bar, (size_t)d * nn);
+    take(&v.ustar_bar, (size_t)d * nn);
+    take(&v.rho_bar, nn);
+    take(&v.rho_m_bar, nm);
+    take(&v.Fm_bar, (size_t)d * nm);
+    take(&v.U_bar, (size_t)d * nm);
+    take(&v.m_bar, (size_t)(d + 1) * nn);
     return o;
 }
 
-// One workgroup: recompute the IBM iterations from the state, then run them backwards.
-// Seeds: nu (dL/dh at the forced nodes, d x n_total, written by k_adj_pull) and
-// body_weight[d] = dL/d(body force). Outputs: lam[slot] += (M^T m_bar)[slot] through
-// the transposed moment rows (distinct slots with their (row, coeff) lists), and
-// grad_w[l][s] += dL/dw_ls in float64.
-template <class T, int D>
-__global__ __launch_bounds__(IBM_ADJ_THREADS) void k_ibm_adj(
+// One workgroup: the forward intermediates of the step (read from `record` when given,
+// else recomputed from the state with the forward kernel's own code), then the iterations
+// backwards. Seeds: nu (dL/dh at the forced nodes, d x n_total, written by k_adj_pull) and
+// body_weight[d] = dL/d(body force). Outputs: lam[slot] += (M^T m_bar)[slot] through the
+// transposed moment rows (distinct slots with their (row, coeff) lists), and
+// grad_w[l][s] += dL/dw_ls in float64. With n_rules > 0 the transposed link rules are
+// applied first (they act on slots the IBM rows never read).
+// Byte offsets of the staged variant's LDS (16-byte aligned parts): float64 weight
+// gradients, intermediates, stencil tables and, when they fit, the rule adjoints a_r.
+struct AdjLdsPlan {
+    size_t grad_w, base, weights, areas, inverse, entry, start, rule_a, total;
+};
+
+__host__ __device__ inline AdjLdsPlan adj_lds_plan(int d, int nn, int nm, int iters, size_t real, int lds_rules)
+{
+    auto up = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    const size_t S = d == 2 ? 9 : 27;
+    const size_t rules = lds_rules;
+    AdjLdsPlan p{};
+    size_t o = 0;
+    p.grad_w = o;
+    o = up(o + nm * S * sizeof(double));
+    p.base = o;
+    o = up(o + ibm_adj_layout<float>(nullptr, d, nn, nm, iters, nullptr) * real);
+    p.weights = o;
+    o = up(o + nm * S * real);
+    p.areas = o;
+    o = up(o + nm * real);
+    p.inverse = o;
+    o = up(o + nm * S * sizeof(int));
+    p.entry = o;
+    o = up(o + nm * S * sizeof(int));
+    p.start = o;
+    o = up(o + (nn + 1) * sizeof(int));
+    p.rule_a = o;
+    o = up(o + rules * real);
+    p.total = o;
+    return p;
+}
+
+// One workgroup: the forward intermediates of the step (read from `record` when given,
+// else recomputed from the state with the forward kernel's own code), then the iterations
+// backwards. Seeds: nu (dL/dh at the forced nodes, d x n_total, written by k_adj_pull) and
+// body_weight[d] = dL/d(body force). Outputs: lam[slot] += (M^T m_bar)[slot] through the
+// transposed moment rows (distinct slots with their (row, coeff) lists), and
+// grad_w[l][s] += dL/dw_ls in float64. With n_rules > 0 the transposed link rules are
+// applied first (they act on slots the IBM rows never read).
+template <class T, int D, bool STAGED>
+__global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
     const T* __restrict__ F, const int64_t* __restrict__ slot, const T* __restrict__ coeff,
-    const T* __restrict__ constant, int m, const int64_t* __restrict__ nodes, int nn, const int* __restrict__ inverse,
-    const T* __restrict__ weights, const T* __restrict__ areas, int nm, const int* __restrict__ spread_start,
-    const int* __restrict__ spread_entry, int iters, T hfx, T hfy, T hfz, uint32_t n_total,
+    const T* __restrict__ constant, int m, const int64_t* __restrict__ nodes, int nn, const int* __restrict__ inverse_g,
+    const T* __restrict__ weights_g, const T* __restrict__ areas_g, int nm, const int* __restrict__ start_g,
+    const int* __restrict__ entry_g, int iters, T hfx, T hfy, T hfz, uint32_t n_total,
     const T* __restrict__ nu, const double* __restrict__ body_weight, T* __restrict__ lam,
     const int64_t* __restrict__ tslot, const int* __restrict__ tstart, const int* __restrict__ trow,
-    const T* __restrict__ tcoeff, int n_tslots, double* __restrict__ grad_w, T* __restrict__ scratch,
+    const T* __restrict__ tcoeff, int n_tslots, double* __restrict__ grad_w_g, T* __restrict__ scratch,
     const int64_t* __restrict__ rtarget, int n_rules, const int64_t* __restrict__ ruslot,
     const int* __restrict__ rustart, const int* __restrict__ rurule, const T* __restrict__ rucoeff, int n_runique,
-    T* __restrict__ rule_a)
+    T* __restrict__ rule_a, int lds_rules, const T* __restrict__ record)
 {
     constexpr int S = D == 2 ? 9 : 27;
-    IbmAdjScratch<T> s;
-    ibm_adj_layout<T>(scratch, D, nn, nm, iters, &s);
+    extern __shared__ __align__(16) unsigned char adj_lds[];
     const int tid = threadIdx.x;
-    // Transposed link rules first (they act on slots the IBM rows never read).
-    if (n_rules > 0) {
-        for (int r = tid; r < n_rules; r += IBM_ADJ_THREADS) rule_a[r] = lam[rtarget[r]];
-        __syncthreads();
-        for (int r = tid; r < n_rules; r += IBM_ADJ_THREADS) lam[rtarget[r]] = T(0);
-        __syncthreads();
-        for (int u = tid; u < n_runique; u += IBM_ADJ_THREADS) {
-            T acc = T(0);
-            for (int e = rustart[u]; e < rustart[u + 1]; ++e) acc += rucoeff[e] * rule_a[rurule[e]];
-            lam[ruslot[u]] += acc;
+    const int nthreads = blockDim.x;
+    // Index loads of the first interval go out before the table copies, so their second
+    // level (lam at the rule targets, nu at the nodes) costs one more latency, not three.
+    const bool has_rule = tid < n_rules, has_node = tid < nn;
+    const int64_t target0 = has_rule ? rtarget[tid] : 0;
+    const int64_t node0 = has_node ? nodes[tid] : 0;
+    IbmAdjScratch<T> s;
+    IbmTables<T> tb{inverse_g, weights_g, areas_g, start_g, entry_g};
+    double* grad_w = grad_w_g;
+    T* base;
+    if constexpr (STAGED) {
+        const AdjLdsPlan p = adj_lds_plan(D, nn, nm, iters, sizeof(T), lds_rules);
+        double* gw = reinterpret_cast<double*>(adj_lds + p.grad_w);
+        base = reinterpret_cast<T*>(adj_lds + p.base);
+        ibm_adj_layout<T>(base, D, nn, nm, iters, &s);
+        T* w = reinterpret_cast<T*>(adj_lds + p.weights);
+        T* a = reinterpret_cast<T*>(adj_lds + p.areas);
+        int* inv = reinterpret_cast<int*>(adj_lds + p.inverse);
+        int* ent = reinterpret_cast<int*>(adj_lds + p.entry);
+        int* st = reinterpret_cast<int*>(adj_lds + p.start);
+        if (lds_rules) rule_a = reinterpret_cast<T*>(adj_lds + p.rule_a);
+        for (int e = tid; e < nm * S; e += nthreads) gw[e] = 0.0;
+        const WordCopy tables[6] = {
+            word_copy(inv, inverse_g, (size_t)nm * S),
+            word_copy(w, weights_g, (size_t)nm * S),
+            word_copy(ent, entry_g, (size_t)nm * S),
+            word_copy(a, areas_g, (size_t)nm),
+            word_copy(st, start_g, (size_t)nn + 1),
+            word_copy(base, record, record ? ibm_record_elems(D, nn, nm, iters) : 0),
+        };
+        copy_tables(tables, tid, nthreads);
+        tb = IbmTables<T>{inv, w, a, st, ent};
+        grad_w = gw;
+    } else {
+        base = scratch;
+        ibm_adj_layout<T>(scratch, D, nn, nm, iters, &s);
+        if (record) {
+            const WordCopy rec[1] = {word_copy(base, record, ibm_record_elems(D, nn, nm, iters))};
+            copy_tables(rec, tid, nthreads);
         }
-        __syncthreads();
     }
-    const int rows = (D + 1) * nn;
-    const T hf[3] = {hfx, hfy, hfz};
-    // ---- forward recompute (same expressions as k_ibm; summation order is irrelevant here)
-    for (int r = tid; r < rows; r += IBM_ADJ_THREADS) {
-        T acc = constant[r];
-        for (int p = 0; p < m; ++p) {
-            const int64_t e = (int64_t)p * rows + r;
-            const T a = coeff[e];
-            if (a != T(0)) acc += a * F[slot[e]];
+    // First interval, all independent: the tables above, the recorded intermediates, the
+    // rule adjoints a_r (targets cleared) and the reverse-sweep seeds.
+    {
+        const T lam0 = has_rule ? lam[target0] : T(0);
+        T nu0[D];
+#pragma unroll
+        for (int a = 0; a < D; ++a) nu0[a] = has_node ? nu[(size_t)a * n_total + node0] : T(0);
+        if (has_rule) {
+            rule_a[tid] = lam0;
+            lam[target0] = T(0);
         }
-        s.moments[r] = acc;
-    }
-    __syncthreads();
-    for (int n = tid; n < nn; n += IBM_ADJ_THREADS) {
-        const T rho = T(1) + s.moments[n];
-        s.rho[n] = rho;
-        for (int a = 0; a < D; ++a) {
-            s.ustar[a * nn + n] = (s.moments[(1 + a) * nn + n] + hf[a]) / rho;
-            s.force[a * nn + n] = T(0);
-        }
-    }
-    __syncthreads();
-    for (int l = tid; l < nm; l += IBM_ADJ_THREADS) {
-        T acc = T(0);
-        for (int p = 0; p < S; ++p) acc += weights[l * S + p] * s.rho[inverse[l * S + p]];
-        s.rho_m[l] = acc;
-    }
-    __syncthreads();
-    for (int it = 0; it < iters; ++it) {
-        const T* fo = s.force + (size_t)it * D * nn;
-        T* fn = s.force + (size_t)(it + 1) * D * nn;
-        T* u = s.u + (size_t)it * D * nn;
-        T* U = s.U + (size_t)it * D * nm;
-        T* Fm = s.Fm + (size_t)it * D * nm;
-        for (int n = tid; n < nn; n += IBM_ADJ_THREADS)
-            for (int a = 0; a < D; ++a) u[a * nn + n] = s.ustar[a * nn + n] + fo[a * nn + n] / (T(2) * s.rho[n]);
-        __syncthreads();
-        for (int l = tid; l < nm; l += IBM_ADJ_THREADS)
+        if (has_node) {
+#pragma unroll
             for (int a = 0; a < D; ++a) {
-                T acc = T(0);
-                for (int p = 0; p < S; ++p) acc += weights[l * S + p] * u[a * nn + inverse[l * S + p]];
-                U[a * nm + l] = acc;
-                Fm[a * nm + l] = (T(2) * s.rho_m[l]) * (T(0) - acc) * areas[l];
+                s.force_bar[a * nn + tid] = nu0[a];
+                s.ustar_bar[a * nn + tid] = T(0);
             }
-        __syncthreads();
-        for (int n = tid; n < nn; n += IBM_ADJ_THREADS)
-            for (int a = 0; a < D; ++a) {
-                T acc = T(0);
-                for (int e = spread_start[n]; e < spread_start[n + 1]; ++e) {
-                    const int k = spread_entry[e];
-                    acc += weights[k] * Fm[a * nm + k / S];
-                }
-                fn[a * nn + n] = fo[a * nn + n] + acc;
-            }
-        __syncthreads();
+            s.rho_bar[tid] = T(0);
+        }
     }
-    // ---- reverse sweep
-    const T total_bar[3] = {T(-body_weight[0]), T(-body_weight[1]), T(D == 3 ? -body_weight[2] : 0.0)};
-    for (int n = tid; n < nn; n += IBM_ADJ_THREADS) {
+    for (int r = tid + nthreads; r < n_rules; r += nthreads) {
+        const int64_t t = rtarget[r];
+        rule_a[r] = lam[t];
+        lam[t] = T(0);
+    }
+    for (int n = tid + nthreads; n < nn; n += nthreads) {
         for (int a = 0; a < D; ++a) {
             s.force_bar[a * nn + n] = nu[(size_t)a * n_total + nodes[n]];
             s.ustar_bar[a * nn + n] = T(0);
         }
         s.rho_bar[n] = T(0);
     }
-    for (int l = tid; l < nm; l += IBM_ADJ_THREADS) s.rho_m_bar[l] = T(0);
+    for (int l = tid; l < nm; l += nthreads) s.rho_m_bar[l] = T(0);
     __syncthreads();
+    if (n_rules > 0) scatter_rows(lam, ruslot, rustart, rurule, rucoeff, n_runique, rule_a, tid, nthreads);
+    if (!record) {
+        const int rows = (D + 1) * nn;
+        const T hf[3] = {hfx, hfy, hfz};
+        for (int r = tid; r < rows; r += nthreads) s.moments[r] = affine_row(F, slot, coeff, constant, rows, m, r);
+        __syncthreads();
+        ibm_iterate<T, D>(s, tb, nn, nm, iters, true, hf, nullptr, tid, nthreads);
+    }
+    // ---- reverse sweep (the rule scatter above touches only lam, which the sweep leaves
+    // alone until the final scatter, several barriers later)
+    const T total_bar[3] = {T(-body_weight[0]), T(-body_weight[1]), T(D == 3 ? -body_weight[2] : 0.0)};
     for (int it = iters - 1; it >= 0; --it) {
         const T* fo = s.force + (size_t)it * D * nn;
         const T* u = s.u + (size_t)it * D * nn;
         const T* U = s.U + (size_t)it * D * nm;
         const T* Fm = s.Fm + (size_t)it * D * nm;
-        // Fm_bar from the spread and the total; weight gradient of the spread.
-        for (int l = tid; l < nm; l += IBM_ADJ_THREADS) {
-            T fb[3] = {total_bar[0], total_bar[1], total_bar[2]};
+        // Fm_bar from the spread and the total; weight gradients of the spread and of U.
+        for (int l = tid; l < nm; l += nthreads) {
+            int inv[S];
+            T w[S];
+#pragma unroll
             for (int p = 0; p < S; ++p) {
-                const int n = inverse[l * S + p];
+                inv[p] = tb.inverse[(size_t)l * S + p];
+                w[p] = tb.weights[(size_t)l * S + p];
+            }
+            T fm[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) fm[a] = Fm[a * nm + l];
+            T fb[3] = {total_bar[0], total_bar[1], total_bar[2]};
+#pragma unroll
+            for (int p = 0; p < S; ++p) {
                 double gw = 0.0;
+#pragma unroll
                 for (int a = 0; a < D; ++a) {
-                    const T fbar = s.force_bar[a * nn + n];
-                    fb[a] += weights[l * S + p] * fbar;
-                    gw += (double)(Fm[a * nm + l] * fbar);
+                    const T fbar = s.force_bar[a * nn + inv[p]];
+                    fb[a] += w[p] * fbar;
+                    gw += (double)(fm[a] * fbar);
                 }
                 grad_w[l * S + p] += gw;
             }
+            const T area = tb.areas[l];
             T rmb = T(0);
+            T ub[D];
+#pragma unroll
             for (int a = 0; a < D; ++a) {
                 s.Fm_bar[a * nm + l] = fb[a];
-                rmb += fb[a] * T(2) * (T(0) - U[a * nm + l]) * areas[l];
-                s.U_bar[a * nm + l] = -fb[a] * T(2) * s.rho_m[l] * areas[l];
+                rmb += fb[a] * T(2) * (T(0) - U[a * nm + l]) * area;
+                ub[a] = -fb[a] * T(2) * s.rho_m[l] * area;
+                s.U_bar[a * nm + l] = ub[a];
             }
             s.rho_m_bar[l] += rmb;
-            // weight gradient of the interpolation U = sum w u
+#pragma unroll
             for (int p = 0; p < S; ++p) {
                 double gw = 0.0;
-                for (int a = 0; a < D; ++a) gw += (double)(s.U_bar[a * nm + l] * u[a * nn + inverse[l * S + p]]);
+#pragma unroll
+                for (int a = 0; a < D; ++a) gw += (double)(ub[a] * u[a * nn + inv[p]]);
                 grad_w[l * S + p] += gw;
             }
         }
         __syncthreads();
-        // u_bar (gather over the spread lists), then into ustar_bar, force_bar, rho_bar.
-        for (int n = tid; n < nn; n += IBM_ADJ_THREADS) {
-            for (int a = 0; a < D; ++a) {
-                T acc = T(0);
-                for (int e = spread_start[n]; e < spread_start[n + 1]; ++e) {
-                    const int k = spread_entry[e];
-                    acc += weights[k] * s.U_bar[a * nm + k / S];
-                }
-                const T two_rho = T(2) * s.rho[n];
-                s.ustar_bar[a * nn + n] += acc;
-                s.force_bar[a * nn + n] += acc / two_rho;
-                s.rho_bar[n] -= acc * fo[a * nn + n] / (two_rho * s.rho[n]);
+        // u_bar (one walk of the spread list for all components), then into ustar_bar,
+        // force_bar and rho_bar.
+        for (int n = tid; n < nn; n += nthreads) {
+            T acc[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc[a] = T(0);
+            for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
+                const int k = tb.spread_entry[e];
+                const T w = tb.weights[k];
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc[a] += w * s.U_bar[a * nm + k / S];
             }
+            const T rho = s.rho[n];
+            const T two_rho = T(2) * rho;
+            T rb = s.rho_bar[n];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                s.ustar_bar[a * nn + n] += acc[a];
+                s.force_bar[a * nn + n] += acc[a] / two_rho;
+                rb -= acc[a] * fo[a * nn + n] / (two_rho * rho);
+            }
+            s.rho_bar[n] = rb;
         }
         __syncthreads();
     }
-    // rho_m = sum_s w rho(x_ls): weight gradient and rho_bar.
-    for (int l = tid; l < nm; l += IBM_ADJ_THREADS)
-        for (int p = 0; p < S; ++p) grad_w[l * S + p] += (double)(s.rho_m_bar[l] * s.rho[inverse[l * S + p]]);
-    __syncthreads();
-    for (int n = tid; n < nn; n += IBM_ADJ_THREADS) {
+    // rho_m = sum_s w rho(x_ls): weight gradient (per marker) and, per node, rho_bar and the
+    // moment adjoints m_bar.
+    for (int l = tid; l < nm; l += nthreads)
+        for (int p = 0; p < S; ++p) grad_w[l * S + p] += (double)(s.rho_m_bar[l] * s.rho[tb.inverse[l * S + p]]);
+    for (int n = tid; n < nn; n += nthreads) {
         T acc = T(0);
-        for (int e = spread_start[n]; e < spread_start[n + 1]; ++e) {
-            const int k = spread_entry[e];
-            acc += weights[k] * s.rho_m_bar[k / S];
+        for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
+            const int k = tb.spread_entry[e];
+            acc += tb.weights[k] * s.rho_m_bar[k / S];
         }
         T rb = s.rho_bar[n] + acc;
         const T rho = s.rho[n];
@@ -406,10 +521,9 @@ __global__ __launch_bounds__(IBM_ADJ_THREADS) void k_ibm_adj(
         s.m_bar[n] = rb;
     }
     __syncthreads();
-    for (int t = tid; t < n_tslots; t += IBM_ADJ_THREADS) {
-        T acc = T(0);
-        for (int e = tstart[t]; e < tstart[t + 1]; ++e) acc += tcoeff[e] * s.m_bar[trow[e]];
-        lam[tslot[t]] += acc;
+    scatter_rows(lam, tslot, tstart, trow, tcoeff, n_tslots, s.m_bar, tid, nthreads);
+    if constexpr (STAGED) {
+        for (int e = tid; e < nm * S; e += nthreads) grad_w_g[e] += grad_w[e];
     }
 }
 
@@ -500,7 +614,7 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
                             const int* tstart, const int* trow, const void* tcoeff, int n_tslots, double* grad_w,
                             void* scratch, const int64_t* rule_target, int n_rules, const int64_t* rule_uslot,
                             const int* rule_ustart, const int* rule_urule, const void* rule_ucoeff, int n_rule_unique,
-                            void* rule_scratch, void* stream)
+                            void* rule_scratch, const void* record, void* stream)
 {
     return guarded("lbm_ibm_adjoint", [&] {
         require(n_rules == 0 || (rule_target && rule_uslot && rule_ustart && rule_urule && rule_ucoeff && rule_scratch),
@@ -513,14 +627,27 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
             using L = decltype(Lt);
             using T = decltype(Tt);
-            k_ibm_adj<T, L::D><<<1, IBM_ADJ_THREADS, 0, as_stream(stream)>>>(
-                static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), m,
-                nodes, n_nodes, inverse, static_cast<const T*>(weights), static_cast<const T*>(areas), n_markers,
-                spread_start, spread_entry, iterations, T(0.5 * fx), T(0.5 * fy), T(0.5 * fz), (uint32_t)n_total,
-                static_cast<const T*>(nu), body_weight, static_cast<T*>(lam), tslot, tstart, trow,
-                static_cast<const T*>(tcoeff), n_tslots, grad_w, static_cast<T*>(scratch), rule_target, n_rules,
-                rule_uslot, rule_ustart, rule_urule, static_cast<const T*>(rule_ucoeff), n_rule_unique,
-                static_cast<T*>(rule_scratch));
+            const size_t limit = lds_limit();
+            const size_t lds_tables = adj_lds_plan(L::D, n_nodes, n_markers, iterations, sizeof(T), 0).total;
+            const size_t lds_rules = adj_lds_plan(L::D, n_nodes, n_markers, iterations, sizeof(T), n_rules).total;
+            const int rules_in_lds = n_rules > 0 && lds_rules <= limit ? n_rules : 0;
+            const size_t lds = rules_in_lds ? lds_rules : lds_tables;
+            auto go = [&](auto staged) {
+                constexpr bool ST = decltype(staged)::value;
+                const int rows = (L::D + 1) * n_nodes;
+                const int threads = std::max(256, std::min(ibm_adj_max_threads(L::D), 64 * ((rows + 63) / 64)));
+                if (ST) allow_lds(k_ibm_adj<T, L::D, ST>, lds);
+                k_ibm_adj<T, L::D, ST><<<1, threads, ST ? lds : 0, as_stream(stream)>>>(
+                    static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), m,
+                    nodes, n_nodes, inverse, static_cast<const T*>(weights), static_cast<const T*>(areas), n_markers,
+                    spread_start, spread_entry, iterations, T(0.5 * fx), T(0.5 * fy), T(0.5 * fz),
+                    (uint32_t)n_total, static_cast<const T*>(nu), body_weight, static_cast<T*>(lam), tslot, tstart,
+                    trow, static_cast<const T*>(tcoeff), n_tslots, grad_w, static_cast<T*>(scratch), rule_target,
+                    n_rules, rule_uslot, rule_ustart, rule_urule, static_cast<const T*>(rule_ucoeff), n_rule_unique,
+                    static_cast<T*>(rule_scratch), ST ? rules_in_lds : 0, static_cast<const T*>(record));
+            };
+            if (lds_tables <= limit) go(std::true_type{});
+            else go(std::false_type{});
             check_launch("IBM adjoint");
         });
     });

@@ -1,0 +1,90 @@
+// Evaluation of one row of a compiled affine set: constant[r] + sum_t coeff[t][r] F[slot[t][r]].
+//
+// Term arrays are term-major with row stride `stride` (slot[t * stride + r]). A row's sum is
+// NumPy's pairwise summation over its m padded terms (8 interleaved partial sums when
+// m >= 8, a plain running sum from 0 otherwise; numpy/_core/src/umath/loops_utils.h.src,
+// pairwise_sum for n <= PW_BLOCKSIZE), the order in which the reference evaluates
+// (const + (coeff * f[slots]).sum(axis=1)) (src/lbm/link_rules.py:269-272), so a row
+// gives the same bits as the NumPy oracle.
+//
+// Terms are fetched eight at a time: the coefficient and slot loads of a chunk are issued
+// together, then the eight population gathers, so a row of m terms costs about
+// 2 * ceil(m / 8) memory latencies instead of 2 m. A zero coefficient contributes an exact
+// zero whatever the population holds; its gather reads element 0 instead of the padded slot.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lbm {
+
+template <class T>
+__device__ __forceinline__ void affine_terms(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                             const T* __restrict__ coeff, int64_t stride, int64_t r, int p0,
+                                             int count, T (&term)[8])
+{
+    T a[8];
+    int64_t s[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k < count) {
+            const int64_t e = (int64_t)(p0 + k) * stride + r;
+            a[k] = coeff[e];
+            s[k] = slot[e];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if (k < count) {
+            const bool zero = a[k] == T(0);
+            const T v = F[zero ? 0 : s[k]];
+            term[k] = zero ? T(0) : a[k] * v;
+        }
+    }
+}
+
+// Pairwise sum of the m terms of row r (no constant).
+template <class T>
+__device__ __forceinline__ T affine_sum(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                        const T* __restrict__ coeff, int64_t stride, int64_t r, int m)
+{
+    T t[8];
+    if (m < 8) {
+        affine_terms(F, slot, coeff, stride, r, 0, m, t);
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < m) s = s + t[k];
+        return s;
+    }
+    T acc[8];
+    affine_terms(F, slot, coeff, stride, r, 0, 8, acc);
+    const int lim = m - (m % 8);
+    int p = 8;
+    for (; p < lim; p += 8) {
+        affine_terms(F, slot, coeff, stride, r, p, 8, t);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] = acc[k] + t[k];
+    }
+    T s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    if (p < m) {
+        const int rest = m - p;
+        affine_terms(F, slot, coeff, stride, r, p, rest, t);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < rest) s = s + t[k];
+    }
+    return s;
+}
+
+template <class T>
+__device__ __forceinline__ T affine_row(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                        const T* __restrict__ coeff, const T* __restrict__ constant,
+                                        int64_t stride, int m, int64_t r)
+{
+    const T c = constant[r];
+    return c + affine_sum(F, slot, coeff, stride, r, m);
+}
+
+}  // namespace lbm

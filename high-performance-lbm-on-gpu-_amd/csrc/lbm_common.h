@@ -105,4 +105,23 @@ inline dim3 grid_of(const Grid& g, dim3 block)
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// LDS bytes one workgroup may allocate on the current device (160 KiB on gfx950).
+inline size_t lds_limit()
+{
+    int dev = 0, bytes = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "LDS size query");
+    return (size_t)bytes;
+}
+
+// Allow a kernel more than the default 64 KiB of dynamic LDS before launching it with `bytes`.
+template <class K>
+inline void allow_lds(K* kernel, size_t bytes)
+{
+    if (bytes > 64 * 1024)
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes),
+                  "hipFuncSetAttribute(LDS)");
+}
+
 }  // namespace lbm

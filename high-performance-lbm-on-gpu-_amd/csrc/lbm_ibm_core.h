@@ -1,0 +1,258 @@
+// Multi-direct immersed-boundary forcing (Roma kernel), the part shared by the forward IBM
+// kernel and its adjoint: the layout of the intermediates and the iteration phases.
+//
+// The reference evaluates the forcing as array operations (src/lbm/ibm.py:189-214,
+// ImmersedForcing.force_field). Here one workgroup runs it with barriers between phases:
+//
+//   P0  moments of the streamed populations at the unique stencil nodes (affine rows)
+//   P1  rho = 1 + drho,  u* = (rho u + F0/2) / rho,  force = 0
+//   P2  rho_l = sum_s w_ls rho(x_ls)                         (per marker)
+//   per iteration:
+//   P3  u = u* + force / (2 rho)                             (per node)
+//   P4  U_l = sum_s w_ls u(x_ls),  F_l = 2 rho_l (0 - U_l) dA_l
+//   P5  total += sum_l F_l;  force(x) += sum over entries touching x of w_ls F_l
+//
+// Summation orders are the ones NumPy uses for the reference's expressions:
+// pairwise over the padded terms of a moment row and over the stencil for
+// rho_l (contiguous row sums), but sequential over the stencil for U_l and over
+// the markers for the total, because u[:, inverse] is laid out with the
+// component axis innermost and NumPy then reduces those axes as outer loops;
+// the spread accumulates in float64 in flat (marker, stencil) order from 0 and
+// is added to the float64-widened force, as numpy.bincount does. Each node
+// walks its own precomputed entry list, so the result is deterministic
+// without atomics.
+//
+// Layout of the intermediates (nn stencil nodes, nm markers, `slots` iteration slots, d
+// components), which is also the per-step record the adjoint reads back:
+//   moments (d+1) nn | rho nn | ustar d nn | rho_m nm |
+//   force (slots+1) d nn   (force before iteration it, and after the last)
+//   u slots d nn | U slots d nm | Fm slots d nm
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace lbm {
+
+// NumPy's pairwise summation of term(0..m-1) for m <= 128.
+template <class T, class Term>
+__device__ __forceinline__ T pairwise_block(int m, Term term)
+{
+    if (m < 8) {
+        T s = T(0);
+        for (int p = 0; p < m; ++p) s = s + term(p);
+        return s;
+    }
+    T r0 = term(0), r1 = term(1), r2 = term(2), r3 = term(3);
+    T r4 = term(4), r5 = term(5), r6 = term(6), r7 = term(7);
+    const int lim = m - (m % 8);
+    int p = 8;
+    for (; p < lim; p += 8) {
+        r0 = r0 + term(p + 0);
+        r1 = r1 + term(p + 1);
+        r2 = r2 + term(p + 2);
+        r3 = r3 + term(p + 3);
+        r4 = r4 + term(p + 4);
+        r5 = r5 + term(p + 5);
+        r6 = r6 + term(p + 6);
+        r7 = r7 + term(p + 7);
+    }
+    T s = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+    for (; p < m; ++p) s = s + term(p);
+    return s;
+}
+
+// Sequential sum term(0) + term(1) + ..., the order NumPy uses when the reduced
+// axis is an outer loop of the reduction. Terms are fetched eight at a time before
+// they are added in order, so a long sum costs one memory latency per eight terms.
+template <class T, class Term>
+__device__ __forceinline__ T sequential_sum(int n, Term term)
+{
+    T s = term(0);
+    int p = 1;
+    for (; p + 8 <= n; p += 8) {
+        T v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = term(p + k);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s = s + v[k];
+    }
+    for (; p < n; ++p) s = s + term(p);
+    return s;
+}
+
+// Copies of several tables at once, in 4-byte words: each lane issues the loads of every
+// table for its next four indices before it stores any, so staging N tables costs about
+// one memory latency instead of N.
+struct WordCopy {
+    uint32_t* dst;
+    const uint32_t* src;
+    size_t words;
+};
+
+template <class T, class U>
+__device__ __forceinline__ WordCopy word_copy(T* dst, const U* src, size_t count)
+{
+    static_assert(sizeof(T) == sizeof(U) && sizeof(T) % 4 == 0, "word copies of equal 4-byte multiples");
+    return WordCopy{reinterpret_cast<uint32_t*>(dst), reinterpret_cast<const uint32_t*>(src),
+                    count * (sizeof(T) / 4)};
+}
+
+template <int N>
+__device__ __forceinline__ void copy_tables(const WordCopy (&c)[N], int tid, int nthreads)
+{
+    constexpr int K = 4;
+    size_t most = 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) most = c[j].words > most ? c[j].words : most;
+    const size_t step = (size_t)nthreads;
+    for (size_t i = tid; i < most; i += K * step) {
+        uint32_t v[N][K];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (i + k * step < c[j].words) v[j][k] = c[j].src[i + k * step];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (i + k * step < c[j].words) c[j].dst[i + k * step] = v[j][k];
+    }
+}
+
+template <class T>
+struct IbmForward {
+    T *moments, *rho, *ustar, *rho_m;
+    T *force, *u, *U, *Fm;
+};
+
+template <class T>
+__host__ __device__ inline size_t ibm_forward_layout(T* base, int d, int nn, int nm, int slots, IbmForward<T>* s)
+{
+    size_t o = 0;
+    IbmForward<T> dummy;
+    IbmForward<T>& v = s ? *s : dummy;
+    auto take = [&](T** p, size_t count) {
+        *p = base + o;
+        o += count;
+    };
+    take(&v.moments, (size_t)(d + 1) * nn);
+    take(&v.rho, nn);
+    take(&v.ustar, (size_t)d * nn);
+    take(&v.rho_m, nm);
+    take(&v.force, (size_t)(slots + 1) * d * nn);
+    take(&v.u, (size_t)slots * d * nn);
+    take(&v.U, (size_t)slots * d * nm);
+    take(&v.Fm, (size_t)slots * d * nm);
+    return o;
+}
+
+__host__ __device__ inline size_t ibm_record_elems(int d, int nn, int nm, int slots)
+{
+    return ibm_forward_layout<float>(nullptr, d, nn, nm, slots, nullptr);
+}
+
+// Stencil tables of the body: inverse[nm][S] (stencil entry -> node), weights[nm][S],
+// areas[nm], and per node the increasing list spread_entry[spread_start[n] ..
+// spread_start[n + 1]) of flat (marker, stencil) entries that touch it.
+template <class T>
+struct IbmTables {
+    const int* inverse;
+    const T* weights;
+    const T* areas;
+    const int* spread_start;
+    const int* spread_entry;
+};
+
+// P1..P5 after P0 has filled s.moments (and a barrier). With every_slot, iteration it
+// uses slot it of u, U, Fm and reads force slot it, writing slot it + 1; otherwise all
+// iterations share slot 0 and the force update is in place. `total` (d values, may be
+// null) accumulates the marker forces; its lanes are the last d threads of the group.
+// Ends with a barrier; the final force is slot (every_slot ? iterations : 0).
+template <class T, int D>
+__device__ __forceinline__ void ibm_iterate(const IbmForward<T> s, const IbmTables<T> tb, int nn, int nm,
+                                            int iterations, bool every_slot, const T (&half_force)[3],
+                                            T* total, int tid, int nthreads)
+{
+    constexpr int S = D == 2 ? 9 : 27;
+    const size_t node_stride = every_slot ? (size_t)D * nn : 0;
+    const size_t marker_stride = every_slot ? (size_t)D * nm : 0;
+    // P1
+    for (int n = tid; n < nn; n += nthreads) {
+        const T rho = T(1) + s.moments[n];
+        s.rho[n] = rho;
+        for (int a = 0; a < D; ++a) {
+            s.ustar[(size_t)a * nn + n] = (s.moments[(size_t)(1 + a) * nn + n] + half_force[a]) / rho;
+            s.force[(size_t)a * nn + n] = T(0);
+        }
+    }
+    if (total && tid < D) total[tid] = T(0);
+    __syncthreads();
+    // P2
+    for (int l = tid; l < nm; l += nthreads) {
+        const int* inv = tb.inverse + (size_t)l * S;
+        const T* w = tb.weights + (size_t)l * S;
+        s.rho_m[l] = pairwise_block<T>(S, [&](int p) { return w[p] * s.rho[inv[p]]; });
+    }
+    __syncthreads();
+    const int total_lane = tid - (nthreads - D);
+    for (int it = 0; it < iterations; ++it) {
+        const T* fo = s.force + it * node_stride;
+        T* fn = s.force + (it + 1) * node_stride;
+        T* u = s.u + it * node_stride;
+        T* U = s.U + it * marker_stride;
+        T* Fm = s.Fm + it * marker_stride;
+        // P3
+        for (int n = tid; n < nn; n += nthreads) {
+            const T two_rho = T(2) * s.rho[n];
+            for (int a = 0; a < D; ++a)
+                u[(size_t)a * nn + n] = s.ustar[(size_t)a * nn + n] + fo[(size_t)a * nn + n] / two_rho;
+        }
+        __syncthreads();
+        // P4
+        for (int l = tid; l < nm; l += nthreads) {
+            int inv[S];
+            T w[S];
+#pragma unroll
+            for (int p = 0; p < S; ++p) {
+                inv[p] = tb.inverse[(size_t)l * S + p];
+                w[p] = tb.weights[(size_t)l * S + p];
+            }
+            const T two_rho_m = T(2) * s.rho_m[l];
+            const T area = tb.areas[l];
+            for (int a = 0; a < D; ++a) {
+                const T* ua = u + (size_t)a * nn;
+                const T um = sequential_sum<T>(S, [&](int p) { return w[p] * ua[inv[p]]; });
+                U[(size_t)a * nm + l] = um;
+                Fm[(size_t)a * nm + l] = (two_rho_m * (T(0) - um)) * area;
+            }
+        }
+        __syncthreads();
+        // P5: marker totals and the deterministic spread (all components per entry walk).
+        if (total && total_lane >= 0) {
+            const T* fa = Fm + (size_t)total_lane * nm;
+            total[total_lane] = total[total_lane] + sequential_sum<T>(nm, [&](int l) { return fa[l]; });
+        }
+        for (int n = tid; n < nn; n += nthreads) {
+            const int begin = tb.spread_start[n], end = tb.spread_start[n + 1];
+            double acc[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc[a] = 0.0;
+            for (int e = begin; e < end; ++e) {
+                const int k = tb.spread_entry[e];
+                const int l = k / S;
+                const T w = tb.weights[k];
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc[a] += (double)(w * Fm[(size_t)a * nm + l]);
+            }
+#pragma unroll
+            for (int a = 0; a < D; ++a) fn[(size_t)a * nn + n] = T((double)fo[(size_t)a * nn + n] + acc[a]);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace lbm

@@ -14,51 +14,10 @@
 // Up to 4096 rules this is one workgroup with a barrier between the phases;
 // above that, an evaluate launch into scratch followed by a write launch.
 
+#include "lbm_affine.h"
 #include "lbm_common.h"
 
 namespace lbm {
-
-// NumPy's pairwise summation of term(0..m-1) for m <= 128
-// (numpy/_core/src/umath/loops_utils.h.src, pairwise_sum for n <= PW_BLOCKSIZE).
-template <class T, class Term>
-__device__ __forceinline__ T pairwise_sum(int m, Term term)
-{
-    if (m < 8) {
-        T s = T(0);
-        for (int p = 0; p < m; ++p) s = s + term(p);
-        return s;
-    }
-    T r0 = term(0), r1 = term(1), r2 = term(2), r3 = term(3);
-    T r4 = term(4), r5 = term(5), r6 = term(6), r7 = term(7);
-    const int lim = m - (m % 8);
-    int p = 8;
-    for (; p < lim; p += 8) {
-        r0 = r0 + term(p + 0);
-        r1 = r1 + term(p + 1);
-        r2 = r2 + term(p + 2);
-        r3 = r3 + term(p + 3);
-        r4 = r4 + term(p + 4);
-        r5 = r5 + term(p + 5);
-        r6 = r6 + term(p + 6);
-        r7 = r7 + term(p + 7);
-    }
-    T s = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
-    for (; p < m; ++p) s = s + term(p);
-    return s;
-}
-
-template <class T>
-__device__ __forceinline__ T affine_row(const T* __restrict__ F, const int64_t* __restrict__ slot,
-                                        const T* __restrict__ coeff, const T* __restrict__ constant, int64_t n,
-                                        int m, int64_t r)
-{
-    const T sum = pairwise_sum<T>(m, [&](int p) {
-        const int64_t e = (int64_t)p * n + r;
-        const T a = coeff[e];
-        return a == T(0) ? T(0) : a * F[slot[e]];
-    });
-    return constant[r] + sum;
-}
 
 template <class T>
 __global__ __launch_bounds__(256) void k_affine_values(const T* __restrict__ F, const int64_t* __restrict__ slot,
@@ -101,6 +60,11 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_rules_small(T* __restrict__ F
                                                                const int64_t* __restrict__ target, int64_t n, int m)
 {
     T v[SMALL_PER_THREAD];
+    int64_t t[SMALL_PER_THREAD];
+    static_for<0, SMALL_PER_THREAD>([&](auto K) {
+        const int64_t r = (int64_t)K * SMALL_THREADS + threadIdx.x;
+        if (r < n) t[K] = target[r];  // fetched with the terms, used after the barrier
+    });
     static_for<0, SMALL_PER_THREAD>([&](auto K) {
         const int64_t r = (int64_t)K * SMALL_THREADS + threadIdx.x;
         v[K] = r < n ? affine_row(F, slot, coeff, constant, n, m, r) : T(0);
@@ -108,7 +72,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_rules_small(T* __restrict__ F
     __syncthreads();
     static_for<0, SMALL_PER_THREAD>([&](auto K) {
         const int64_t r = (int64_t)K * SMALL_THREADS + threadIdx.x;
-        if (r < n) F[target[r]] = v[K];
+        if (r < n) F[t[K]] = v[K];
     });
 }
 

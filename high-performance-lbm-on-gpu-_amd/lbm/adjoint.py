@@ -12,9 +12,11 @@ to the uniform force:
 ``DragAdjoint.backward`` returns dL/dg (d, *shape) and dL/dX (n_markers, d), both float64.
 The adjoint runs backwards over segments of ``checkpoint_every`` steps: each segment is
 recomputed forward from its checkpoint (states kept in HBM), then swept backwards with
-``lbm_adj_pull_step`` (transposed collision Jacobian + push), ``lbm_adj_rules`` (transposed
-link rules) and ``lbm_ibm_adjoint`` (reverse multi-direct forcing). dL/dX follows from the
-accumulated dL/dw and the derivative of the Roma kernel.
+``lbm_adj_pull_step`` (transposed collision Jacobian + push) and ``lbm_ibm_adjoint``
+(transposed link rules and reverse multi-direct forcing in one launch). While HBM allows
+(``field_store_bytes``), the forward run keeps each step's IBM field and the intermediates
+of its forcing iterations, so neither the segment recompute nor the IBM adjoint re-runs the
+forcing. dL/dX follows from the accumulated dL/dw and the derivative of the Roma kernel.
 
 Segments are captured once into HIP graphs (through ``torch.cuda.CUDAGraph``) and replayed,
 so the Python loop does not set the step rate. ``DragLoss`` wraps the whole thing as a
@@ -22,6 +24,8 @@ so the Python loop does not set the step rate. ``DragLoss`` wraps the whole thin
 """
 
 from __future__ import annotations
+
+import ctypes
 
 import numpy as np
 
@@ -98,9 +102,17 @@ class DragAdjoint:
         # Each forward step records its IBM force field into its own pre-zeroed dense buffer
         # (the IBM kernel writes only the stencil nodes), so the segment recompute reuses it
         # instead of re-running the IBM; above ``field_store_bytes`` the recompute re-runs it.
-        field_bytes = steps * d * n * s._np_dtype.itemsize
+        # With the fields, each step also keeps the IBM intermediates (lbm_ibm_record_size
+        # bytes), which the IBM adjoint then reads instead of recomputing the iterations.
+        ibm = s._ibm
+        rec = ctypes.c_int64()
+        s._lib.call("lbm_ibm_record_size", s._lattice_code, s._dtype_code, ibm.n_nodes, ibm.n_markers,
+                    ibm.iterations, ctypes.byref(rec))
+        rec_elems = rec.value // s._np_dtype.itemsize
+        field_bytes = steps * (d * n + rec_elems) * s._np_dtype.itemsize
         self.store_fields = field_bytes <= field_store_bytes
         self._field_hist = torch.zeros((steps if self.store_fields else k, d, n), dtype=tdt, device=dev)
+        self._records = torch.zeros((steps if self.store_fields else 0, rec_elems), dtype=tdt, device=dev)
         self._start = torch.zeros((q, n), dtype=tdt, device=dev)
         self._drag = torch.zeros((steps, d), dtype=tdt, device=dev)
         self._drag_stage = torch.zeros((k, d), dtype=tdt, device=dev)
@@ -120,8 +132,6 @@ class DragAdjoint:
     # ------------------------------------------------------------------ set-up
 
     def _setup_ibm_adjoint(self) -> None:
-        import ctypes
-
         s, torch, dev = self.solver, self._torch, self.solver.device
         ibm = s._ibm
         mp = s._ibm_pack("pull")
@@ -192,14 +202,14 @@ class DragAdjoint:
 
     # ------------------------------------------------------------------ launches
 
-    def _ibm_forward(self, f, field, body_out) -> None:
+    def _ibm_forward(self, f, field, body_out, record=None) -> None:
         s, ibm, mp = self.solver, self.solver._ibm, self._mp
         fx, fy, fz = s._force3
         s._call("lbm_ibm_force", s._lattice_code, s._dtype_code, pointer(f), pointer(mp.slot), pointer(mp.coeff),
                 pointer(mp.constant), mp.m, pointer(s._ibm_nodes), ibm.n_nodes, pointer(s._ibm_inverse),
                 pointer(s._ibm_weights), pointer(s._ibm_areas), ibm.n_markers, pointer(s._ibm_start),
                 pointer(s._ibm_entry), ibm.iterations, fx, fy, fz, s._n, pointer(field), body_out,
-                pointer(s._ibm_scratch), s._stream)
+                pointer(s._ibm_scratch), pointer(record), s._stream)
 
     def _rules_forward(self, f) -> None:
         s = self.solver
@@ -213,23 +223,24 @@ class DragAdjoint:
         s._call("lbm_pull_step", s._lattice_code, s._dtype_code, pointer(src), pointer(dst), pointer(s._kind),
                 pointer(s._wall), pointer(field), pointer(s._body), *s._step_args())
 
-    def _forward_core(self, length: int, drag, fields) -> None:
+    def _forward_core(self, length: int, drag, fields, records=None) -> None:
         """``length`` steps from solver._f (ending there: length is even or a copy closes it).
 
         ``fields`` is the per-step field buffer (j-th step writes fields[j]) or None to use
-        the solver's single field buffer."""
+        the solver's single field buffer; ``records[j]`` (optional) receives the j-th step's
+        IBM intermediates."""
         s = self.solver
         a, b = s._f, s._f_next
         for j in range(length):
             cur, nxt = (a, b) if j % 2 == 0 else (b, a)
             field = s._field if fields is None else fields[j]
-            self._ibm_forward(cur, field, drag[j].data_ptr())
+            self._ibm_forward(cur, field, drag[j].data_ptr(), None if records is None else records[j])
             self._rules_forward(cur)
             self._pull(cur, nxt, field)
         if length % 2:
             a.copy_(b)
 
-    def _backward_core(self, length: int, weights, fields) -> None:
+    def _backward_core(self, length: int, weights, fields, records=None) -> None:
         """Recompute ``length`` steps from seg[0], then sweep them backwards.
 
         ``fields[j]`` is the IBM field of the j-th step of the segment; with recorded
@@ -265,7 +276,7 @@ class DragAdjoint:
                     pointer(self._trow), pointer(self._tcoeff), len(self._tslot), pointer(self._grad_w),
                     pointer(self._ibm_adj_scratch), pointer(r["target"]), r["n"], pointer(r["uslot"]),
                     pointer(r["ustart"]), pointer(r["urule"]), pointer(r["ucoeff"]), r["nu"], pointer(r["scratch"]),
-                    s._stream)
+                    None if records is None else pointer(records[j]), s._stream)
 
     @property
     def adjoint_block(self) -> tuple[int, int]:
@@ -305,8 +316,9 @@ class DragAdjoint:
             if store:
                 self._ckpt[seg].copy_(self.solver._f)
             fields = self._field_hist[t0 : t0 + length] if (store and self.store_fields) else None
+            records = self._records[t0 : t0 + length] if fields is not None else None
             self._replay(("fwd", length, store, t0 if fields is not None else 0),
-                         lambda: self._forward_core(length, self._drag_stage, fields))
+                         lambda: self._forward_core(length, self._drag_stage, fields, records))
             self._drag[t0 : t0 + length].copy_(self._drag_stage[:length])
 
     def forward(self) -> float:
@@ -335,8 +347,9 @@ class DragAdjoint:
             if length % 2 and self.n_segments > 1:
                 raise ValueError("segments after the first must have an even length")
             fields = self._field_hist[t0 : t0 + length] if self.store_fields else self._field_hist
+            records = self._records[t0 : t0 + length] if self.store_fields else None
             self._replay(("bwd", length, t0 if self.store_fields else 0),
-                         lambda: self._backward_core(length, self._weight_stage, fields))
+                         lambda: self._backward_core(length, self._weight_stage, fields, records))
         self._torch.cuda.synchronize()
         p = self.params
         grad_body = self._grad_body.cpu().numpy().reshape(p.d, *p.shape)

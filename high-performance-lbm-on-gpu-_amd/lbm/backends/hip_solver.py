@@ -317,7 +317,7 @@ class HipSolver(Solver):
                 pointer(mp.coeff), pointer(mp.constant), mp.m, pointer(self._ibm_nodes), ibm.n_nodes,
                 pointer(self._ibm_inverse), pointer(self._ibm_weights), pointer(self._ibm_areas), ibm.n_markers,
                 pointer(self._ibm_start), pointer(self._ibm_entry), ibm.iterations, fx, fy, fz, self._n,
-                pointer(self._field), pointer(self._body_force), pointer(self._ibm_scratch), self._stream,
+                pointer(self._field), pointer(self._body_force), pointer(self._ibm_scratch), None, self._stream,
             )
         pack = self._rule_passes.get(layout)
         if pack is not None:

@@ -157,14 +157,22 @@ LBM_API int lbm_ibm_scratch_size(int lattice, int dtype, int n_nodes, int n_mark
  * Roma stencils, areas[n_markers] the surface elements; spread_start[n_nodes + 1] /
  * spread_entry[] list, per node, the flat (marker, stencil) entries touching it in
  * increasing order. Writes field[a][nodes[n]] (d x n_total, dtype) and
- * body_force[d] (dtype) = -(sum of marker forces over all iterations).
+ * body_force[d] (dtype) = -(sum of marker forces over all iterations). record (optional,
+ * lbm_ibm_record_size bytes) receives the per-iteration intermediates for the adjoint; the
+ * outputs do not depend on it.
  * Replaces ImmersedForcing.force_field (src/lbm/ibm.py:189-214) and the field
  * scatter of CudaSolver._pre_pass (src/lbm/backends/cuda_solver.py:286-292). */
 LBM_API int lbm_ibm_force(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
                           const void* constant, int m, const int64_t* nodes, int n_nodes, const int* inverse,
                           const void* weights, const void* areas, int n_markers, const int* spread_start,
                           const int* spread_entry, int iterations, double fx, double fy, double fz,
-                          int64_t n_total, void* field, void* body_force, void* scratch, void* stream);
+                          int64_t n_total, void* field, void* body_force, void* scratch, void* record,
+                          void* stream);
+
+/* Bytes of one step's record of the forcing intermediates (optional `record` argument of
+ * lbm_ibm_force, read back by lbm_ibm_adjoint). No counterpart in the reference. Host only. */
+LBM_API int lbm_ibm_record_size(int lattice, int dtype, int n_nodes, int n_markers, int iterations,
+                                int64_t* bytes);
 
 /* Copy of `bytes` (multiple of 16, 16-byte aligned buffers) with
  * 16-byte accesses: the device-copy bandwidth probe. Replaces the cp.copyto peak
@@ -195,8 +203,9 @@ LBM_API int lbm_adj_rules(int dtype, void* lam, const int64_t* target, int n_rul
 LBM_API int lbm_ibm_adjoint_scratch_size(int lattice, int dtype, int n_nodes, int n_markers, int iterations,
                                          int64_t* bytes);
 
-/* Adjoint of lbm_ibm_force (no counterpart in the reference): recomputes the multi-direct
- * iterations from f and runs them backwards with seeds nu (dL/dh at forced nodes, d x n_total)
+/* Adjoint of lbm_ibm_force (no counterpart in the reference): takes the multi-direct
+ * intermediates from record (written by lbm_ibm_force for the same f and markers) or, when
+ * record is null, recomputes them from f, and runs the iterations backwards with seeds nu (dL/dh at forced nodes, d x n_total)
  * and body_weight[d] (dL/d body force, double). Adds the transposed moment rows into lam
  * (distinct slots tslot[n_tslots] with (trow, tcoeff) lists in tstart order) and accumulates
  * dL/dw into grad_w[n_markers][3^d] (double). When n_rules > 0 it first applies the
@@ -211,7 +220,7 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
                             int n_tslots, double* grad_w, void* scratch, const int64_t* rule_target,
                             int n_rules, const int64_t* rule_uslot, const int* rule_ustart,
                             const int* rule_urule, const void* rule_ucoeff, int n_rule_unique,
-                            void* rule_scratch, void* stream);
+                            void* rule_scratch, const void* record, void* stream);
 
 /* Size in bytes of one lattice descriptor of the batched step (no counterpart in the
  * reference, which runs one lattice per solver). Host only. */

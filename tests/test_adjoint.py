@@ -117,11 +117,11 @@ def test_torch_model_reproduces_the_oracle_drag(rules):
 # ---------------------------------------------------------------- GPU
 
 
-def make_problem(params, graphs=True):
+def make_problem(params, graphs=True, **kw):
     from lbm.adjoint import DragAdjoint
 
     return DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params),
-                       graphs=graphs)
+                       graphs=graphs, **kw)
 
 
 def random_body_field(params, seed=3, scale=1e-6):
@@ -215,3 +215,22 @@ def test_graph_replay_equals_eager_launches():
     a.set_body_force(g0)
     assert a.forward() == results[0][0]
     np.testing.assert_array_equal(a.backward()[0], results[0][1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_recorded_forcing_equals_recomputed_forcing(dtype):
+    """With the per-step IBM fields and intermediates kept (the default) and without them
+    (field_store_bytes=0: the segment recompute and the IBM adjoint re-run the forcing),
+    the loss and both gradients carry the same bits."""
+    params = case(dtype=dtype, rules=True)
+    g0 = random_body_field(params)
+    results = []
+    for store in (16 * 2**30, 0):
+        problem = make_problem(params, field_store_bytes=store)
+        assert problem.store_fields == bool(store)
+        problem.set_body_force(g0)
+        results.append((problem.forward(), *problem.backward()))
+    assert results[0][0] == results[1][0]
+    np.testing.assert_array_equal(results[0][1], results[1][1])
+    np.testing.assert_array_equal(results[0][2], results[1][2])

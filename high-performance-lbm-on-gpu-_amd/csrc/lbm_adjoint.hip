@@ -177,19 +177,24 @@ __global__ __launch_bounds__(256) void k_adj_pull(const T* __restrict__ lam_next
     }
 }
 
-// lam[slot[t]] += sum_{e in [start[t], start[t+1])} coeff[e] * x[row[e]] for every t < n.
-// The slots are distinct, so no two threads touch one. A thread takes four slots at a
-// time and issues their loads together, the first four entries of each list included;
-// longer lists finish in a loop.
-template <class T>
-__device__ __forceinline__ void scatter_rows(T* __restrict__ lam, const int64_t* __restrict__ slot,
-                                             const int* __restrict__ start, const int* __restrict__ row,
-                                             const T* __restrict__ coeff, int n, const T* x, int tid, int nthreads)
-{
-    constexpr int K = 4, E = 4;
-    for (int t0 = tid; t0 < n; t0 += K * nthreads) {
-        int64_t sl[K];
-        int b[K], end[K];
+// Transposed rows: lam[slot[t]] += sum_{e in [start[t], start[t+1])} coeff[e] * x[row[e]]
+// for every t < n. The slots are distinct, so no two threads touch one.
+//
+// A RowBatch holds, for slots t0 + k * nthreads (k < K), the slot, the list bounds and the
+// first E entries of each list. fetch() reads only the transposed tables, which no launch
+// writes, so it can run ahead of the barrier that publishes lam; apply() then costs one
+// memory latency (the old lam values) plus the LDS reads of x. Longer lists finish in a loop.
+template <class T, int K, int E = 4>
+struct RowBatch {
+    int64_t sl[K];
+    int b[K], end[K];
+    int rr[K][E];
+    T cc[K][E];
+
+    __device__ __forceinline__ void fetch(const int64_t* __restrict__ slot, const int* __restrict__ start,
+                                          const int* __restrict__ row, const T* __restrict__ coeff, int n, int t0,
+                                          int nthreads)
+    {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int t = t0 + k * nthreads;
@@ -199,21 +204,24 @@ __device__ __forceinline__ void scatter_rows(T* __restrict__ lam, const int64_t*
                 end[k] = start[t + 1];
             }
         }
-        T old[K];
-        int rr[K][E];
-        T cc[K][E];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (t0 + k * nthreads < n) {
-                old[k] = lam[sl[k]];
+        for (int k = 0; k < K; ++k)
+            if (t0 + k * nthreads < n)
 #pragma unroll
                 for (int j = 0; j < E; ++j)
                     if (b[k] + j < end[k]) {
                         rr[k][j] = row[b[k] + j];
                         cc[k][j] = coeff[b[k] + j];
                     }
-            }
-        }
+    }
+
+    __device__ __forceinline__ void apply(T* __restrict__ lam, const int* __restrict__ row,
+                                          const T* __restrict__ coeff, int n, int t0, int nthreads, const T* x)
+    {
+        T old[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (t0 + k * nthreads < n) old[k] = lam[sl[k]];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (t0 + k * nthreads < n) {
@@ -225,6 +233,19 @@ __device__ __forceinline__ void scatter_rows(T* __restrict__ lam, const int64_t*
                 lam[sl[k]] = old[k] + acc;
             }
         }
+    }
+};
+
+// All slots from `first` on (first = tid: every slot), K per lane at a time.
+template <class T, int K = 4>
+__device__ __forceinline__ void scatter_rows(T* __restrict__ lam, const int64_t* __restrict__ slot,
+                                             const int* __restrict__ start, const int* __restrict__ row,
+                                             const T* __restrict__ coeff, int n, const T* x, int first, int nthreads)
+{
+    for (int t0 = first; t0 < n; t0 += K * nthreads) {
+        RowBatch<T, K> rb;
+        rb.fetch(slot, start, row, coeff, n, t0, nthreads);
+        rb.apply(lam, row, coeff, n, t0, nthreads, x);
     }
 }
 
@@ -347,6 +368,9 @@ __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
     const bool has_rule = tid < n_rules, has_node = tid < nn;
     const int64_t target0 = has_rule ? rtarget[tid] : 0;
     const int64_t node0 = has_node ? nodes[tid] : 0;
+    // The first two transposed rule slots of this lane, fetched ahead of the barrier.
+    RowBatch<T, 2> rule_rows;
+    if (n_rules > 0) rule_rows.fetch(ruslot, rustart, rurule, rucoeff, n_runique, tid, nthreads);
     IbmAdjScratch<T> s;
     IbmTables<T> tb{inverse_g, weights_g, areas_g, start_g, entry_g};
     double* grad_w = grad_w_g;
@@ -363,23 +387,22 @@ __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
         int* st = reinterpret_cast<int*>(adj_lds + p.start);
         if (lds_rules) rule_a = reinterpret_cast<T*>(adj_lds + p.rule_a);
         for (int e = tid; e < nm * S; e += nthreads) gw[e] = 0.0;
-        const WordCopy tables[6] = {
+        const WordCopy tables[5] = {
             word_copy(inv, inverse_g, (size_t)nm * S),
             word_copy(w, weights_g, (size_t)nm * S),
             word_copy(ent, entry_g, (size_t)nm * S),
             word_copy(a, areas_g, (size_t)nm),
             word_copy(st, start_g, (size_t)nn + 1),
-            word_copy(base, record, record ? ibm_record_elems(D, nn, nm, iters) : 0),
         };
-        copy_tables(tables, tid, nthreads);
+        copy_tables(tables, word_copy(base, record, record ? ibm_record_elems(D, nn, nm, iters) : 0), tid, nthreads);
         tb = IbmTables<T>{inv, w, a, st, ent};
         grad_w = gw;
     } else {
         base = scratch;
         ibm_adj_layout<T>(scratch, D, nn, nm, iters, &s);
         if (record) {
-            const WordCopy rec[1] = {word_copy(base, record, ibm_record_elems(D, nn, nm, iters))};
-            copy_tables(rec, tid, nthreads);
+            const WordCopy none[1] = {WordCopy{nullptr, nullptr, 0}};
+            copy_tables(none, word_copy(base, record, ibm_record_elems(D, nn, nm, iters)), tid, nthreads);
         }
     }
     // First interval, all independent: the tables above, the recorded intermediates, the
@@ -416,7 +439,10 @@ __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
     }
     for (int l = tid; l < nm; l += nthreads) s.rho_m_bar[l] = T(0);
     __syncthreads();
-    if (n_rules > 0) scatter_rows(lam, ruslot, rustart, rurule, rucoeff, n_runique, rule_a, tid, nthreads);
+    if (n_rules > 0) {
+        rule_rows.apply(lam, rurule, rucoeff, n_runique, tid, nthreads, rule_a);
+        scatter_rows(lam, ruslot, rustart, rurule, rucoeff, n_runique, rule_a, tid + 2 * nthreads, nthreads);
+    }
     if (!record) {
         const int rows = (D + 1) * nn;
         const T hf[3] = {hfx, hfy, hfz};
@@ -482,12 +508,11 @@ __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
             T acc[D];
 #pragma unroll
             for (int a = 0; a < D; ++a) acc[a] = T(0);
-            for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
-                const int k = tb.spread_entry[e];
+            for_each_entry(tb.spread_start, tb.spread_entry, n, [&](int k) {
                 const T w = tb.weights[k];
 #pragma unroll
                 for (int a = 0; a < D; ++a) acc[a] += w * s.U_bar[a * nm + k / S];
-            }
+            });
             const T rho = s.rho[n];
             const T two_rho = T(2) * rho;
             T rb = s.rho_bar[n];
@@ -504,13 +529,13 @@ __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
     // rho_m = sum_s w rho(x_ls): weight gradient (per marker) and, per node, rho_bar and the
     // moment adjoints m_bar.
     for (int l = tid; l < nm; l += nthreads)
-        for (int p = 0; p < S; ++p) grad_w[l * S + p] += (double)(s.rho_m_bar[l] * s.rho[tb.inverse[l * S + p]]);
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            grad_w[l * S + p] += (double)(s.rho_m_bar[l] * s.rho[tb.inverse[l * S + p]]);
+        }
     for (int n = tid; n < nn; n += nthreads) {
         T acc = T(0);
-        for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
-            const int k = tb.spread_entry[e];
-            acc += tb.weights[k] * s.rho_m_bar[k / S];
-        }
+        for_each_entry(tb.spread_start, tb.spread_entry, n, [&](int k) { acc += tb.weights[k] * s.rho_m_bar[k / S]; });
         T rb = s.rho_bar[n] + acc;
         const T rho = s.rho[n];
         for (int a = 0; a < D; ++a) {

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(ibm_max_threads(D)) void k_ibm(const T* __restrict_
         const WordCopy tables[5] = {word_copy(inv, inverse_g, (size_t)nm * S), word_copy(w, weights_g, (size_t)nm * S),
                                     word_copy(ent, entry_g, (size_t)nm * S), word_copy(a, areas_g, (size_t)nm),
                                     word_copy(st, start_g, (size_t)nn + 1)};
-        copy_tables(tables, tid, nthreads);
+        copy_tables(tables, WordCopy{nullptr, nullptr, 0}, tid, nthreads);
         tb = IbmTables<T>{inv, w, a, st, ent};
     } else {
         base = RECORD ? record : scratch;
@@ -93,8 +93,8 @@ __global__ __launch_bounds__(ibm_max_threads(D)) void k_ibm(const T* __restrict_
         for (int a = 0; a < D; ++a) field[(size_t)a * n_total + nodes[n]] = last[(size_t)a * nn + n];
     if (tid < D) body_force[tid] = -total[tid];
     if constexpr (STAGED && RECORD) {
-        const WordCopy out[1] = {word_copy(record, base, ibm_record_elems(D, nn, nm, iterations))};
-        copy_tables(out, tid, nthreads);
+        const WordCopy none[1] = {WordCopy{nullptr, nullptr, 0}};
+        copy_tables(none, word_copy(record, base, ibm_record_elems(D, nn, nm, iterations)), tid, nthreads);
     }
 }
 
@@ -122,7 +122,9 @@ LBM_API int lbm_ibm_record_size(int lattice, int dtype, int n_nodes, int n_marke
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
             using L = decltype(Lt);
             using T = decltype(Tt);
-            *bytes = (int64_t)(ibm_record_elems(L::D, n_nodes, n_markers, iterations) * sizeof(T));
+            // rounded up to 16 bytes, so consecutive records keep 16-byte alignment
+            const size_t raw = ibm_record_elems(L::D, n_nodes, n_markers, iterations) * sizeof(T);
+            *bytes = (int64_t)((raw + 15) / 16 * 16);
         });
     });
 }

@@ -170,7 +170,8 @@ LBM_API int lbm_ibm_force(int lattice, int dtype, const void* f, const int64_t* 
                           void* stream);
 
 /* Bytes of one step's record of the forcing intermediates (optional `record` argument of
- * lbm_ibm_force, read back by lbm_ibm_adjoint). No counterpart in the reference. Host only. */
+ * lbm_ibm_force, read back by lbm_ibm_adjoint), a multiple of 16; records at 16-byte aligned
+ * addresses are copied with 16-byte accesses. No counterpart in the reference. Host only. */
 LBM_API int lbm_ibm_record_size(int lattice, int dtype, int n_nodes, int n_markers, int iterations,
                                 int64_t* bytes);
 

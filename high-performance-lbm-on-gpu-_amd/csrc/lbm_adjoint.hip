@@ -15,6 +15,9 @@
 #include "lbm_affine.h"
 #include "lbm_common.h"
 #include "lbm_ibm_core.h"
+#include "lbm_step.h"
+
+#include <initializer_list>
 
 namespace lbm {
 
@@ -112,17 +115,15 @@ __device__ __forceinline__ uint32_t nbr(int x, int y, int z, const Grid& g)
     return ((uint32_t)zz * (uint32_t)g.ny + (uint32_t)yy) * (uint32_t)g.nx + (uint32_t)xx;
 }
 
+// The adjoint update of node (x, y, z) (inside the grid).
 template <class L, class T, ForceMode FM, bool MASK>
-__global__ __launch_bounds__(256) void k_adj_pull(const T* __restrict__ lam_next, T* __restrict__ lam_out,
-                                                  const T* __restrict__ f, const uint8_t* __restrict__ kind,
-                                                  const T* __restrict__ wall_u, const T* __restrict__ field,
-                                                  const T* __restrict__ body, double* __restrict__ grad_body,
-                                                  T* __restrict__ nu_forced, Grid g, Relaxation<T> r)
+__device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T* __restrict__ lam_out,
+                                              const T* __restrict__ f, const uint8_t* __restrict__ kind,
+                                              const T* __restrict__ wall_u, const T* __restrict__ field,
+                                              const T* __restrict__ body, double* __restrict__ grad_body,
+                                              T* __restrict__ nu_forced, const Grid& g, const Relaxation<T>& r, int x,
+                                              int y, int z)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y * blockDim.y + threadIdx.y;
-    const int z = blockIdx.z;
-    if (x >= g.nx || y >= g.ny) return;
     const uint32_t node = ((uint32_t)z * (uint32_t)g.ny + (uint32_t)y) * (uint32_t)g.nx + (uint32_t)x;
     const uint8_t k = MASK ? kind[node] : 0;
     if (MASK && (k & KIND_SOLID)) return;
@@ -175,6 +176,49 @@ __global__ __launch_bounds__(256) void k_adj_pull(const T* __restrict__ lam_next
         nu_forced[(size_t)g.n + node] = Fby;
         if constexpr (L::D == 3) nu_forced[2 * (size_t)g.n + node] = Fbz;
     }
+}
+
+template <class L, class T, ForceMode FM, bool MASK>
+__global__ __launch_bounds__(256) void k_adj_pull(const T* __restrict__ lam_next, T* __restrict__ lam_out,
+                                                  const T* __restrict__ f, const uint8_t* __restrict__ kind,
+                                                  const T* __restrict__ wall_u, const T* __restrict__ field,
+                                                  const T* __restrict__ body, double* __restrict__ grad_body,
+                                                  T* __restrict__ nu_forced, Grid g, Relaxation<T> r)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x < g.nx && y < g.ny)
+        adj_pull_node<L, T, FM, MASK>(lam_next, lam_out, f, kind, wall_u, field, body, grad_body, nu_forced, g, r, x,
+                                      y, blockIdx.z);
+}
+
+// The adjoint step plus an independent link-rule pre-pass on another state buffer, in one
+// launch: the grid has one extra z layer whose block (0, 0) runs the pre-pass
+// (rules_in_one_group, at most 4 rules per lane). Used to overlap the forward recompute
+// of one checkpoint segment with the backward sweep of the next.
+constexpr int ADJ_RULES_PER_LANE = 4;
+
+template <class L, class T, ForceMode FM, bool MASK>
+__global__ __launch_bounds__(256) void k_adj_pull_rules(const T* __restrict__ lam_next, T* __restrict__ lam_out,
+                                                        const T* __restrict__ f, const uint8_t* __restrict__ kind,
+                                                        const T* __restrict__ wall_u, const T* __restrict__ field,
+                                                        const T* __restrict__ body, double* __restrict__ grad_body,
+                                                        T* __restrict__ nu_forced, Grid g, Relaxation<T> r,
+                                                        T* __restrict__ rf, const int64_t* __restrict__ rslot,
+                                                        const T* __restrict__ rcoeff, const T* __restrict__ rconstant,
+                                                        const int64_t* __restrict__ rtarget, int64_t n_rules, int rm)
+{
+    if ((int)blockIdx.z == g.nz) {
+        if (blockIdx.x == 0 && blockIdx.y == 0)
+            rules_in_one_group<T, ADJ_RULES_PER_LANE>(rf, rslot, rcoeff, rconstant, rtarget, n_rules, rm,
+                                                      threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
+        return;
+    }
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (x < g.nx && y < g.ny)
+        adj_pull_node<L, T, FM, MASK>(lam_next, lam_out, f, kind, wall_u, field, body, grad_body, nu_forced, g, r, x,
+                                      y, blockIdx.z);
 }
 
 // Transposed rows: lam[slot[t]] += sum_{e in [start[t], start[t+1])} coeff[e] * x[row[e]]
@@ -347,7 +391,7 @@ __host__ __device__ inline AdjLdsPlan adj_lds_plan(int d, int nn, int nm, int it
 // grad_w[l][s] += dL/dw_ls in float64. With n_rules > 0 the transposed link rules are
 // applied first (they act on slots the IBM rows never read).
 template <class T, int D, bool STAGED>
-__global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
+__device__ __forceinline__ void ibm_adj_group(
     const T* __restrict__ F, const int64_t* __restrict__ slot, const T* __restrict__ coeff,
     const T* __restrict__ constant, int m, const int64_t* __restrict__ nodes, int nn, const int* __restrict__ inverse_g,
     const T* __restrict__ weights_g, const T* __restrict__ areas_g, int nm, const int* __restrict__ start_g,
@@ -357,12 +401,10 @@ __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
     const T* __restrict__ tcoeff, int n_tslots, double* __restrict__ grad_w_g, T* __restrict__ scratch,
     const int64_t* __restrict__ rtarget, int n_rules, const int64_t* __restrict__ ruslot,
     const int* __restrict__ rustart, const int* __restrict__ rurule, const T* __restrict__ rucoeff, int n_runique,
-    T* __restrict__ rule_a, int lds_rules, const T* __restrict__ record)
+    T* __restrict__ rule_a, int lds_rules, const T* __restrict__ record, int tid, int nthreads)
 {
     constexpr int S = D == 2 ? 9 : 27;
     extern __shared__ __align__(16) unsigned char adj_lds[];
-    const int tid = threadIdx.x;
-    const int nthreads = blockDim.x;
     // Index loads of the first interval go out before the table copies, so their second
     // level (lam at the rule targets, nu at the nodes) costs one more latency, not three.
     const bool has_rule = tid < n_rules, has_node = tid < nn;
@@ -508,11 +550,12 @@ __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
             T acc[D];
 #pragma unroll
             for (int a = 0; a < D; ++a) acc[a] = T(0);
-            for_each_entry(tb.spread_start, tb.spread_entry, n, [&](int k) {
+            for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
+                const int k = tb.spread_entry[e];
                 const T w = tb.weights[k];
 #pragma unroll
                 for (int a = 0; a < D; ++a) acc[a] += w * s.U_bar[a * nm + k / S];
-            });
+            }
             const T rho = s.rho[n];
             const T two_rho = T(2) * rho;
             T rb = s.rho_bar[n];
@@ -535,7 +578,10 @@ __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
         }
     for (int n = tid; n < nn; n += nthreads) {
         T acc = T(0);
-        for_each_entry(tb.spread_start, tb.spread_entry, n, [&](int k) { acc += tb.weights[k] * s.rho_m_bar[k / S]; });
+        for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
+            const int k = tb.spread_entry[e];
+            acc += tb.weights[k] * s.rho_m_bar[k / S];
+        }
         T rb = s.rho_bar[n] + acc;
         const T rho = s.rho[n];
         for (int a = 0; a < D; ++a) {
@@ -550,6 +596,116 @@ __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(
     if constexpr (STAGED) {
         for (int e = tid; e < nm * S; e += nthreads) grad_w_g[e] += grad_w[e];
     }
+}
+
+#define LBM_IBM_ADJ_PARAMS                                                                                              \
+    const T *__restrict__ F, const int64_t *__restrict__ slot, const T *__restrict__ coeff,                             \
+        const T *__restrict__ constant, int m, const int64_t *__restrict__ nodes, int nn,                               \
+        const int *__restrict__ inverse_g, const T *__restrict__ weights_g, const T *__restrict__ areas_g, int nm,      \
+        const int *__restrict__ start_g, const int *__restrict__ entry_g, int iters, T hfx, T hfy, T hfz,               \
+        uint32_t n_total, const T *__restrict__ nu, const double *__restrict__ body_weight, T *__restrict__ lam,         \
+        const int64_t *__restrict__ tslot, const int *__restrict__ tstart, const int *__restrict__ trow,                \
+        const T *__restrict__ tcoeff, int n_tslots, double *__restrict__ grad_w_g, T *__restrict__ scratch,             \
+        const int64_t *__restrict__ rtarget, int n_rules, const int64_t *__restrict__ ruslot,                           \
+        const int *__restrict__ rustart, const int *__restrict__ rurule, const T *__restrict__ rucoeff, int n_runique,  \
+        T *__restrict__ rule_a, int lds_rules, const T *__restrict__ record
+#define LBM_IBM_ADJ_ARGS                                                                                                \
+    F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g, nm, start_g, entry_g, iters, hfx, hfy, hfz,  \
+        n_total, nu, body_weight, lam, tslot, tstart, trow, tcoeff, n_tslots, grad_w_g, scratch, rtarget, n_rules,      \
+        ruslot, rustart, rurule, rucoeff, n_runique, rule_a, lds_rules, record
+
+template <class T, int D, bool STAGED>
+__global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(LBM_IBM_ADJ_PARAMS)
+{
+    ibm_adj_group<T, D, STAGED>(LBM_IBM_ADJ_ARGS, threadIdx.x, blockDim.x);
+}
+
+// The IBM adjoint (block 0) and an independent pull step on other buffers (blocks 1..,
+// tiles of pbx x pby nodes, lanes beyond pbx * pby idle) in one launch. Used to overlap the
+// forward recompute of one checkpoint segment with the backward sweep of the next; the
+// long single-group IBM adjoint is dispatched first.
+template <class L, class T, ForceMode FM, bool MASK, bool STAGED>
+__global__ __launch_bounds__(ibm_adj_max_threads(L::D)) void k_ibm_adj_pull(
+    LBM_IBM_ADJ_PARAMS, const T* __restrict__ src, T* __restrict__ dst, const uint8_t* __restrict__ kind,
+    const T* __restrict__ wall_u, const T* __restrict__ field, const T* __restrict__ body, Grid g, Relaxation<T> r,
+    int pbx, int pby, int tiles_x, int tiles_y)
+{
+    if (blockIdx.x == 0) {
+        ibm_adj_group<T, L::D, STAGED>(LBM_IBM_ADJ_ARGS, threadIdx.x, blockDim.x);
+        return;
+    }
+    const int t = blockIdx.x - 1;
+    const int lane = threadIdx.x;
+    if (lane >= pbx * pby) return;
+    const int tx = t % tiles_x, rest = t / tiles_x;
+    Coord c{(tx * pbx) + lane % pbx, (rest % tiles_y) * pby + lane / pbx, rest / tiles_y};
+    if (c.x < g.nx && c.y < g.ny) pull_update<L, T, FM, MASK>(src, dst, kind, wall_u, field, body, g, r, c);
+}
+
+#undef LBM_IBM_ADJ_PARAMS
+#undef LBM_IBM_ADJ_ARGS
+
+// Call go(FM, MASK) for the force and mask modes of the C ABI (field forcing needs the kind
+// mask).
+template <class Go>
+void dispatch_force_mask(int force_mode, int mask_mode, Go&& go)
+{
+    using U = std::integral_constant<ForceMode, ForceMode::UNIFORM>;
+    using Fi = std::integral_constant<ForceMode, ForceMode::FIELD>;
+    using De = std::integral_constant<ForceMode, ForceMode::DENSE>;
+    using M0 = std::integral_constant<bool, false>;
+    using M1 = std::integral_constant<bool, true>;
+    if (force_mode == 0) {
+        if (mask_mode) go(U{}, M1{});
+        else go(U{}, M0{});
+    } else if (force_mode == 1) {
+        go(Fi{}, M1{});
+    } else {
+        if (mask_mode) go(De{}, M1{});
+        else go(De{}, M0{});
+    }
+}
+
+void check_step_args(int force_mode, int mask_mode, const void* kind, const void* wall_u, const void* force_field,
+                     const void* body_field)
+{
+    require(mask_mode == 0 || (kind && wall_u), "mask_mode 1 needs kind and wall velocity");
+    require(force_mode >= 0 && force_mode <= 2, "unknown force mode");
+    require(force_mode != 1 || (force_field && mask_mode == 1), "field force mode needs the field and kind");
+    require(force_mode != 2 || body_field, "dense force mode needs a body field");
+}
+
+// Launch shape of the IBM adjoint: lanes, LDS bytes, staging and rule-adjoint placement.
+struct IbmAdjShape {
+    int threads;
+    size_t lds;
+    bool staged;
+    int rules_in_lds;
+};
+
+template <class L, class T>
+IbmAdjShape ibm_adj_shape(int n_nodes, int n_markers, int iterations, int n_rules)
+{
+    const size_t limit = lds_limit();
+    const size_t lds_tables = adj_lds_plan(L::D, n_nodes, n_markers, iterations, sizeof(T), 0).total;
+    const size_t lds_rules = adj_lds_plan(L::D, n_nodes, n_markers, iterations, sizeof(T), n_rules).total;
+    IbmAdjShape sh;
+    sh.staged = lds_tables <= limit;
+    sh.rules_in_lds = sh.staged && n_rules > 0 && lds_rules <= limit ? n_rules : 0;
+    sh.lds = sh.staged ? (sh.rules_in_lds ? lds_rules : lds_tables) : 0;
+    const int rows = (L::D + 1) * n_nodes;
+    sh.threads = std::max(256, std::min(ibm_adj_max_threads(L::D), 64 * ((rows + 63) / 64)));
+    return sh;
+}
+
+void check_ibm_adj_args(int m, int n_nodes, int n_markers, int iterations, int64_t n_total, int n_rules,
+                        std::initializer_list<const void*> required, std::initializer_list<const void*> rule_ptrs)
+{
+    require(n_nodes > 0 && n_markers > 0 && iterations >= 1 && m >= 1, "bad IBM sizes");
+    require(n_total > 0 && n_total < (1ll << 31), "node count out of range");
+    for (const void* p : required) require(p != nullptr, "null pointer");
+    if (n_rules > 0)
+        for (const void* p : rule_ptrs) require(p != nullptr, "null rule pointer");
 }
 
 }  // namespace lbm
@@ -568,10 +724,7 @@ LBM_API int lbm_adj_pull_step(int lattice, int dtype, const void* lam_next, void
         const Grid g = make_grid(lattice, nx, ny, nz);
         require(bx > 0 && by > 0 && bx * by <= 256, "adjoint block must be positive with at most 256 threads");
         require(lam_next && lam_out && f && lam_next != lam_out, "adjoint buffers must be distinct and non-null");
-        require(mask_mode == 0 || (kind && wall_u), "mask_mode 1 needs kind and wall velocity");
-        require(force_mode >= 0 && force_mode <= 2, "unknown force mode");
-        require(force_mode != 1 || (force_field && mask_mode == 1), "field force mode needs the field and kind");
-        require(force_mode != 2 || body_field, "dense force mode needs a body field");
+        check_step_args(force_mode, mask_mode, kind, wall_u, force_field, body_field);
         const double force[3] = {fx, fy, fz};
         const dim3 block(bx, by, 1);
         const dim3 grid = grid_of(g, block);
@@ -579,25 +732,53 @@ LBM_API int lbm_adj_pull_step(int lattice, int dtype, const void* lam_next, void
             using L = decltype(Lt);
             using T = decltype(Tt);
             const Relaxation<T> r = make_relaxation<L, T>(omega_plus, omega_minus, force);
-            auto go = [&](auto fm, auto mk) {
+            dispatch_force_mask(force_mode, mask_mode, [&](auto fm, auto mk) {
                 k_adj_pull<L, T, decltype(fm)::value, decltype(mk)::value><<<grid, block, 0, as_stream(stream)>>>(
                     static_cast<const T*>(lam_next), static_cast<T*>(lam_out), static_cast<const T*>(f), kind,
                     static_cast<const T*>(wall_u), static_cast<const T*>(force_field),
                     static_cast<const T*>(body_field), grad_body, static_cast<T*>(nu_forced), g, r);
-            };
-            using U = std::integral_constant<ForceMode, ForceMode::UNIFORM>;
-            using Fi = std::integral_constant<ForceMode, ForceMode::FIELD>;
-            using De = std::integral_constant<ForceMode, ForceMode::DENSE>;
-            using M0 = std::integral_constant<bool, false>;
-            using M1 = std::integral_constant<bool, true>;
-            if (force_mode == 0) {
-                if (mask_mode) go(U{}, M1{}); else go(U{}, M0{});
-            } else if (force_mode == 1) {
-                go(Fi{}, M1{});
-            } else {
-                if (mask_mode) go(De{}, M1{}); else go(De{}, M0{});
-            }
+            });
             check_launch("adjoint step");
+        });
+    });
+}
+
+LBM_API int lbm_adj_pull_step_rules(int lattice, int dtype, const void* lam_next, void* lam_out, const void* f,
+                                    const uint8_t* kind, const void* wall_u, const void* force_field,
+                                    const void* body_field, int nx, int ny, int nz, double omega_plus,
+                                    double omega_minus, double fx, double fy, double fz, int force_mode,
+                                    int mask_mode, double* grad_body, void* nu_forced, int bx, int by, void* rule_f,
+                                    const int64_t* rule_slot, const void* rule_coeff, const void* rule_constant,
+                                    const int64_t* rule_target, int64_t n_rules, int rule_m, void* stream)
+{
+    return guarded("lbm_adj_pull_step_rules", [&] {
+        const Grid g = make_grid(lattice, nx, ny, nz);
+        require(bx > 0 && by > 0 && bx * by <= 256, "adjoint block must be positive with at most 256 threads");
+        require(lam_next && lam_out && f && lam_next != lam_out, "adjoint buffers must be distinct and non-null");
+        check_step_args(force_mode, mask_mode, kind, wall_u, force_field, body_field);
+        require(n_rules >= 1 && n_rules <= (int64_t)ADJ_RULES_PER_LANE * bx * by,
+                "the fused pre-pass takes 1 .. 4 * bx * by rules");
+        require(rule_m >= 1 && rule_m <= 128, "rules need 1 <= m <= 128 terms");
+        require(rule_f && rule_slot && rule_coeff && rule_constant && rule_target, "null rule pointer");
+        require(rule_f != f && rule_f != lam_out, "the pre-pass state must be another buffer");
+        const double force[3] = {fx, fy, fz};
+        const dim3 block(bx, by, 1);
+        dim3 grid = grid_of(g, block);
+        grid.z += 1;  // the pre-pass layer
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            const Relaxation<T> r = make_relaxation<L, T>(omega_plus, omega_minus, force);
+            dispatch_force_mask(force_mode, mask_mode, [&](auto fm, auto mk) {
+                k_adj_pull_rules<L, T, decltype(fm)::value, decltype(mk)::value>
+                    <<<grid, block, 0, as_stream(stream)>>>(
+                        static_cast<const T*>(lam_next), static_cast<T*>(lam_out), static_cast<const T*>(f), kind,
+                        static_cast<const T*>(wall_u), static_cast<const T*>(force_field),
+                        static_cast<const T*>(body_field), grad_body, static_cast<T*>(nu_forced), g, r,
+                        static_cast<T*>(rule_f), rule_slot, static_cast<const T*>(rule_coeff),
+                        static_cast<const T*>(rule_constant), rule_target, n_rules, rule_m);
+            });
+            check_launch("adjoint step with pre-pass");
         });
     });
 }
@@ -631,6 +812,21 @@ LBM_API int lbm_ibm_adjoint_scratch_size(int lattice, int dtype, int n_nodes, in
     });
 }
 
+#define LBM_IBM_ADJ_CALL_ARGS                                                                                           \
+    static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), m, nodes, n_nodes,    \
+        inverse, static_cast<const T*>(weights), static_cast<const T*>(areas), n_markers, spread_start, spread_entry,   \
+        iterations, T(0.5 * fx), T(0.5 * fy), T(0.5 * fz), (uint32_t)n_total, static_cast<const T*>(nu), body_weight,  \
+        static_cast<T*>(lam), tslot, tstart, trow, static_cast<const T*>(tcoeff), n_tslots, grad_w,                     \
+        static_cast<T*>(scratch), rule_target, n_rules, rule_uslot, rule_ustart, rule_urule,                           \
+        static_cast<const T*>(rule_ucoeff), n_rule_unique, static_cast<T*>(rule_scratch), sh.rules_in_lds,             \
+        static_cast<const T*>(record)
+
+#define LBM_IBM_ADJ_CHECK                                                                                              \
+    check_ibm_adj_args(m, n_nodes, n_markers, iterations, n_total, n_rules,                                            \
+                       {f, slot, coeff, constant, nodes, inverse, weights, areas, spread_start, spread_entry, nu,      \
+                        body_weight, lam, tslot, tstart, trow, tcoeff, grad_w, scratch},                               \
+                       {rule_target, rule_uslot, rule_ustart, rule_urule, rule_ucoeff, rule_scratch})
+
 LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
                             const void* constant, int m, const int64_t* nodes, int n_nodes, const int* inverse,
                             const void* weights, const void* areas, int n_markers, const int* spread_start,
@@ -642,40 +838,74 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
                             void* rule_scratch, const void* record, void* stream)
 {
     return guarded("lbm_ibm_adjoint", [&] {
-        require(n_rules == 0 || (rule_target && rule_uslot && rule_ustart && rule_urule && rule_ucoeff && rule_scratch),
-                "null rule pointer");
-        require(n_nodes > 0 && n_markers > 0 && iterations >= 1 && m >= 1, "bad IBM sizes");
-        require(n_total > 0 && n_total < (1ll << 31), "node count out of range");
-        require(f && slot && coeff && constant && nodes && inverse && weights && areas && spread_start &&
-                    spread_entry && nu && body_weight && lam && tslot && tstart && trow && tcoeff && grad_w && scratch,
-                "null pointer");
+        LBM_IBM_ADJ_CHECK;
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
             using L = decltype(Lt);
             using T = decltype(Tt);
-            const size_t limit = lds_limit();
-            const size_t lds_tables = adj_lds_plan(L::D, n_nodes, n_markers, iterations, sizeof(T), 0).total;
-            const size_t lds_rules = adj_lds_plan(L::D, n_nodes, n_markers, iterations, sizeof(T), n_rules).total;
-            const int rules_in_lds = n_rules > 0 && lds_rules <= limit ? n_rules : 0;
-            const size_t lds = rules_in_lds ? lds_rules : lds_tables;
+            const IbmAdjShape sh = ibm_adj_shape<L, T>(n_nodes, n_markers, iterations, n_rules);
             auto go = [&](auto staged) {
                 constexpr bool ST = decltype(staged)::value;
-                const int rows = (L::D + 1) * n_nodes;
-                const int threads = std::max(256, std::min(ibm_adj_max_threads(L::D), 64 * ((rows + 63) / 64)));
-                if (ST) allow_lds(k_ibm_adj<T, L::D, ST>, lds);
-                k_ibm_adj<T, L::D, ST><<<1, threads, ST ? lds : 0, as_stream(stream)>>>(
-                    static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), m,
-                    nodes, n_nodes, inverse, static_cast<const T*>(weights), static_cast<const T*>(areas), n_markers,
-                    spread_start, spread_entry, iterations, T(0.5 * fx), T(0.5 * fy), T(0.5 * fz),
-                    (uint32_t)n_total, static_cast<const T*>(nu), body_weight, static_cast<T*>(lam), tslot, tstart,
-                    trow, static_cast<const T*>(tcoeff), n_tslots, grad_w, static_cast<T*>(scratch), rule_target,
-                    n_rules, rule_uslot, rule_ustart, rule_urule, static_cast<const T*>(rule_ucoeff), n_rule_unique,
-                    static_cast<T*>(rule_scratch), ST ? rules_in_lds : 0, static_cast<const T*>(record));
+                if (ST) allow_lds(k_ibm_adj<T, L::D, ST>, sh.lds);
+                k_ibm_adj<T, L::D, ST><<<1, sh.threads, sh.lds, as_stream(stream)>>>(LBM_IBM_ADJ_CALL_ARGS);
             };
-            if (lds_tables <= limit) go(std::true_type{});
+            if (sh.staged) go(std::true_type{});
             else go(std::false_type{});
             check_launch("IBM adjoint");
         });
     });
 }
+
+LBM_API int lbm_ibm_adjoint_pull(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
+                                 const void* constant, int m, const int64_t* nodes, int n_nodes, const int* inverse,
+                                 const void* weights, const void* areas, int n_markers, const int* spread_start,
+                                 const int* spread_entry, int iterations, double fx, double fy, double fz,
+                                 int64_t n_total, const void* nu, const double* body_weight, void* lam,
+                                 const int64_t* tslot, const int* tstart, const int* trow, const void* tcoeff,
+                                 int n_tslots, double* grad_w, void* scratch, const int64_t* rule_target, int n_rules,
+                                 const int64_t* rule_uslot, const int* rule_ustart, const int* rule_urule,
+                                 const void* rule_ucoeff, int n_rule_unique, void* rule_scratch, const void* record,
+                                 const void* pull_src, void* pull_dst, const uint8_t* kind, const void* wall_u,
+                                 const void* force_field, const void* body_field, int nx, int ny, int nz,
+                                 double omega_plus, double omega_minus, int force_mode, int mask_mode, void* stream)
+{
+    return guarded("lbm_ibm_adjoint_pull", [&] {
+        LBM_IBM_ADJ_CHECK;
+        const Grid g = make_grid(lattice, nx, ny, nz);
+        require((int64_t)g.n == n_total, "the pull grid must have n_total nodes");
+        require(pull_src && pull_dst && pull_src != pull_dst, "pull buffers must be distinct and non-null");
+        require(pull_src != f && pull_dst != f && pull_dst != lam, "the pull must run on other buffers");
+        check_step_args(force_mode, mask_mode, kind, wall_u, force_field, body_field);
+        const double force[3] = {fx, fy, fz};
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            const IbmAdjShape sh = ibm_adj_shape<L, T>(n_nodes, n_markers, iterations, n_rules);
+            const Relaxation<T> r = make_relaxation<L, T>(omega_plus, omega_minus, force);
+            const int pbx = std::min(256, 64 * ((nx + 63) / 64));
+            const int pby = std::max(1, std::min(ny, sh.threads / pbx));
+            const int tiles_x = (nx + pbx - 1) / pbx, tiles_y = (ny + pby - 1) / pby;
+            const int64_t blocks = 1 + (int64_t)tiles_x * tiles_y * nz;
+            require(blocks < (1ll << 31), "grid too large");
+            dispatch_force_mask(force_mode, mask_mode, [&](auto fm, auto mk) {
+                constexpr ForceMode FM = decltype(fm)::value;
+                constexpr bool MK = decltype(mk)::value;
+                auto go = [&](auto staged) {
+                    constexpr bool ST = decltype(staged)::value;
+                    if (ST) allow_lds(k_ibm_adj_pull<L, T, FM, MK, ST>, sh.lds);
+                    k_ibm_adj_pull<L, T, FM, MK, ST><<<(unsigned)blocks, sh.threads, sh.lds, as_stream(stream)>>>(
+                        LBM_IBM_ADJ_CALL_ARGS, static_cast<const T*>(pull_src), static_cast<T*>(pull_dst), kind,
+                        static_cast<const T*>(wall_u), static_cast<const T*>(force_field),
+                        static_cast<const T*>(body_field), g, r, pbx, pby, tiles_x, tiles_y);
+                };
+                if (sh.staged) go(std::true_type{});
+                else go(std::false_type{});
+            });
+            check_launch("IBM adjoint with pull step");
+        });
+    });
+}
+
+#undef LBM_IBM_ADJ_CALL_ARGS
+#undef LBM_IBM_ADJ_CHECK
 
 }  // extern "C"

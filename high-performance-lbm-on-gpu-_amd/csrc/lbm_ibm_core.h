@@ -83,9 +83,11 @@ __device__ __forceinline__ T sequential_sum(int n, Term term)
     return s;
 }
 
-// Copies of several tables at once, in 4-byte words: each lane issues the loads of every
-// table for its next four indices before it stores any, so staging N tables costs about
-// one memory latency instead of N.
+// Copies of several tables at once. Each lane issues the loads of every table for its next
+// indices before it stores any, so staging them costs about one memory latency instead of
+// one per table. The WordCopy list moves 4-byte words; the extra `vec` copy moves 16-byte
+// units when both of its ends are 16-byte aligned (the last words that do not fill a unit,
+// or the whole copy when misaligned, go word by word).
 struct WordCopy {
     uint32_t* dst;
     const uint32_t* src;
@@ -101,25 +103,46 @@ __device__ __forceinline__ WordCopy word_copy(T* dst, const U* src, size_t count
 }
 
 template <int N>
-__device__ __forceinline__ void copy_tables(const WordCopy (&c)[N], int tid, int nthreads)
+__device__ __forceinline__ void copy_tables(const WordCopy (&c)[N], WordCopy vec, int tid, int nthreads)
 {
-    constexpr int K = 4;
-    size_t most = 0;
+    using V4 = uint32_t __attribute__((ext_vector_type(4)));
+    constexpr int KW = 2, KV = 2;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(vec.dst) | reinterpret_cast<uintptr_t>(vec.src)) & 15) == 0;
+    const size_t units = aligned ? vec.words / 4 : 0;
+    const WordCopy rest{vec.dst + units * 4, vec.src + units * 4, vec.words - units * 4};
+    const V4* vsrc = reinterpret_cast<const V4*>(vec.src);
+    V4* vdst = reinterpret_cast<V4*>(vec.dst);
+    const size_t step = (size_t)nthreads;
+    size_t rounds = (units + KV * step - 1) / (KV * step);
+    size_t most = rest.words;
 #pragma unroll
     for (int j = 0; j < N; ++j) most = c[j].words > most ? c[j].words : most;
-    const size_t step = (size_t)nthreads;
-    for (size_t i = tid; i < most; i += K * step) {
-        uint32_t v[N][K];
+    const size_t word_rounds = (most + KW * step - 1) / (KW * step);
+    if (word_rounds > rounds) rounds = word_rounds;
+    for (size_t r = 0; r < rounds; ++r) {
+        const size_t iw = tid + r * KW * step, iv = tid + r * KV * step;
+        uint32_t w[N + 1][KW];
+        V4 v[KV];
 #pragma unroll
-        for (int j = 0; j < N; ++j)
+        for (int k = 0; k < KV; ++k)
+            if (iv + k * step < units) v[k] = vsrc[iv + k * step];
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (i + k * step < c[j].words) v[j][k] = c[j].src[i + k * step];
+        for (int j = 0; j <= N; ++j) {
+            const WordCopy& cj = j < N ? c[j] : rest;
 #pragma unroll
-        for (int j = 0; j < N; ++j)
+            for (int k = 0; k < KW; ++k)
+                if (iw + k * step < cj.words) w[j][k] = cj.src[iw + k * step];
+        }
 #pragma unroll
-            for (int k = 0; k < K; ++k)
-                if (i + k * step < c[j].words) c[j].dst[i + k * step] = v[j][k];
+        for (int k = 0; k < KV; ++k)
+            if (iv + k * step < units) vdst[iv + k * step] = v[k];
+#pragma unroll
+        for (int j = 0; j <= N; ++j) {
+            const WordCopy& cj = j < N ? c[j] : rest;
+#pragma unroll
+            for (int k = 0; k < KW; ++k)
+                if (iw + k * step < cj.words) cj.dst[iw + k * step] = w[j][k];
+        }
     }
 }
 

@@ -59,21 +59,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_rules_small(T* __restrict__ F
                                                                const T* __restrict__ constant,
                                                                const int64_t* __restrict__ target, int64_t n, int m)
 {
-    T v[SMALL_PER_THREAD];
-    int64_t t[SMALL_PER_THREAD];
-    static_for<0, SMALL_PER_THREAD>([&](auto K) {
-        const int64_t r = (int64_t)K * SMALL_THREADS + threadIdx.x;
-        if (r < n) t[K] = target[r];  // fetched with the terms, used after the barrier
-    });
-    static_for<0, SMALL_PER_THREAD>([&](auto K) {
-        const int64_t r = (int64_t)K * SMALL_THREADS + threadIdx.x;
-        v[K] = r < n ? affine_row(F, slot, coeff, constant, n, m, r) : T(0);
-    });
-    __syncthreads();
-    static_for<0, SMALL_PER_THREAD>([&](auto K) {
-        const int64_t r = (int64_t)K * SMALL_THREADS + threadIdx.x;
-        if (r < n) F[t[K]] = v[K];
-    });
+    rules_in_one_group<T, SMALL_PER_THREAD>(F, slot, coeff, constant, target, n, m, threadIdx.x, SMALL_THREADS);
 }
 
 constexpr int FORCE_THREADS = 256;

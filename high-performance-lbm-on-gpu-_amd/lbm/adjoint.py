@@ -19,7 +19,10 @@ of its forcing iterations, so neither the segment recompute nor the IBM adjoint 
 forcing. dL/dX follows from the accumulated dL/dw and the derivative of the Roma kernel.
 
 Segments are captured once into HIP graphs (through ``torch.cuda.CUDAGraph``) and replayed,
-so the Python loop does not set the step rate. ``DragLoss`` wraps the whole thing as a
+so the Python loop does not set the step rate. The forward recompute of segment s - 1 runs
+inside the launches of segment s's backward sweep (``lbm_adj_pull_step_rules``,
+``lbm_ibm_adjoint_pull``): both are small, latency-bound launches, and pairing them halves
+the launches per backward step. ``DragLoss`` wraps the whole thing as a
 ``torch.autograd.Function``.
 """
 
@@ -98,7 +101,10 @@ class DragAdjoint:
         s._force_mode = FORCE_DENSE
         self.n_segments = -(-steps // k)
         self._ckpt = torch.zeros((self.n_segments, q, n), dtype=tdt, device=dev)
-        self._seg = torch.zeros((k + 1, q, n), dtype=tdt, device=dev)
+        # Two segment buffers: one segment is swept backwards while the previous one is
+        # recomputed forward inside the same launches.
+        self._segs = (torch.zeros((k + 1, q, n), dtype=tdt, device=dev),
+                      torch.zeros((k + 1, q, n), dtype=tdt, device=dev))
         # Each forward step records its IBM force field into its own pre-zeroed dense buffer
         # (the IBM kernel writes only the stencil nodes), so the segment recompute reuses it
         # instead of re-running the IBM; above ``field_store_bytes`` the recompute re-runs it.
@@ -111,7 +117,7 @@ class DragAdjoint:
         rec_elems = rec.value // s._np_dtype.itemsize
         field_bytes = steps * (d * n + rec_elems) * s._np_dtype.itemsize
         self.store_fields = field_bytes <= field_store_bytes
-        self._field_hist = torch.zeros((steps if self.store_fields else k, d, n), dtype=tdt, device=dev)
+        self._field_hist = torch.zeros((steps if self.store_fields else 2 * k, d, n), dtype=tdt, device=dev)
         self._records = torch.zeros((steps if self.store_fields else 0, rec_elems), dtype=tdt, device=dev)
         self._start = torch.zeros((q, n), dtype=tdt, device=dev)
         self._drag = torch.zeros((steps, d), dtype=tdt, device=dev)
@@ -120,7 +126,6 @@ class DragAdjoint:
         weights = np.zeros((steps, d))
         weights[steps - window:, 0] = 1.0 / window
         self._body_weight = torch.as_tensor(weights, device=dev)
-        self._weight_stage = torch.zeros((k, d), dtype=torch.float64, device=dev)
         self._lam = torch.zeros((2, q, n), dtype=tdt, device=dev)
         self._grad_body = torch.zeros((d, n), dtype=torch.float64, device=dev)
         self._nu = torch.zeros((d, n), dtype=tdt, device=dev)
@@ -240,43 +245,72 @@ class DragAdjoint:
         if length % 2:
             a.copy_(b)
 
-    def _backward_core(self, length: int, weights, fields, records=None) -> None:
-        """Recompute ``length`` steps from seg[0], then sweep them backwards.
+    def _recompute(self, buf, fields, j0: int, j1: int) -> None:
+        """Forward steps j0 .. j1 - 1 of a segment held in ``buf`` (buf[j] -> buf[j + 1]).
 
-        ``fields[j]`` is the IBM field of the j-th step of the segment; with recorded
-        fields the recompute skips the IBM. The adjoint buffers alternate with the parity
-        of the step inside the segment; segments start at even steps (k is even), so the
-        parity is the global one.
+        ``fields[j]`` is the IBM field of step j; recorded fields spare the IBM."""
+        for j in range(j0, j1):
+            if not self.store_fields:
+                self._ibm_forward(buf[j], fields[j], self._bf_scratch.data_ptr())
+            self._rules_forward(buf[j])
+            self._pull(buf[j], buf[j + 1], fields[j])
+
+    def _sweep(self, buf, fields, records, weights, length: int, partner) -> None:
+        """The backward sweep of a recomputed segment (states buf[0 .. length]).
+
+        ``partner`` = (buffer, fields, length) of the previous segment, or None. Its forward
+        recompute rides in the same launches: partner step jp = length - 1 - j goes with
+        backward step j, its link-rule pre-pass in the adjoint step's launch and its pull in
+        the IBM adjoint's launch; partner steps beyond ``length`` run after the sweep. The
+        adjoint buffers alternate with the parity of the step inside the segment; segments
+        start at even steps (k is even), so the parity is the global one.
         """
         s = self.solver
         L, D = s._lattice_code, s._dtype_code
-        for j in range(length):
-            cur = self._seg[j]
-            if not self.store_fields:
-                self._ibm_forward(cur, fields[j], self._bf_scratch.data_ptr())
-            self._rules_forward(cur)
-            self._pull(cur, self._seg[j + 1], fields[j])
         ibm, mp = s._ibm, self._mp
         fx, fy, fz = s._force3
         wp, wm = s._rates
         bx, by = self.adjoint_block
+        nx, ny, nz = s.params.nx, s.params.ny, s._nz
+        pack = s._rule_passes.get("pull")
+        fuse_rules = pack is not None and pack.n <= 4 * bx * by
+        r = self._rules or {"target": None, "n": 0, "uslot": None, "ustart": None, "urule": None,
+                            "ucoeff": None, "nu": 0, "scratch": None}
         for j in reversed(range(length)):
             lam_next, lam_out = self._lam[(j + 1) % 2], self._lam[j % 2]
-            s._call("lbm_adj_pull_step", L, D, pointer(lam_next), pointer(lam_out), pointer(self._seg[j]),
-                    pointer(s._kind), pointer(s._wall), pointer(fields[j]), pointer(s._body),
-                    s.params.nx, s.params.ny, s._nz, wp, wm, fx, fy, fz, FORCE_DENSE, s._mask_mode,
-                    pointer(self._grad_body), pointer(self._nu), bx, by, s._stream)
-            r = self._rules or {"target": None, "n": 0, "uslot": None, "ustart": None, "urule": None,
-                                "ucoeff": None, "nu": 0, "scratch": None}
-            s._call("lbm_ibm_adjoint", L, D, pointer(self._seg[j]), pointer(mp.slot), pointer(mp.coeff),
-                    pointer(mp.constant), mp.m, pointer(s._ibm_nodes), ibm.n_nodes, pointer(s._ibm_inverse),
-                    pointer(s._ibm_weights), pointer(s._ibm_areas), ibm.n_markers, pointer(s._ibm_start),
-                    pointer(s._ibm_entry), ibm.iterations, fx, fy, fz, s._n, pointer(self._nu),
-                    weights[j].data_ptr(), pointer(lam_out), pointer(self._tslot), pointer(self._tstart),
-                    pointer(self._trow), pointer(self._tcoeff), len(self._tslot), pointer(self._grad_w),
-                    pointer(self._ibm_adj_scratch), pointer(r["target"]), r["n"], pointer(r["uslot"]),
-                    pointer(r["ustart"]), pointer(r["urule"]), pointer(r["ucoeff"]), r["nu"], pointer(r["scratch"]),
-                    None if records is None else pointer(records[j]), s._stream)
+            jp = length - 1 - j
+            with_partner = partner is not None and jp < partner[2]
+            if with_partner:
+                buf_p, fields_p, _ = partner
+                if not self.store_fields:
+                    self._ibm_forward(buf_p[jp], fields_p[jp], self._bf_scratch.data_ptr())
+            adj = (L, D, pointer(lam_next), pointer(lam_out), pointer(buf[j]), pointer(s._kind), pointer(s._wall),
+                   pointer(fields[j]), pointer(s._body), nx, ny, nz, wp, wm, fx, fy, fz, FORCE_DENSE, s._mask_mode,
+                   pointer(self._grad_body), pointer(self._nu), bx, by)
+            if with_partner and fuse_rules:
+                s._call("lbm_adj_pull_step_rules", *adj, pointer(buf_p[jp]), pointer(pack.slot), pointer(pack.coeff),
+                        pointer(pack.constant), pointer(pack.target), pack.n, pack.m, s._stream)
+            else:
+                if with_partner:
+                    self._rules_forward(buf_p[jp])
+                s._call("lbm_adj_pull_step", *adj, s._stream)
+            ibm_args = (L, D, pointer(buf[j]), pointer(mp.slot), pointer(mp.coeff), pointer(mp.constant), mp.m,
+                        pointer(s._ibm_nodes), ibm.n_nodes, pointer(s._ibm_inverse), pointer(s._ibm_weights),
+                        pointer(s._ibm_areas), ibm.n_markers, pointer(s._ibm_start), pointer(s._ibm_entry),
+                        ibm.iterations, fx, fy, fz, s._n, pointer(self._nu), weights[j].data_ptr(),
+                        pointer(lam_out), pointer(self._tslot), pointer(self._tstart), pointer(self._trow),
+                        pointer(self._tcoeff), len(self._tslot), pointer(self._grad_w), pointer(self._ibm_adj_scratch),
+                        pointer(r["target"]), r["n"], pointer(r["uslot"]), pointer(r["ustart"]), pointer(r["urule"]),
+                        pointer(r["ucoeff"]), r["nu"], pointer(r["scratch"]),
+                        None if records is None else pointer(records[j]))
+            if with_partner:
+                s._call("lbm_ibm_adjoint_pull", *ibm_args, pointer(buf_p[jp]), pointer(buf_p[jp + 1]),
+                        pointer(s._kind), pointer(s._wall), pointer(fields_p[jp]), pointer(s._body), nx, ny, nz,
+                        wp, wm, FORCE_DENSE, s._mask_mode, s._stream)
+            else:
+                s._call("lbm_ibm_adjoint", *ibm_args, s._stream)
+        if partner is not None and partner[2] > length:
+            self._recompute(partner[0], partner[1], length, partner[2])
 
     @property
     def adjoint_block(self) -> tuple[int, int]:
@@ -339,17 +373,35 @@ class DragAdjoint:
         self._grad_body.zero_()
         self._grad_w.zero_()
         self._nu.zero_()
-        for seg in reversed(range(self.n_segments)):
-            t0 = seg * self.k
-            length = min(self.k, self.steps - t0)
-            self._seg[0].copy_(self._ckpt[seg])
-            self._weight_stage[:length].copy_(self._body_weight[t0 : t0 + length])
-            if length % 2 and self.n_segments > 1:
-                raise ValueError("segments after the first must have an even length")
-            fields = self._field_hist[t0 : t0 + length] if self.store_fields else self._field_hist
-            records = self._records[t0 : t0 + length] if self.store_fields else None
-            self._replay(("bwd", length, t0 if self.store_fields else 0),
-                         lambda: self._backward_core(length, self._weight_stage, fields, records))
+        k, n_seg = self.k, self.n_segments
+        if n_seg > 1 and (self.steps - (n_seg - 1) * k) % 2:
+            raise ValueError("segments after the first must have an even length")
+
+        def length(seg):
+            return min(k, self.steps - seg * k)
+
+        def fields(seg, slot):
+            if self.store_fields:
+                return self._field_hist[seg * k : seg * k + length(seg)]
+            return self._field_hist[slot * k : (slot + 1) * k]
+
+        def records(seg):
+            return self._records[seg * k : seg * k + length(seg)] if self.store_fields else None
+
+        # The last segment is recomputed on its own; every other segment's recompute rides
+        # in the launches of the following segment's backward sweep.
+        last = n_seg - 1
+        self._segs[0][0].copy_(self._ckpt[last])
+        self._replay(("rec", last), lambda: self._recompute(self._segs[0], fields(last, 0), 0, length(last)))
+        for i, seg in enumerate(reversed(range(n_seg))):
+            slot = i % 2
+            partner = None
+            if seg > 0:
+                self._segs[1 - slot][0].copy_(self._ckpt[seg - 1])
+                partner = (self._segs[1 - slot], fields(seg - 1, 1 - slot), length(seg - 1))
+            weights = self._body_weight[seg * k : seg * k + length(seg)]
+            self._replay(("bwd", seg), lambda: self._sweep(self._segs[slot], fields(seg, slot), records(seg),
+                                                           weights, length(seg), partner))
         self._torch.cuda.synchronize()
         p = self.params
         grad_body = self._grad_body.cpu().numpy().reshape(p.d, *p.shape)

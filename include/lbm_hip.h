@@ -193,6 +193,19 @@ LBM_API int lbm_adj_pull_step(int lattice, int dtype, const void* lam_next, void
                               double omega_minus, double fx, double fy, double fz, int force_mode,
                               int mask_mode, double* grad_body, void* nu_forced, int bx, int by, void* stream);
 
+/* lbm_adj_pull_step plus, in the same launch, an independent link-rule pre-pass (as
+ * lbm_rules_apply, 1 <= n_rules <= 4 * bx * by) on another state buffer rule_f. No counterpart
+ * in the reference: the checkpointed adjoint uses it to run the forward recompute of one
+ * segment under the backward sweep of the next, saving a launch per step. */
+LBM_API int lbm_adj_pull_step_rules(int lattice, int dtype, const void* lam_next, void* lam_out, const void* f,
+                                    const uint8_t* kind, const void* wall_u, const void* force_field,
+                                    const void* body_field, int nx, int ny, int nz, double omega_plus,
+                                    double omega_minus, double fx, double fy, double fz, int force_mode,
+                                    int mask_mode, double* grad_body, void* nu_forced, int bx, int by,
+                                    void* rule_f, const int64_t* rule_slot, const void* rule_coeff,
+                                    const void* rule_constant, const int64_t* rule_target, int64_t n_rules,
+                                    int rule_m, void* stream);
+
 /* Transposed link-rule pre-pass (no counterpart in the reference): a_r = lam[target_r],
  * lam[target_r] = 0, then lam[uslot[u]] += sum_{e in [ustart[u], ustart[u+1])} ucoeff[e] a[urule[e]]
  * over the distinct term slots. One workgroup; scratch holds n_rules values of dtype. */
@@ -222,6 +235,25 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
                             int n_rules, const int64_t* rule_uslot, const int* rule_ustart,
                             const int* rule_urule, const void* rule_ucoeff, int n_rule_unique,
                             void* rule_scratch, const void* record, void* stream);
+
+/* lbm_ibm_adjoint plus, in the same launch, an independent pull step (as lbm_pull_step, with
+ * the uniform force fx, fy, fz of the IBM arguments and an n_total-node grid) from pull_src to
+ * pull_dst, buffers other than f and lam. No counterpart in the reference: the checkpointed
+ * adjoint uses it to run the forward recompute of one segment under the backward sweep of the
+ * next, saving a launch per step. */
+LBM_API int lbm_ibm_adjoint_pull(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
+                                 const void* constant, int m, const int64_t* nodes, int n_nodes,
+                                 const int* inverse, const void* weights, const void* areas, int n_markers,
+                                 const int* spread_start, const int* spread_entry, int iterations, double fx,
+                                 double fy, double fz, int64_t n_total, const void* nu, const double* body_weight,
+                                 void* lam, const int64_t* tslot, const int* tstart, const int* trow,
+                                 const void* tcoeff, int n_tslots, double* grad_w, void* scratch,
+                                 const int64_t* rule_target, int n_rules, const int64_t* rule_uslot,
+                                 const int* rule_ustart, const int* rule_urule, const void* rule_ucoeff,
+                                 int n_rule_unique, void* rule_scratch, const void* record, const void* pull_src,
+                                 void* pull_dst, const uint8_t* kind, const void* wall_u, const void* force_field,
+                                 const void* body_field, int nx, int ny, int nz, double omega_plus,
+                                 double omega_minus, int force_mode, int mask_mode, void* stream);
 
 /* Size in bytes of one lattice descriptor of the batched step (no counterpart in the
  * reference, which runs one lattice per solver). Host only. */

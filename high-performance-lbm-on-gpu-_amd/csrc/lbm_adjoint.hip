@@ -195,11 +195,15 @@ __global__ __launch_bounds__(256) void k_adj_pull(const T* __restrict__ lam_next
 // The adjoint step plus an independent link-rule pre-pass on another state buffer, in one
 // launch: the grid has one extra z layer whose block (0, 0) runs the pre-pass
 // (rules_in_one_group, at most 4 rules per lane). Used to overlap the forward recompute
-// of one checkpoint segment with the backward sweep of the next.
+// of one checkpoint segment with the backward sweep of the next. Blocks may have up to 1024
+// lanes (D2Q9 float32), so the pre-pass gets one lane per rule; the float64 and D3Q19
+// adjoints need more than the 128 VGPRs a 1024-lane block leaves, so their blocks stay at
+// 512 and 256 lanes.
 constexpr int ADJ_RULES_PER_LANE = 4;
+constexpr int adj_rules_max_threads(int d, size_t real) { return d == 3 ? 256 : (real == 4 ? 1024 : 512); }
 
 template <class L, class T, ForceMode FM, bool MASK>
-__global__ __launch_bounds__(256) void k_adj_pull_rules(const T* __restrict__ lam_next, T* __restrict__ lam_out,
+__global__ __launch_bounds__(adj_rules_max_threads(L::D, sizeof(T))) void k_adj_pull_rules(const T* __restrict__ lam_next, T* __restrict__ lam_out,
                                                         const T* __restrict__ f, const uint8_t* __restrict__ kind,
                                                         const T* __restrict__ wall_u, const T* __restrict__ field,
                                                         const T* __restrict__ body, double* __restrict__ grad_body,
@@ -753,7 +757,9 @@ LBM_API int lbm_adj_pull_step_rules(int lattice, int dtype, const void* lam_next
 {
     return guarded("lbm_adj_pull_step_rules", [&] {
         const Grid g = make_grid(lattice, nx, ny, nz);
-        require(bx > 0 && by > 0 && bx * by <= 256, "adjoint block must be positive with at most 256 threads");
+        const int max_threads = adj_rules_max_threads(lattice == LBM_D2Q9 ? 2 : 3, dtype == LBM_F32 ? 4 : 8);
+        require(bx > 0 && by > 0 && bx * by <= max_threads,
+                "block must be positive with at most 1024 (D2Q9 f32), 512 (D2Q9 f64) or 256 (D3Q19) threads");
         require(lam_next && lam_out && f && lam_next != lam_out, "adjoint buffers must be distinct and non-null");
         check_step_args(force_mode, mask_mode, kind, wall_u, force_field, body_field);
         require(n_rules >= 1 && n_rules <= (int64_t)ADJ_RULES_PER_LANE * bx * by,

@@ -273,7 +273,8 @@ class DragAdjoint:
         bx, by = self.adjoint_block
         nx, ny, nz = s.params.nx, s.params.ny, s._nz
         pack = s._rule_passes.get("pull")
-        fuse_rules = pack is not None and pack.n <= 4 * bx * by
+        fbx, fby = self.fused_block
+        fuse_rules = pack is not None and pack.n <= 4 * fbx * fby
         r = self._rules or {"target": None, "n": 0, "uslot": None, "ustart": None, "urule": None,
                             "ucoeff": None, "nu": 0, "scratch": None}
         for j in reversed(range(length)):
@@ -288,8 +289,8 @@ class DragAdjoint:
                    pointer(fields[j]), pointer(s._body), nx, ny, nz, wp, wm, fx, fy, fz, FORCE_DENSE, s._mask_mode,
                    pointer(self._grad_body), pointer(self._nu), bx, by)
             if with_partner and fuse_rules:
-                s._call("lbm_adj_pull_step_rules", *adj, pointer(buf_p[jp]), pointer(pack.slot), pointer(pack.coeff),
-                        pointer(pack.constant), pointer(pack.target), pack.n, pack.m, s._stream)
+                s._call("lbm_adj_pull_step_rules", *adj[:-2], fbx, fby, pointer(buf_p[jp]), pointer(pack.slot),
+                        pointer(pack.coeff), pointer(pack.constant), pointer(pack.target), pack.n, pack.m, s._stream)
             else:
                 if with_partner:
                     self._rules_forward(buf_p[jp])
@@ -316,6 +317,14 @@ class DragAdjoint:
     def adjoint_block(self) -> tuple[int, int]:
         bx = min(256, 64 * -(-self.params.nx // 64))
         return bx, max(1, min(self.params.ny, 256 // bx))
+
+    @property
+    def fused_block(self) -> tuple[int, int]:
+        """Block of the adjoint step that carries a link-rule pre-pass: up to 1024 lanes for
+        D2Q9 float32 (one lane per rule), 512 for D2Q9 float64, 256 for D3Q19."""
+        bx, _ = self.adjoint_block
+        lanes = 256 if self.params.d == 3 else (1024 if self.solver._np_dtype.itemsize == 4 else 512)
+        return bx, max(1, min(self.params.ny, lanes // bx))
 
     # ------------------------------------------------------------------ graphs
 

@@ -194,7 +194,8 @@ LBM_API int lbm_adj_pull_step(int lattice, int dtype, const void* lam_next, void
                               int mask_mode, double* grad_body, void* nu_forced, int bx, int by, void* stream);
 
 /* lbm_adj_pull_step plus, in the same launch, an independent link-rule pre-pass (as
- * lbm_rules_apply, 1 <= n_rules <= 4 * bx * by) on another state buffer rule_f. No counterpart
+ * lbm_rules_apply, 1 <= n_rules <= 4 * bx * by) on another state buffer rule_f; bx * by up to
+ * 1024 for D2Q9 float32, 512 for D2Q9 float64 and 256 for D3Q19. No counterpart
  * in the reference: the checkpointed adjoint uses it to run the forward recompute of one
  * segment under the backward sweep of the next, saving a launch per step. */
 LBM_API int lbm_adj_pull_step_rules(int lattice, int dtype, const void* lam_next, void* lam_out, const void* f,

@@ -1,0 +1,191 @@
+"""Case drivers and the full-size configurations of the benchmark list.
+
+* Ghia cavity (tests/validation/test_cavity.py of the reference) and, at the float64
+  parity size (128 x 128 fluid nodes, Re = 100, lid 0.1, tau = 0.884), the GPU
+  populations against the NumPy oracle every 1,000 steps.
+* Schaefer-Turek 2D-1 / 2D-2 (tests/validation/test_cylinder_benchmark.py) and the
+  880 x 164 Bouzidi drag channel against the oracle, with its per-step force history
+  reduced on the device.
+* Sphere drag against Clift-Grace-Weber (tests/validation/test_sphere_drag.py).
+"""
+
+import numpy as np
+import pytest
+
+from lbm.cases.cylinder_benchmark import build, drag_channel, force_history, lift_period
+from lbm.cases.ghia1982 import REYNOLDS_NUMBERS, ghia_centerlines
+from lbm.cases.lid_driven_cavity import cavity_params, run_cavity
+from lbm.cases.sphere_drag import clift_grace_weber, schiller_naumann, sphere_params
+
+# ---------------------------------------------------------------- host
+
+
+def test_ghia_tables_are_consistent():
+    for re in REYNOLDS_NUMBERS:
+        ref = ghia_centerlines(re)
+        assert ref.y.shape == ref.u.shape == ref.x.shape == ref.v.shape == (17,)
+        assert ref.u[0] == 1.0 and ref.u[-1] == 0.0  # lid and bottom wall
+        assert ref.v[0] == 0.0 and ref.v[-1] == 0.0  # side walls
+        assert np.all(np.diff(ref.y) < 0) and np.all(np.diff(ref.x) < 0)
+        assert not ref.u.flags.writeable
+
+
+def test_unknown_reynolds_number_is_rejected():
+    with pytest.raises(ValueError, match="Re"):
+        ghia_centerlines(5000)
+    with pytest.raises(ValueError, match="reynolds"):
+        run_cavity(reynolds=5000)
+
+
+def test_cavity_parameters_of_the_parity_configuration():
+    p = cavity_params(100, 128, 0.1)
+    assert (p.nx, p.ny) == (130, 130)
+    assert p.tau == pytest.approx(0.884)
+    assert p.solid.sum() == 130 * 130 - 128 * 128
+    assert np.all(p.wall_velocity[0, -1, :] == 0.1)
+
+
+def test_build_rejects_bad_inputs():
+    with pytest.raises(ValueError, match="test"):
+        build("3D-1", 20)
+    with pytest.raises(ValueError, match="multiple of 10"):
+        build("2D-1", 25)
+    with pytest.raises(ValueError, match="method"):
+        build("2D-1", 20, method="spectral")
+
+
+def test_benchmark_geometry_matches_the_definition():
+    g = build("2D-1", 20)
+    p = g.params
+    assert (p.nx, p.ny) == (443, 84)  # L = 22 D + 3 layers, H = 4.1 D + 2 wall rows
+    assert g.u_mean == pytest.approx(2 * 0.01875 / 3)
+    assert p.tau == pytest.approx(3 * g.u_mean * 20 / 20 + 0.5)  # Re = 20
+    assert g.inflow.max() <= 0.01875 and g.inflow[0] == 0.0 and g.inflow[-1] == 0.0
+
+
+def test_lift_period_of_a_sampled_sine():
+    t = np.arange(0, 1000, 7.0)
+    cl = np.sin(2 * np.pi * t / 123.0 + 0.3)
+    assert lift_period(t, cl) == pytest.approx(123.0, rel=1e-3)
+    with pytest.raises(ValueError, match="zero crossings"):
+        lift_period(t[:5], cl[:5])
+
+
+def test_drag_correlations_agree_at_re_100():
+    assert schiller_naumann(100) == pytest.approx(clift_grace_weber(100), rel=0.01)
+    assert clift_grace_weber(20) == pytest.approx(2.735, rel=1e-3)
+
+
+def test_sphere_box_geometry():
+    params, links = sphere_params(4, 20.0)
+    assert (params.nz, params.ny, params.nx) == (40, 40, 83)
+    assert len(links.node) > 0
+    with pytest.raises(ValueError, match="method"):
+        sphere_params(4, 20.0, method="spectral")
+
+
+# ---------------------------------------------------------------- GPU
+
+
+def _states_close(params, ref, gpu, rtol):
+    fluid = params.fluid
+    a = ref.populations()[:, fluid]
+    b = gpu.populations()[:, fluid]
+    scale = np.abs(a).max()
+    return float(np.abs(a - b).max() / scale), rtol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend", ["hip-pull", "hip-aa"])
+def test_cavity_128_float64_matches_the_oracle_every_1000_steps(backend):
+    from lbm import make_solver
+    from oracle import ReferenceSolver
+
+    params = cavity_params(100, 128, 0.1, dtype="float64")
+    ref, gpu = ReferenceSolver(params), make_solver(params, backend=backend)
+    for s in (ref, gpu):
+        s.initialize()
+    for _ in range(2):
+        for s in (ref, gpu):
+            s.run(1000, check_every=1000)
+        err, tol = _states_close(params, ref, gpu, 1e-14)
+        assert err <= tol, f"relative population difference {err:.3e} after {ref.time_step} steps"
+
+
+@pytest.mark.gpu
+def test_cavity_re100_matches_ghia():
+    assert run_cavity(reynolds=100, n=64, backend="hip-aa", dtype="float32").max_error < 0.01
+
+
+@pytest.mark.gpu
+def test_drag_channel_880x164_matches_the_oracle_with_per_step_forces():
+    """Populations after 100 steps within 1e-6 (float32) and the drag/lift history, recorded
+    on the device after every step, within float32 round-off of the oracle's."""
+    from lbm import make_solver
+    from lbm.forces import ForceProbe
+    from oracle import ReferenceSolver
+
+    params, links, profile = drag_channel(reynolds=100.0)
+    assert (params.nx, params.ny) == (880, 164)
+    probe = ForceProbe.build(params, links)
+    ux = np.where(params.solid, 0.0, profile[:, None])
+    ref, gpu = ReferenceSolver(params), make_solver(params, backend="hip-pull")
+    for s in (ref, gpu):
+        s.initialize(1.0, ux, 0.0)
+    history = force_history(gpu, probe, 100)
+    expected = []
+    for _ in range(100):
+        ref.run(1, check_every=1)
+        expected.append(probe.evaluate(ref))
+    expected = np.asarray(expected)
+    fluid = params.fluid
+    assert np.abs(ref.populations()[:, fluid] - gpu.populations()[:, fluid]).max() <= 1e-6
+    scale = np.abs(expected).max()
+    np.testing.assert_allclose(history, expected, rtol=0, atol=1e-5 * scale)
+
+
+@pytest.mark.gpu
+def test_force_history_equals_host_evaluation_every_pair():
+    from lbm import make_solver
+    from lbm.forces import ForceProbe
+
+    params, links, profile = drag_channel(reynolds=20.0, nx=220, ny=42, radius=5.0, centre=(50.0, 20.0))
+    probe = ForceProbe.build(params, links)
+    ux = np.where(params.solid, 0.0, profile[:, None])
+    a, b = make_solver(params, backend="hip-aa"), make_solver(params, backend="hip-aa")
+    for s in (a, b):
+        s.initialize(1.0, ux, 0.0)
+    history = force_history(a, probe, 40)
+    host = []
+    for _ in range(20):
+        b.run(2, check_every=2)
+        host.append(probe.evaluate(b))
+    np.testing.assert_allclose(history, np.asarray(host), rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_steady_2d1_drag_lift_and_pressure_drop():
+    from lbm.cases.cylinder_benchmark import run_steady
+
+    r = run_steady(20, backend="hip-aa")
+    assert r.cd == pytest.approx(5.58, rel=0.01)
+    assert 0.0104 < r.cl < 0.0110
+    assert r.dp == pytest.approx(0.1174, rel=0.02)
+
+
+@pytest.mark.gpu
+def test_periodic_2d2_shedding_frequency_and_lift():
+    from lbm.cases.cylinder_benchmark import run_periodic
+
+    r = run_periodic(20)
+    assert 0.295 < r.strouhal < 0.305
+    assert r.cl_max == pytest.approx(1.0, rel=0.08)
+    assert r.cd_max == pytest.approx(3.23, rel=0.03)
+
+
+@pytest.mark.gpu
+def test_bouzidi_sphere_drag_at_re_20():
+    from lbm.cases.sphere_drag import run_sphere_drag
+
+    r = run_sphere_drag(10, 20.0, "bouzidi")
+    assert r.cd == pytest.approx(r.reference_cgw, rel=0.03)

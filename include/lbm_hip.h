@@ -138,9 +138,10 @@ LBM_API int lbm_rules_apply_widths(int dtype, void* f, const int64_t* slot, cons
 LBM_API int lbm_affine_values(int dtype, const void* f, const int64_t* slot, const void* coeff,
                               const void* constant, int64_t n_rows, int m, void* out, void* stream);
 
-/* Momentum-exchange force: out[a] (double) += sum_r weight[r][a] * value[r] over the
- * affine rows (value rounded to dtype, then widened), summed in row order by one
- * workgroup; d components. Replaces ForceProbe.evaluate's host matmul
+/* Momentum-exchange force: out[a] (double) = sum_r weight[r][a] * value[r] over the
+ * affine rows (value rounded to dtype, then widened), reduced by one workgroup in a fixed
+ * order (lane-strided partial sums, then a tree), so repeated calls give the same bits;
+ * d components. Replaces ForceProbe.evaluate's host matmul
  * (src/lbm/forces.py:98-103) so a per-step drag history stays on the device. */
 LBM_API int lbm_affine_force(int dtype, const void* f, const int64_t* slot, const void* coeff,
                              const void* constant, const double* weight, int64_t n_rows, int m,

@@ -223,6 +223,7 @@ def main(argv=None) -> int:
     parser.add_argument("--size", type=int, default=256)
     parser.add_argument("--variant", default="aa", choices=("aa", "pull"))
     parser.add_argument("--block", type=int, nargs=2, default=None)
+    parser.add_argument("--groups-per-cu", type=int, default=None, help="resident workgroups per CU (0: no cap)")
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-adjoint", action="store_true")
     parser.add_argument("--no-batched", action="store_true")
@@ -241,9 +242,12 @@ def main(argv=None) -> int:
     n = args.size
     params = abc_params(n)
     options = {"block": tuple(args.block)} if args.block else {}
+    if args.groups_per_cu is not None:
+        options["groups_per_cu"] = args.groups_per_cu
     solver = make_solver(params, backend=f"hip-{args.variant}", **options)
     solver.initialize(*abc_fields(n))
-    print(f"# {torch.cuda.get_device_name(0)}: D3Q19 fp32 {args.variant} {n}^3, block {solver.block}", flush=True)
+    print(f"# {torch.cuda.get_device_name(0)}: D3Q19 fp32 {args.variant} {n}^3, block {solver.block}, "
+          f"{solver.groups_per_cu or 'no'} resident groups/CU cap", flush=True)
 
     if args.warmup:
         solver.run(args.warmup, check_every=args.warmup)
@@ -317,6 +321,7 @@ def main(argv=None) -> int:
             "nodes": nodes,
             "bytes_per_update": bytes_per_update,
             "block": list(solver.block),
+            "groups_per_cu": solver.groups_per_cu,
             "copy_peak_gbs": round(peak, 1),
             "copy_kernel_gbs": round(copies["kernel"], 1),
             "copy_runtime_gbs": round(copies["runtime"], 1),

@@ -105,13 +105,40 @@ inline dim3 grid_of(const Grid& g, dim3 block)
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// A device attribute of the current device, queried once per device.
+inline size_t device_attribute(hipDeviceAttribute_t attribute, size_t (&cache)[64])
+{
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev >= 0 && dev < 64 && cache[dev]) return cache[dev];
+    int value = 0;
+    hip_check(hipDeviceGetAttribute(&value, attribute, dev), "device attribute query");
+    if (dev >= 0 && dev < 64) cache[dev] = (size_t)value;
+    return (size_t)value;
+}
+
 // LDS bytes one workgroup may allocate on the current device (160 KiB on gfx950).
 inline size_t lds_limit()
 {
-    int dev = 0, bytes = 0;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    hip_check(hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "LDS size query");
-    return (size_t)bytes;
+    static size_t cache[64];
+    return device_attribute(hipDeviceAttributeMaxSharedMemoryPerBlock, cache);
+}
+
+// LDS bytes of one compute unit on the current device (160 KiB on gfx950).
+inline size_t lds_per_cu()
+{
+    static size_t cache[64];
+    return device_attribute(hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, cache);
+}
+
+// Dynamic LDS a launch requests so that at most `groups_per_cu` of its workgroups are
+// resident on one compute unit (0: no cap), never less than the `needed` bytes it uses.
+// Capping residency this way is how the streaming kernels pick their occupancy.
+inline size_t occupancy_lds(int groups_per_cu, size_t needed)
+{
+    if (groups_per_cu <= 0) return needed;
+    const size_t cap = (lds_per_cu() / (size_t)groups_per_cu) & ~(size_t)255;
+    return std::max(needed, std::min(cap, lds_limit()));
 }
 
 // Allow a kernel more than the default 64 KiB of dynamic LDS before launching it with `bytes`.

@@ -241,27 +241,35 @@ enum class Variant { PULL, EVEN, ODD, LDS };
 
 template <class L, class T, ForceMode FM, bool MASK>
 void launch_step(Variant v, const void* a, void* b, const uint8_t* kind, const void* wall, const void* field,
-                 const void* body, const Grid& g, const Relaxation<T>& r, dim3 block, hipStream_t s)
+                 const void* body, const Grid& g, const Relaxation<T>& r, dim3 block, int groups_per_cu,
+                 hipStream_t s)
 {
     const dim3 grid = grid_of(g, block);
     const T* wall_t = static_cast<const T*>(wall);
     const T* field_t = static_cast<const T*>(field);
     const T* body_t = static_cast<const T*>(body);
+    // The streaming kernels use no LDS; a cap on resident workgroups is requested as LDS.
+    const size_t cap = occupancy_lds(groups_per_cu, 0);
     switch (v) {
     case Variant::PULL:
-        k_pull<L, T, FM, MASK><<<grid, block, 0, s>>>(static_cast<const T*>(a), static_cast<T*>(b), kind, wall_t,
-                                                       field_t, body_t, g, r);
+        allow_lds(k_pull<L, T, FM, MASK>, cap);
+        k_pull<L, T, FM, MASK><<<grid, block, cap, s>>>(static_cast<const T*>(a), static_cast<T*>(b), kind, wall_t,
+                                                         field_t, body_t, g, r);
         break;
     case Variant::EVEN:
-        k_aa_even<L, T, FM, MASK><<<grid, block, 0, s>>>(static_cast<T*>(b), kind, field_t, body_t, g, r);
+        allow_lds(k_aa_even<L, T, FM, MASK>, cap);
+        k_aa_even<L, T, FM, MASK><<<grid, block, cap, s>>>(static_cast<T*>(b), kind, field_t, body_t, g, r);
         break;
     case Variant::ODD:
-        k_aa_odd<L, T, FM, MASK><<<grid, block, 0, s>>>(static_cast<T*>(b), kind, wall_t, field_t, body_t, g, r);
+        allow_lds(k_aa_odd<L, T, FM, MASK>, cap);
+        k_aa_odd<L, T, FM, MASK><<<grid, block, cap, s>>>(static_cast<T*>(b), kind, wall_t, field_t, body_t, g, r);
         break;
     case Variant::LDS:
         if constexpr (L::D == 2) {
-            const size_t lds = sizeof(T) * (size_t)L::Q * (block.x + 2) * (block.y + 2) + (block.x + 2) * (block.y + 2);
-            require(lds <= 160 * 1024, "LDS tile does not fit in 160 KB; use a smaller block");
+            const size_t tile = sizeof(T) * (size_t)L::Q * (block.x + 2) * (block.y + 2) + (block.x + 2) * (block.y + 2);
+            require(tile <= 160 * 1024, "LDS tile does not fit in 160 KB; use a smaller block");
+            const size_t lds = occupancy_lds(groups_per_cu, tile);
+            allow_lds(k_lds<L, T, FM, MASK>, lds);
             k_lds<L, T, FM, MASK><<<grid, block, lds, s>>>(static_cast<const T*>(a), static_cast<T*>(b), kind,
                                                            wall_t, field_t, body_t, g, r);
         } else {
@@ -275,11 +283,12 @@ void launch_step(Variant v, const void* a, void* b, const uint8_t* kind, const v
 int run_step(const char* name, Variant v, int lattice, int dtype, const void* a, void* b, const uint8_t* kind,
              const void* wall, const void* field, const void* body, int nx, int ny, int nz, double wp,
              double wm, double fx, double fy, double fz, int force_mode, int mask_mode, int bx, int by,
-             void* stream)
+             int groups_per_cu, void* stream)
 {
     return guarded(name, [&] {
         const Grid g = make_grid(lattice, nx, ny, nz);
         const dim3 block = block_of(bx, by);
+        require(groups_per_cu >= 0, "groups_per_cu must be >= 0 (0: no occupancy cap)");
         require(b != nullptr && (v == Variant::EVEN || v == Variant::ODD || a != nullptr), "null population buffer");
         require(mask_mode == 0 || kind != nullptr, "mask_mode 1 needs the kind array");
         require(force_mode >= 0 && force_mode <= 2, "unknown force mode");
@@ -294,7 +303,7 @@ int run_step(const char* name, Variant v, int lattice, int dtype, const void* a,
             const Relaxation<T> r = make_relaxation<L, T>(wp, wm, force);
             auto go = [&](auto fm, auto mk) {
                 launch_step<L, T, decltype(fm)::value, decltype(mk)::value>(v, a, b, kind, wall, field, body, g, r,
-                                                                             block, as_stream(stream));
+                                                                             block, groups_per_cu, as_stream(stream));
             };
             using U = std::integral_constant<ForceMode, ForceMode::UNIFORM>;
             using Fi = std::integral_constant<ForceMode, ForceMode::FIELD>;
@@ -360,36 +369,37 @@ LBM_API int lbm_lattice_tables_device(int lattice, int dtype, int* c, int* opp, 
 LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst, const uint8_t* kind,
                           const void* wall_u, const void* force_field, const void* body_field, int nx, int ny,
                           int nz, double omega_plus, double omega_minus, double fx, double fy, double fz,
-                          int force_mode, int mask_mode, int bx, int by, void* stream)
+                          int force_mode, int mask_mode, int bx, int by, int groups_per_cu, void* stream)
 {
     return run_step("lbm_pull_step", Variant::PULL, lattice, dtype, f_src, f_dst, kind, wall_u, force_field,
                     body_field, nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by,
-                    stream);
+                    groups_per_cu, stream);
 }
 
 LBM_API int lbm_aa_even(int lattice, int dtype, void* f, const uint8_t* kind, const void* force_field,
                         const void* body_field, int nx, int ny, int nz, double omega_plus, double omega_minus,
                         double fx, double fy, double fz, int force_mode, int mask_mode, int bx, int by,
-                        void* stream)
+                        int groups_per_cu, void* stream)
 {
     return run_step("lbm_aa_even", Variant::EVEN, lattice, dtype, nullptr, f, kind, nullptr, force_field,
                     body_field, nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by,
-                    stream);
+                    groups_per_cu, stream);
 }
 
 LBM_API int lbm_aa_odd(int lattice, int dtype, void* f, const uint8_t* kind, const void* wall_u,
                        const void* force_field, const void* body_field, int nx, int ny, int nz, double omega_plus,
                        double omega_minus, double fx, double fy, double fz, int force_mode, int mask_mode, int bx,
-                       int by, void* stream)
+                       int by, int groups_per_cu, void* stream)
 {
     return run_step("lbm_aa_odd", Variant::ODD, lattice, dtype, nullptr, f, kind, wall_u, force_field, body_field,
-                    nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by, stream);
+                    nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by, groups_per_cu,
+                    stream);
 }
 
 LBM_API int lbm_lds_step(int lattice, int dtype, const void* f_src, void* f_dst, const uint8_t* kind,
                          const void* wall_u, const void* force_field, const void* body_field, int nx, int ny,
                          int nz, double omega_plus, double omega_minus, double fx, double fy, double fz,
-                         int force_mode, int mask_mode, int bx, int by, void* stream)
+                         int force_mode, int mask_mode, int bx, int by, int groups_per_cu, void* stream)
 {
     if (lattice != LBM_D2Q9) {
         set_error("lbm_lds_step: the LDS-tiled variant is 2D only");
@@ -397,7 +407,7 @@ LBM_API int lbm_lds_step(int lattice, int dtype, const void* f_src, void* f_dst,
     }
     return run_step("lbm_lds_step", Variant::LDS, lattice, dtype, f_src, f_dst, kind, wall_u, force_field,
                     body_field, nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by,
-                    stream);
+                    groups_per_cu, stream);
 }
 
 LBM_API int lbm_init_equilibrium(int lattice, int dtype, void* f, const void* drho, const void* ux, const void* uy,

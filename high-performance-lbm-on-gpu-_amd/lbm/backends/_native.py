@@ -24,10 +24,10 @@ _i, _d, _p, _i64 = c_int, c_double, c_void_p, c_int64
 SIGNATURES: dict[str, list] = {
     "lbm_lattice_tables": [_i, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_double)],
     "lbm_lattice_tables_device": [_i, _i, _p, _p, _p, _p],
-    "lbm_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _p],
-    "lbm_aa_even": [_i, _i, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _p],
-    "lbm_aa_odd": [_i, _i, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _p],
-    "lbm_lds_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _p],
+    "lbm_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _i, _p],
+    "lbm_aa_even": [_i, _i, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _i, _p],
+    "lbm_aa_odd": [_i, _i, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _i, _p],
+    "lbm_lds_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _i, _p],
     "lbm_init_equilibrium": [_i, _i, _p, _p, _p, _p, _p, _i64, _i, _p],
     "lbm_macroscopic": [_i, _i, _p, _p, _p, _i64, _i, _d, _d, _d, _i, _p, _p, _p],
     "lbm_health": [_i, _i, _p, _p, _i64, _i, _p, _p],

@@ -34,8 +34,12 @@ def candidate_blocks(nx: int, variant: str = "pull"):
     return [b for b in BLOCKS if b[0] <= limit]
 
 
-def tune_block(solver, candidates=None, reps: int = 6) -> tuple[int, int]:
-    """Fastest (bx, by) for a ``HipSolver``; its state is restored afterwards."""
+SOLVER_CAPS = (0, 2, 4)
+"""Resident-workgroup caps per compute unit tried with every block shape (0: none)."""
+
+
+def tune_block(solver, candidates=None, caps=SOLVER_CAPS, reps: int = 6) -> tuple[tuple[int, int], int]:
+    """Fastest (block, groups_per_cu) for a ``HipSolver``; its state is restored afterwards."""
     p = solver.params
     key = ("solver", solver.variant, p.lattice.name, p.dtype, p.nx, p.ny, p.nz, solver._mask_mode)
     if key in _CACHE:
@@ -44,19 +48,20 @@ def tune_block(solver, candidates=None, reps: int = 6) -> tuple[int, int]:
     saved = solver._f.clone()
     saved_next = None if solver._f_next is None else solver._f_next.clone()
     steps = solver.step_multiple
-    best, best_ms = solver.block, float("inf")
+    best, best_ms = (solver.block, solver.groups_per_cu), float("inf")
     for block in candidates or candidate_blocks(p.nx, solver.variant):
-        solver.block = block
-        try:
-            ms = _time(torch, lambda: solver._advance(steps), reps)
-        except (ValueError, RuntimeError):
-            continue
-        if ms < best_ms:
-            best, best_ms = block, ms
+        for cap in caps:
+            solver.block, solver.groups_per_cu = block, cap
+            try:
+                ms = _time(torch, lambda: solver._advance(steps), reps)
+            except (ValueError, RuntimeError):
+                continue
+            if ms < best_ms:
+                best, best_ms = (block, cap), ms
     solver._f.copy_(saved)
     if saved_next is not None:
         solver._f_next.copy_(saved_next)
-    solver.block = best
+    solver.block, solver.groups_per_cu = best
     _CACHE[key] = best
     return best
 

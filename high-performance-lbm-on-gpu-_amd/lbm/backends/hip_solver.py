@@ -51,13 +51,23 @@ def check_block(block) -> tuple[int, int]:
 
 
 def default_block(params, variant: str) -> tuple[int, int]:
-    """Whole rows per workgroup where possible: bx covers nx (a multiple of the
-    64-lane wavefront, at most 256) and by stacks rows up to 1024 threads, the
-    fastest shape measured at 256^3 (256 x 4: 4 % above 128 x 2, 27 % above 64 x 4)."""
+    """Whole rows per workgroup where possible: bx covers nx (a multiple of the 64-lane
+    wavefront, at most 256) and by stacks rows, up to 1024 threads in 2D and 512 in 3D.
+    At 256^3 (D3Q19 fp32 AA) 256 x 2 with two resident groups per CU (see
+    ``default_groups_per_cu``) moves 5.61 TB/s per AA pair against 5.36 TB/s for 256 x 4
+    uncapped; 256 x 4 was 4 % above 128 x 2 and 27 % above 64 x 4."""
     if variant == "lds":
         return (64, 4)
     bx = min(256, 64 * math.ceil(params.nx / 64))
-    return bx, max(1, min(params.ny, 1024 // bx))
+    lanes = 1024 if params.d == 2 else 512
+    return bx, max(1, min(params.ny, lanes // bx))
+
+
+def default_groups_per_cu(params, variant: str) -> int:
+    """Resident workgroups per compute unit for the stream-collide kernels (0: no cap).
+    D3Q19 runs 16 wavefronts per CU (two 512-lane groups): fewer in-flight streams per
+    CU keep the 38 population planes' DRAM pages open longer (measured at 256^3)."""
+    return 2 if params.d == 3 and variant != "lds" else 0
 
 
 def node_kinds(lattice: VelocitySet, solid: np.ndarray, forced_nodes=None) -> np.ndarray:
@@ -102,13 +112,21 @@ class HipSolver(Solver):
         params: Simulation description.
         variant: ``"pull"``, ``"aa"`` or ``"lds"`` (2D only).
         block: Threads per block (bx, by), at most 1024; default from ``default_block``.
-        autotune: Time a few block shapes on this grid and keep the fastest.
+        groups_per_cu: Cap on resident workgroups per compute unit (0: none); default from
+            ``default_groups_per_cu``.
+        autotune: Time a few block shapes and caps on this grid and keep the fastest.
         graphs: Capture two steps (pre-passes and kernels) into a HIP graph and replay it.
     """
 
-    def __init__(self, params, variant: str = "pull", block=None, autotune: bool = False, graphs: bool = False):
+    def __init__(self, params, variant: str = "pull", block=None, autotune: bool = False, graphs: bool = False,
+                 groups_per_cu=None):
         check_variant(variant, params.d)
         block = default_block(params, variant) if block is None else check_block(block)
+        if groups_per_cu is None:
+            groups_per_cu = default_groups_per_cu(params, variant)
+        if int(groups_per_cu) < 0:
+            raise ValueError(f"groups_per_cu must be >= 0; got {groups_per_cu}")
+        self._groups_per_cu = int(groups_per_cu)
         super().__init__(params)
         torch = _require_device()
         self._lib = load_library()
@@ -166,7 +184,7 @@ class HipSolver(Solver):
         if autotune:
             from lbm.backends.autotune import tune_block
 
-            self.block = tune_block(self)
+            self.block, self.groups_per_cu = tune_block(self)
 
     @property
     def block(self) -> tuple[int, int]:
@@ -175,6 +193,17 @@ class HipSolver(Solver):
     @block.setter
     def block(self, value) -> None:
         self._block = check_block(value)
+        self._graph = None
+
+    @property
+    def groups_per_cu(self) -> int:
+        return self._groups_per_cu
+
+    @groups_per_cu.setter
+    def groups_per_cu(self, value) -> None:
+        if int(value) < 0:
+            raise ValueError(f"groups_per_cu must be >= 0; got {value}")
+        self._groups_per_cu = int(value)
         self._graph = None
 
     # ------------------------------------------------------------------ plumbing
@@ -258,7 +287,8 @@ class HipSolver(Solver):
         wp, wm = self._rates
         fx, fy, fz = self._force3
         bx, by = self.block
-        return (p.nx, p.ny, self._nz, wp, wm, fx, fy, fz, self._force_mode, self._mask_mode, bx, by, self._stream)
+        return (p.nx, p.ny, self._nz, wp, wm, fx, fy, fz, self._force_mode, self._mask_mode, bx, by,
+                self._groups_per_cu, self._stream)
 
     def _advance(self, steps: int) -> None:
         if self.graphs and steps >= 2:
@@ -431,4 +461,5 @@ class HipSolver(Solver):
         return out
 
 
-__all__ = ["HipSolver", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block"]
+__all__ = ["HipSolver", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block",
+           "default_groups_per_cu"]

@@ -56,6 +56,11 @@ LBM_API int lbm_lattice_tables(int lattice, int* q, int* c, int* opp, double* w)
  * Replaces the on-device constant check of tests/test_cuda_backend.py:160-183. */
 LBM_API int lbm_lattice_tables_device(int lattice, int dtype, int* c, int* opp, void* w, void* stream);
 
+/* Stream-collide entry points (lbm_pull_step, lbm_aa_even, lbm_aa_odd, lbm_lds_step):
+ * groups_per_cu > 0 caps the resident workgroups per compute unit (requested as unused
+ * LDS: the bandwidth of the AA pair at 256^3 peaks with 2 groups of 256 x 2 per CU);
+ * 0 leaves occupancy to the hardware. */
+
 /* Fused pull stream + halfway bounce-back (Ladd) + TRT/Guo collision, two lattices.
  * Replaces stream_collide_naive (src/lbm/backends/kernels/stream_collide_naive.cu:15-49),
  * launched from CudaSolver._advance (src/lbm/backends/cuda_solver.py:279-284).
@@ -66,14 +71,14 @@ LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst
                           const uint8_t* kind, const void* wall_u, const void* force_field,
                           const void* body_field, int nx, int ny, int nz, double omega_plus,
                           double omega_minus, double fx, double fy, double fz, int force_mode,
-                          int mask_mode, int bx, int by, void* stream);
+                          int mask_mode, int bx, int by, int groups_per_cu, void* stream);
 
 /* AA pattern, even half-step: read F[i](x), collide, write F[opp i](x).
  * Replaces aa_even (src/lbm/backends/kernels/stream_collide_aa.cu:33-57). */
 LBM_API int lbm_aa_even(int lattice, int dtype, void* f, const uint8_t* kind,
                         const void* force_field, const void* body_field, int nx, int ny, int nz,
                         double omega_plus, double omega_minus, double fx, double fy, double fz,
-                        int force_mode, int mask_mode, int bx, int by, void* stream);
+                        int force_mode, int mask_mode, int bx, int by, int groups_per_cu, void* stream);
 
 /* AA pattern, odd half-step: pull F[opp i](x - c_i), collide, push to F[i](x + c_i);
  * solid targets are reflected into F[opp i](x) with the wall term.
@@ -81,7 +86,7 @@ LBM_API int lbm_aa_even(int lattice, int dtype, void* f, const uint8_t* kind,
 LBM_API int lbm_aa_odd(int lattice, int dtype, void* f, const uint8_t* kind, const void* wall_u,
                        const void* force_field, const void* body_field, int nx, int ny, int nz,
                        double omega_plus, double omega_minus, double fx, double fy, double fz,
-                       int force_mode, int mask_mode, int bx, int by, void* stream);
+                       int force_mode, int mask_mode, int bx, int by, int groups_per_cu, void* stream);
 
 /* Two-lattice step with each block's populations plus a one-node halo staged in LDS
  * (2D only; tile = block). Replaces stream_collide_tiled
@@ -90,7 +95,7 @@ LBM_API int lbm_lds_step(int lattice, int dtype, const void* f_src, void* f_dst,
                          const uint8_t* kind, const void* wall_u, const void* force_field,
                          const void* body_field, int nx, int ny, int nz, double omega_plus,
                          double omega_minus, double fx, double fy, double fz, int force_mode,
-                         int mask_mode, int bx, int by, void* stream);
+                         int mask_mode, int bx, int by, int groups_per_cu, void* stream);
 
 /* Shifted equilibrium from drho and u (device arrays of dtype, n nodes each; uz
  * ignored in 2D) into f; aa != 0 stores direction i in slot opp(i) (post-even layout).

@@ -115,6 +115,16 @@ def test_results_do_not_depend_on_the_block_shape(block):
     np.testing.assert_array_equal(other.populations(), ref.populations())
 
 
+@pytest.mark.parametrize("backend", ["hip-pull", "hip-aa", "hip-lds"])
+@pytest.mark.parametrize("cap", [1, 2, 4])
+def test_results_do_not_depend_on_the_occupancy_cap(backend, cap):
+    d = 2 if backend == "hip-lds" else 3
+    params = mixed_params("float64", "trt", d)
+    _, ref = run_pair(params, backend, steps=10, groups_per_cu=0)
+    _, other = run_pair(params, backend, steps=10, groups_per_cu=cap)
+    np.testing.assert_array_equal(other.populations(), ref.populations())
+
+
 def test_device_lattice_tables_match_the_host_tables():
     import torch
 

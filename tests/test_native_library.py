@@ -69,11 +69,11 @@ def test_invalid_block_and_grid_are_rejected_by_the_library_before_launch():
     lib = library()
     args = (0, 0, 1, 2, None, None, None, None, 8, 8, 1, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0)
     with pytest.raises(ValueError, match="block"):
-        lib.call("lbm_pull_step", *args, 64, 32, None)
+        lib.call("lbm_pull_step", *args, 64, 32, 0, None)
     with pytest.raises(ValueError, match="nz"):
-        lib.call("lbm_pull_step", 0, 0, 1, 2, None, None, None, None, 8, 8, 4, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 32, 8, None)
+        lib.call("lbm_pull_step", 0, 0, 1, 2, None, None, None, None, 8, 8, 4, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 32, 8, 0, None)
     with pytest.raises(ValueError, match="2D only"):
-        lib.call("lbm_lds_step", 1, 0, 1, 2, None, None, None, None, 8, 8, 8, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 32, 8, None)
+        lib.call("lbm_lds_step", 1, 0, 1, 2, None, None, None, None, 8, 8, 8, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 32, 8, 0, None)
     with pytest.raises(ValueError, match="terms"):
         lib.call("lbm_rules_apply", 0, 1, 1, 1, 1, 1, 5, 200, None, None)
 

@@ -204,6 +204,7 @@ def batched_measure(torch, count: int = 64, steps: int = 100) -> dict:
         "speedup": round(t_alone / t_batch, 3),
         "block": list(batch.block),
         "groups": batch.groups,
+        "groups_per_cu": batch.groups_per_cu,
         "host_build_seconds": round(t_build, 1),
     }
 

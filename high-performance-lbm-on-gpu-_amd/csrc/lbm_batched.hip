@@ -3,7 +3,9 @@
 // No counterpart in the reference (one lattice per solver there). All lattices keep their
 // populations in one pool (lattice l at element offset f_offset[l], its own SoA planes),
 // their kind bytes and wall velocities in two more pools. A work table lists every
-// (lattice, tile) of bx x by nodes; a fixed grid of workgroups walks it with a static
+// (lattice, tile), a tile being a run of consecutive node indices of one lattice as long
+// as the workgroup (rows wrap inside a tile, so lattices of any width leave no lane idle
+// except in their last tile); a fixed grid of workgroups walks the table with a static
 // stride (workgroup w takes tiles w, w + G, w + 2G, ...), so the assignment is
 // deterministic and no atomics are involved. Each node runs pull_update, the code of the
 // single-lattice pull kernel, with its lattice's own grid and relaxation constants, so a
@@ -25,7 +27,8 @@ struct LatticeDesc {
 };
 
 struct TileDesc {
-    int lattice, x0, y0, z;
+    int lattice;
+    uint32_t start;  // first node index (within the lattice) of the tile
 };
 
 template <class L, class T, bool MASK>
@@ -34,14 +37,18 @@ __global__ __launch_bounds__(1024) void k_batched_pull(const T* __restrict__ src
                                                        const LatticeDesc<T>* __restrict__ lattices,
                                                        const TileDesc* __restrict__ tiles, int n_tiles)
 {
+    const uint32_t lane = threadIdx.y * blockDim.x + threadIdx.x;
     for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
         const TileDesc td = tiles[t];
         const LatticeDesc<T>& d = lattices[td.lattice];
         const Grid g = d.g;
-        Coord c{td.x0 + (int)threadIdx.x, td.y0 + (int)threadIdx.y, td.z};
-        if (c.x < g.nx && c.y < g.ny)
+        const uint32_t node = td.start + lane;
+        if (node < g.n) {
+            const uint32_t row = node / (uint32_t)g.nx;
+            const Coord c{(int)(node - row * (uint32_t)g.nx), (int)(row % (uint32_t)g.ny), (int)(row / (uint32_t)g.ny)};
             pull_update<L, T, ForceMode::UNIFORM, MASK>(src + d.f_offset, dst + d.f_offset, kind + d.kind_offset,
                                                        wall + d.wall_offset, nullptr, nullptr, g, d.r, c);
+        }
     }
 }
 
@@ -91,12 +98,14 @@ LBM_API int lbm_batched_describe(int lattice, int dtype, int n_lattices, const i
 LBM_API int lbm_batched_pull_step(int lattice, int dtype, const void* src_pool, void* dst_pool,
                                   const uint8_t* kind_pool, const void* wall_pool, const void* desc,
                                   const int* tiles, int n_tiles, int mask_mode, int bx, int by, int groups,
-                                  void* stream)
+                                  int groups_per_cu, void* stream)
 {
     return guarded("lbm_batched_pull_step", [&] {
         require(n_tiles > 0 && src_pool && dst_pool && desc && tiles, "bad arguments");
         require(mask_mode == 0 || (kind_pool && wall_pool), "mask_mode 1 needs kind and wall pools");
         require(groups > 0, "groups must be positive");
+        require(groups_per_cu >= 0, "groups_per_cu must be >= 0 (0: no occupancy cap)");
+        const size_t cap = occupancy_lds(groups_per_cu, 0);
         const dim3 block = block_of(bx, by);
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
             using L = decltype(Lt);
@@ -104,14 +113,15 @@ LBM_API int lbm_batched_pull_step(int lattice, int dtype, const void* src_pool, 
             const unsigned grid = (unsigned)std::min(groups, n_tiles);
             auto* td = reinterpret_cast<const TileDesc*>(tiles);
             auto* ld = static_cast<const LatticeDesc<T>*>(desc);
-            if (mask_mode)
-                k_batched_pull<L, T, true><<<grid, block, 0, as_stream(stream)>>>(
+            auto go = [&](auto mk) {
+                constexpr bool MK = decltype(mk)::value;
+                allow_lds(k_batched_pull<L, T, MK>, cap);
+                k_batched_pull<L, T, MK><<<grid, block, cap, as_stream(stream)>>>(
                     static_cast<const T*>(src_pool), static_cast<T*>(dst_pool), kind_pool,
                     static_cast<const T*>(wall_pool), ld, td, n_tiles);
-            else
-                k_batched_pull<L, T, false><<<grid, block, 0, as_stream(stream)>>>(
-                    static_cast<const T*>(src_pool), static_cast<T*>(dst_pool), kind_pool,
-                    static_cast<const T*>(wall_pool), ld, td, n_tiles);
+            };
+            if (mask_mode) go(std::true_type{});
+            else go(std::false_type{});
             check_launch("batched pull step");
         });
     });

@@ -39,7 +39,7 @@ SIGNATURES: dict[str, list] = {
     "lbm_rules_apply_widths": [_i, _p, _p, _p, _p, _p, _p, _i64, _i, _p, _p],
     "lbm_batched_desc_size": [_i, POINTER(c_int64)],
     "lbm_batched_describe": [_i, _i, _i, _p, _p, _p, _p, _p, _p, _p],
-    "lbm_batched_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _p],
+    "lbm_batched_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p],
     "lbm_adj_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _p, _p, _i, _i, _p],
     "lbm_adj_pull_step_rules": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _p, _p,
                                 _i, _i, _p, _p, _p, _p, _p, _i64, _i, _p],

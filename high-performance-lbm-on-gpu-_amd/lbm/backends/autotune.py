@@ -11,6 +11,9 @@ from __future__ import annotations
 BLOCKS = ((256, 1), (256, 2), (256, 4), (128, 2), (128, 4), (128, 8), (64, 4), (64, 8), (64, 16), (512, 1),
           (512, 2), (1024, 1))
 GROUPS_PER_CU = (2, 4, 8, 16, 32)
+BATCHED_BLOCKS = ((256, 1), (512, 1), (1024, 1))
+"""Work-item sizes of the batched step (a work item is a run of consecutive nodes, so only
+the thread count of the workgroup matters)."""
 _CACHE: dict = {}
 
 
@@ -66,8 +69,9 @@ def tune_block(solver, candidates=None, caps=SOLVER_CAPS, reps: int = 6) -> tupl
     return best
 
 
-def tune_batched(batched, reps: int = 4):
-    """Fastest (block, groups) for a ``BatchedSolver``; its state is restored afterwards."""
+def tune_batched(batched, reps: int = 4, caps=(0, 1, 2, 4)):
+    """Fastest (block, groups, groups_per_cu) for a ``BatchedSolver``; its state is restored
+    afterwards."""
     key = ("batched", batched.lattice.name, batched.dtype, tuple(batched.nodes))
     if key in _CACHE:
         return _CACHE[key]
@@ -75,18 +79,19 @@ def tune_batched(batched, reps: int = 4):
     saved = batched.snapshot()
     graphs = batched.graphs
     batched.graphs = False
-    best, best_ms = (batched.block, batched.groups), float("inf")
-    nx_max = max(p.nx for p in batched.params_list)
-    for block in candidate_blocks(nx_max):
+    best, best_ms = (batched.block, batched.groups, batched.groups_per_cu), float("inf")
+    for block in BATCHED_BLOCKS:
         batched.block = block
         batched._build_tiles()
         for per_cu in GROUPS_PER_CU:
             batched.groups = per_cu * 256
-            ms = _time(torch, batched._pair, reps)
-            if ms < best_ms:
-                best, best_ms = (block, batched.groups), ms
+            for cap in caps:
+                batched.groups_per_cu = cap
+                ms = _time(torch, batched._pair, reps)
+                if ms < best_ms:
+                    best, best_ms = (block, batched.groups, cap), ms
     batched.graphs = graphs
-    batched.block, batched.groups = best
+    batched.block, batched.groups, batched.groups_per_cu = best
     batched._build_tiles()
     batched.restore(saved)
     _CACHE[key] = best

@@ -35,7 +35,8 @@ class BatchedSolver:
 
     Args:
         params_list: ``SimulationParams`` of every lattice (same lattice and dtype, no IBM).
-        block: (bx, by) tile = workgroup shape; default (256, 4).
+        block: workgroup shape (bx, by); a work item is bx * by consecutive nodes of one
+            lattice; default (256, 4).
         groups: persistent workgroups; default 8 per compute unit.
         graphs: capture the two-step cycle into a HIP graph.
         autotune: pick block and groups by timing (``lbm.backends.autotune``).
@@ -66,6 +67,7 @@ class BatchedSolver:
         self.total_nodes = int(sum(self.nodes))
         self.block = check_block(block)
         self.groups = int(groups) if groups else DEFAULT_GROUPS_PER_CU * COMPUTE_UNITS
+        self.groups_per_cu = 0  # cap on resident workgroups per CU (0: none); set by the autotuner
         self.graphs = graphs
         self.time_step = 0
 
@@ -102,7 +104,7 @@ class BatchedSolver:
         if autotune:
             from lbm.backends.autotune import tune_batched
 
-            self.block, self.groups = tune_batched(self)
+            self.block, self.groups, self.groups_per_cu = tune_batched(self)
             self._build_tiles()
 
     # ------------------------------------------------------------------ set-up
@@ -136,14 +138,15 @@ class BatchedSolver:
         self._rule_scratch = torch.empty(self._n_rules, dtype=self._tdt, device=dev)
 
     def _build_tiles(self) -> None:
+        """Work items (lattice, first node): runs of bx * by consecutive nodes of one lattice."""
         bx, by = self.block
-        tiles = []
-        for i, p in enumerate(self.params_list):
-            for z in range(p.nz or 1):
-                for y0 in range(0, p.ny, by):
-                    for x0 in range(0, p.nx, bx):
-                        tiles.append((i, x0, y0, z))
-        self._tiles = self._torch.as_tensor(np.asarray(tiles, dtype=np.int32).ravel(), device=self.device)
+        size = bx * by
+        parts = []
+        for i, n in enumerate(self.nodes):
+            starts = np.arange(0, n, size, dtype=np.int64)
+            parts.append(np.stack([np.full(starts.size, i, dtype=np.int64), starts], axis=1))
+        tiles = np.concatenate(parts).astype(np.int32)
+        self._tiles = self._torch.as_tensor(tiles.ravel(), device=self.device)
         self._n_tiles = len(tiles)
         self._graph = None
 
@@ -179,7 +182,7 @@ class BatchedSolver:
         bx, by = self.block
         self._lib.call("lbm_batched_pull_step", self._lcode, self._dcode, pointer(src), pointer(dst),
                        pointer(self._kind), pointer(self._wall), pointer(self._desc), pointer(self._tiles),
-                       self._n_tiles, self._mask_mode, bx, by, self.groups, stream)
+                       self._n_tiles, self._mask_mode, bx, by, self.groups, self.groups_per_cu, stream)
 
     def _pair(self) -> None:
         self._step(self._fa, self._fb)

@@ -274,13 +274,16 @@ LBM_API int lbm_batched_describe(int lattice, int dtype, int n_lattices, const i
                                  const double* force, void* host_desc);
 
 /* One two-lattice pull step of every lattice of a batch in one launch: `groups` workgroups
- * of bx x by threads walk the n_tiles work items tiles[4 t .. 4 t + 3] = (lattice, x0, y0, z)
- * with a static stride; desc is the device copy of the descriptors. Same per-node
- * arithmetic as lbm_pull_step (uniform force mode). No counterpart in the reference. */
+ * of bx x by threads walk the n_tiles work items tiles[2 t], tiles[2 t + 1] = (lattice,
+ * first node) with a static stride; a work item covers bx * by consecutive node indices
+ * of its lattice (rows wrap inside it). desc is the device copy of the descriptors. Same
+ * per-node arithmetic as lbm_pull_step (uniform force mode); groups_per_cu caps resident
+ * workgroups per compute unit as for the step entry points. No counterpart in the
+ * reference. */
 LBM_API int lbm_batched_pull_step(int lattice, int dtype, const void* src_pool, void* dst_pool,
                                   const uint8_t* kind_pool, const void* wall_pool, const void* desc,
                                   const int* tiles, int n_tiles, int mask_mode, int bx, int by, int groups,
-                                  void* stream);
+                                  int groups_per_cu, void* stream);
 
 /* Per-kernel resources of the compiled step kernels for (lattice, dtype):
  * fills up to max_kernels entries of {registers, scratch (private) bytes per lane,

@@ -42,19 +42,16 @@ TRAFFIC_FILE = ROOT / "profiles" / "traffic_d3q19_aa_256.json"
 
 
 def abc_params(n: int, dtype: str = "float32", tau: float = 0.6):
-    from lbm import SimulationParams
+    from lbm.cases.beltrami import abc_box_params
 
-    return SimulationParams(nx=n, ny=n, nz=n, tau=tau, collision="trt", force=(1e-6, 0.0, 0.0), dtype=dtype)
+    return abc_box_params(n, dtype, tau)
 
 
-def abc_fields(n: int, amplitude: float = 0.02):
-    k = 2.0 * np.pi / n
-    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64),) * 3, indexing="ij")
-    ux = amplitude * np.sin(k * z) + amplitude * np.cos(k * y)
-    uy = amplitude * np.sin(k * x) + amplitude * np.cos(k * z)
-    uz = amplitude * np.sin(k * y) + amplitude * np.cos(k * x)
-    rho = 1.0 + (-0.5 * (ux**2 + uy**2 + uz**2)) * 3.0
-    return rho, ux, uy, uz
+def abc_fields(n: int, amplitude: float = 0.02, tau: float = 0.6):
+    from lbm.cases.beltrami import exact_fields
+    from lbm.lattice import CS2
+
+    return exact_fields(n, amplitude, CS2 * (tau - 0.5), 0.0)
 
 
 def copy_peak_gbs(torch, nbytes: int = 2 * 2**30, repeats: int = 10) -> dict[str, float]:
@@ -209,6 +206,27 @@ def batched_measure(torch, count: int = 64, steps: int = 100) -> dict:
     }
 
 
+def lds_compare(torch, peak_gbs: float, n: int = 4096, steps: int = 40) -> dict:
+    """The LDS-tiled variant against the fused pull kernel, as the reference's benchmark
+    compares them (D2Q9 fp32 periodic n^2, TRT tau = 0.6, 72 B per update)."""
+    from lbm import SimulationParams, make_solver
+
+    params = SimulationParams(nx=n, ny=n, tau=0.6, collision="trt", dtype="float32")
+    out = {"workload": f"D2Q9 fp32 TRT periodic {n}^2, {steps} steps"}
+    for variant in ("pull", "lds"):
+        solver = make_solver(params, backend=f"hip-{variant}")
+        solver._advance(4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solver._advance(steps)
+        torch.cuda.synchronize()
+        mlups = n * n * steps / (time.perf_counter() - t0) / 1e6
+        out[variant] = {"mlups": round(mlups, 1), "fraction_of_copy_peak": round(mlups * 72 / 1e3 / peak_gbs, 4),
+                        "block": list(solver.block)}
+        del solver
+    return out
+
+
 def traffic_from_profile():
     if TRAFFIC_FILE.exists():
         data = json.loads(TRAFFIC_FILE.read_text())
@@ -228,6 +246,7 @@ def main(argv=None) -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-adjoint", action="store_true")
     parser.add_argument("--no-batched", action="store_true")
+    parser.add_argument("--no-lds", action="store_true", help="skip the LDS-tiled vs fused pull comparison")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (got --gpus {args.gpus})", file=sys.stderr)
@@ -294,6 +313,11 @@ def main(argv=None) -> int:
         adjoint = adjoint_ratio(torch)
         print(f"# adjoint: {json.dumps(adjoint)}", flush=True)
 
+    lds = None
+    if not args.no_lds:
+        lds = lds_compare(torch, peak)
+        print(f"# lds vs pull: {json.dumps(lds)}", flush=True)
+
     batched = None
     if not args.no_batched:
         batched = batched_measure(torch)
@@ -341,6 +365,7 @@ def main(argv=None) -> int:
         "cpu_baseline": baseline,
         "adjoint": adjoint,
         "batched": batched,
+        "lds_vs_pull": lds,
     }
     print(json.dumps(record), flush=True)
     return 0

@@ -15,7 +15,7 @@ import pytest
 from lbm.cases.cylinder_benchmark import build, drag_channel, force_history, lift_period
 from lbm.cases.ghia1982 import REYNOLDS_NUMBERS, ghia_centerlines
 from lbm.cases.lid_driven_cavity import cavity_params, run_cavity
-from lbm.cases.sphere_drag import clift_grace_weber, schiller_naumann, sphere_params
+from lbm.cases.sphere_drag import clift_grace_weber, duct_sphere, schiller_naumann, sphere_params
 
 # ---------------------------------------------------------------- host
 
@@ -82,6 +82,13 @@ def test_sphere_box_geometry():
     assert len(links.node) > 0
     with pytest.raises(ValueError, match="method"):
         sphere_params(4, 20.0, method="spectral")
+
+
+def test_duct_sphere_geometry():
+    params, links, inflow = duct_sphere(nx=40, ny=16, nz=12, radius=3.0, centre=(12.0, 8.0, 6.0))
+    assert (params.nz, params.ny, params.nx) == (12, 16, 40)
+    assert inflow[0].max() <= 0.075 and inflow[0][0].max() == 0.0 and inflow[0][:, 0].max() == 0.0
+    assert len(links.node) > 0 and len(params.link_rules) > 0
 
 
 # ---------------------------------------------------------------- GPU
@@ -189,3 +196,51 @@ def test_bouzidi_sphere_drag_at_re_20():
 
     r = run_sphere_drag(10, 20.0, "bouzidi")
     assert r.cd == pytest.approx(r.reference_cgw, rel=0.03)
+
+
+@pytest.mark.gpu
+def test_abc_256_cube_matches_the_oracle_and_aa_equals_pull():
+    """The throughput configuration at its full size (D3Q19 fp32 256^3): two steps of the
+    pull kernel against the NumPy oracle (float32 tolerance), and ten AA steps bit for bit
+    equal to ten pull steps."""
+    from lbm import make_solver
+    from lbm.cases.beltrami import abc_box_params, exact_fields
+    from lbm.lattice import CS2
+    from oracle import ReferenceSolver
+
+    params = abc_box_params(256)
+    fields = exact_fields(256, 0.02, CS2 * (params.tau - 0.5), 0.0)
+    pull = make_solver(params, backend="hip-pull")
+    pull.initialize(*fields)
+    pull.run(2, check_every=2)
+    ref = ReferenceSolver(params)
+    ref.initialize(*fields)
+    ref.run(2, check_every=2)
+    assert np.abs(ref.populations() - pull.populations()).max() <= 1e-6
+    del ref
+    aa = make_solver(params, backend="hip-aa")
+    aa.initialize(*fields)
+    aa.run(10, check_every=10)
+    pull.run(8, check_every=8)
+    assert np.array_equal(aa.populations(), pull.populations())
+
+
+@pytest.mark.gpu
+def test_duct_sphere_512x128x128_pull_and_aa_agree_with_per_step_drag():
+    """The 3D drag duct at full size: 20 AA steps equal 20 pull steps bit for bit, and the
+    drag history recorded on the device is finite and positive once the flow reaches the
+    sphere."""
+    from lbm import make_solver
+    from lbm.forces import ForceProbe
+
+    params, links, inflow = duct_sphere()
+    probe = ForceProbe.build(params, links)
+    ux = np.where(params.solid, 0.0, inflow[0][:, :, None])
+    pull, aa = make_solver(params, backend="hip-pull"), make_solver(params, backend="hip-aa")
+    for s in (pull, aa):
+        s.initialize(1.0, ux, 0.0, 0.0)
+    history = force_history(pull, probe, 20)
+    aa.run(20, check_every=20)
+    fluid = params.fluid
+    assert np.array_equal(aa.populations()[:, fluid], pull.populations()[:, fluid])
+    assert np.all(np.isfinite(history)) and history[-1, 0] > 0.0

@@ -227,6 +227,27 @@ def lds_compare(torch, peak_gbs: float, n: int = 4096, steps: int = 40) -> dict:
     return out
 
 
+def two_lattice_measure(torch, n: int, peak_gbs: float, steps: int = 200) -> dict:
+    """The two-lattice (pull) kernel on the same workload, as the reference's benchmark times
+    both layouts (152 B per update, two population buffers)."""
+    from lbm import make_solver
+
+    params = abc_params(n)
+    solver = make_solver(params, backend="hip-pull")
+    solver.initialize(*abc_fields(n))
+    solver._advance(4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver._advance(steps)
+    torch.cuda.synchronize()
+    mlups = n**3 * steps / (time.perf_counter() - t0) / 1e6
+    out = {"workload": f"D3Q19 fp32 TRT+Guo periodic ABC box {n}^3, two-lattice pull, {steps} steps",
+           "mlups": round(mlups, 1), "fraction_of_copy_peak": round(mlups * 152 / 1e3 / peak_gbs, 4),
+           "block": list(solver.block), "groups_per_cu": solver.groups_per_cu}
+    del solver
+    return out
+
+
 def traffic_from_profile():
     if TRAFFIC_FILE.exists():
         data = json.loads(TRAFFIC_FILE.read_text())
@@ -247,6 +268,7 @@ def main(argv=None) -> int:
     parser.add_argument("--no-adjoint", action="store_true")
     parser.add_argument("--no-batched", action="store_true")
     parser.add_argument("--no-lds", action="store_true", help="skip the LDS-tiled vs fused pull comparison")
+    parser.add_argument("--no-two-lattice", action="store_true", help="skip the two-lattice pull timing")
     args = parser.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (got --gpus {args.gpus})", file=sys.stderr)
@@ -298,6 +320,7 @@ def main(argv=None) -> int:
     print(f"# timed {args.steps} steps: host wall {wall * 1e3:.2f} ms (events {gpu_seconds * 1e3:.2f} ms), "
           f"{mlups:.0f} MLUPS, {gbs:.0f} GB/s = {100 * gbs / peak:.1f}% of copy peak", flush=True)
 
+    block, groups_per_cu = solver.block, solver.groups_per_cu
     if args.variant == "aa":
         times = kernel_times_ms(torch, solver)
         dominant = max(times, key=times.get)
@@ -307,6 +330,12 @@ def main(argv=None) -> int:
     achieved = nodes * bytes_per_update / (launch_ms / 1e3) / 1e9
     print(f"# per-kernel ms: {json.dumps({k: round(v, 4) for k, v in (times.items() if args.variant == 'aa' else {})})}",
           flush=True)
+
+    two_lattice = None
+    if not args.no_two_lattice and args.variant == "aa":
+        del solver
+        two_lattice = two_lattice_measure(torch, n, peak)
+        print(f"# two-lattice: {json.dumps(two_lattice)}", flush=True)
 
     adjoint = None
     if not args.no_adjoint:
@@ -345,8 +374,8 @@ def main(argv=None) -> int:
             "workload": f"D3Q19 fp32 TRT+Guo periodic ABC box {n}^3, {args.variant.upper()} pattern",
             "nodes": nodes,
             "bytes_per_update": bytes_per_update,
-            "block": list(solver.block),
-            "groups_per_cu": solver.groups_per_cu,
+            "block": list(block),
+            "groups_per_cu": groups_per_cu,
             "copy_peak_gbs": round(peak, 1),
             "copy_kernel_gbs": round(copies["kernel"], 1),
             "copy_runtime_gbs": round(copies["runtime"], 1),
@@ -366,6 +395,7 @@ def main(argv=None) -> int:
         "adjoint": adjoint,
         "batched": batched,
         "lds_vs_pull": lds,
+        "two_lattice": two_lattice,
     }
     print(json.dumps(record), flush=True)
     return 0

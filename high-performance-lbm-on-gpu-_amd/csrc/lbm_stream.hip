@@ -92,10 +92,37 @@ __global__ __launch_bounds__(1024) void k_aa_odd(T* __restrict__ F, const uint8_
     });
 }
 
+// Periodic image of a tile coordinate v in [-1, n + block): one conditional
+// add or subtract, and a modulo only for lanes of a block wider than the grid.
+__device__ __forceinline__ int tile_wrap(int v, int n)
+{
+    if (v < 0) return v + n;
+    if (v < n) return v;
+    v -= n;
+    return v < n ? v : v % n;
+}
+
+// Stage one tile cell (ex, ey) in [0, hx) x [0, hy): all Q planes and the kind byte.
+template <class L, class T, bool MASK>
+__device__ __forceinline__ void stage_cell(T* tile, uint8_t* kind_tile, int cells, int hx, int ex, int ey,
+                                           int x0, int y0, const T* __restrict__ src,
+                                           const uint8_t* __restrict__ kind, const Grid& g)
+{
+    const int gx = tile_wrap(x0 + ex - 1, g.nx), gy = tile_wrap(y0 + ey - 1, g.ny);
+    const uint32_t gnode = (uint32_t)gy * (uint32_t)g.nx + (uint32_t)gx;
+    const int e = ey * hx + ex;
+    T v[L::Q];
+    static_for<0, L::Q>([&](auto I) { v[I] = src[(size_t)I * g.n + gnode]; });
+    kind_tile[e] = MASK ? kind[gnode] : 0;
+    static_for<0, L::Q>([&](auto I) { tile[I * cells + e] = v[I]; });
+}
+
 // LDS-tiled two-lattice step (2D): the block's tile plus a one-node halo of all
 // Q planes and of the kind bytes is staged in LDS, then each thread pulls its
 // neighbours from LDS. Tile = block; out-of-domain threads help load and reach
-// the barrier (src/lbm/backends/kernels/stream_collide_tiled.cu:24-79).
+// the barrier (src/lbm/backends/kernels/stream_collide_tiled.cu:24-79). Each
+// thread stages its own cell (coalesced rows, no index division); the halo ring
+// of 2 (bx + 2) + 2 by cells is staged by the first threads in a second pass.
 template <class L, class T, ForceMode FM, bool MASK>
 __global__ __launch_bounds__(1024) void k_lds(const T* __restrict__ src, T* __restrict__ dst,
                                               const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
@@ -111,15 +138,19 @@ __global__ __launch_bounds__(1024) void k_lds(const T* __restrict__ src, T* __re
     uint8_t* kind_tile = lds_raw + sizeof(T) * (size_t)L::Q * cells;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = blockIdx.x * bx, y0 = blockIdx.y * by;
-    for (int e = ty * bx + tx; e < cells; e += bx * by) {
-        const int ey = e / hx, ex = e - ey * hx;
-        int gx = (x0 + ex - 1) % g.nx;
-        gx = gx < 0 ? gx + g.nx : gx;
-        int gy = (y0 + ey - 1) % g.ny;
-        gy = gy < 0 ? gy + g.ny : gy;
-        const uint32_t gnode = (uint32_t)gy * (uint32_t)g.nx + (uint32_t)gx;
-        kind_tile[e] = MASK ? kind[gnode] : 0;
-        static_for<0, L::Q>([&](auto I) { tile[I * cells + e] = src[(size_t)I * g.n + gnode]; });
+    stage_cell<L, T, MASK>(tile, kind_tile, cells, hx, tx + 1, ty + 1, x0, y0, src, kind, g);
+    const int ring = 2 * hx + 2 * by;
+    for (int h = ty * bx + tx; h < ring; h += bx * by) {
+        int ex, ey;
+        if (h < 2 * hx) {
+            ex = h < hx ? h : h - hx;
+            ey = h < hx ? 0 : hy - 1;
+        } else {
+            const int c = h - 2 * hx;
+            ex = c < by ? 0 : hx - 1;
+            ey = 1 + (c < by ? c : c - by);
+        }
+        stage_cell<L, T, MASK>(tile, kind_tile, cells, hx, ex, ey, x0, y0, src, kind, g);
     }
     __syncthreads();
     const int x = x0 + tx, y = y0 + ty;

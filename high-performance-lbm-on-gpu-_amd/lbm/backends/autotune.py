@@ -32,7 +32,7 @@ def _time(torch, fn, reps: int) -> float:
 def candidate_blocks(nx: int, variant: str = "pull"):
     """Shapes whose x extent does not exceed the row much (no mostly idle wavefronts)."""
     if variant == "lds":
-        return [(64, 4), (32, 8), (64, 8), (128, 4), (32, 16)]
+        return [(256, 4), (128, 8), (64, 16), (512, 2), (128, 4), (64, 8)]
     limit = max(64, 64 * -(-nx // 64))
     return [b for b in BLOCKS if b[0] <= limit]
 

@@ -55,9 +55,10 @@ def default_block(params, variant: str) -> tuple[int, int]:
     wavefront, at most 256) and by stacks rows, up to 1024 threads in 2D and 512 in 3D.
     At 256^3 (D3Q19 fp32 AA) 256 x 2 with two resident groups per CU (see
     ``default_groups_per_cu``) moves 5.61 TB/s per AA pair against 5.36 TB/s for 256 x 4
-    uncapped; 256 x 4 was 4 % above 128 x 2 and 27 % above 64 x 4."""
-    if variant == "lds":
-        return (64, 4)
+    uncapped; 256 x 4 was 4 % above 128 x 2 and 27 % above 64 x 4. The LDS-tiled kernel
+    takes the same shapes: its halo ring costs 2 (bx + 2) + 2 by cells per bx * by, so
+    1024-lane tiles win (D2Q9 fp32 4096^2: 256 x 4 and 128 x 8 at 76 K MLUPS against 62 K
+    for 64 x 4 and 55 K for 32 x 8)."""
     bx = min(256, 64 * math.ceil(params.nx / 64))
     lanes = 1024 if params.d == 2 else 512
     return bx, max(1, min(params.ny, lanes // bx))

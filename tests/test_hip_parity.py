@@ -115,6 +115,24 @@ def test_results_do_not_depend_on_the_block_shape(block):
     np.testing.assert_array_equal(other.populations(), ref.populations())
 
 
+@pytest.mark.parametrize("shape", [(45, 29), (8, 6), (130, 70)])
+@pytest.mark.parametrize("block", [(64, 4), (32, 8), (256, 4), (64, 16), (16, 2)])
+def test_lds_tile_equals_pull_for_any_tile_and_grid(shape, block):
+    """The LDS tile (own cell plus halo ring, periodic images for tiles wider or taller than
+    the grid) gives the pull kernel's populations bit for bit."""
+    nx, ny = shape
+    rng = np.random.default_rng(3)
+    solid = np.zeros((ny, nx), dtype=bool)
+    solid[rng.random((ny, nx)) < 0.1] = True
+    wall = np.zeros((2, ny, nx))
+    wall[0][solid] = 0.01
+    params = SimulationParams(nx=nx, ny=ny, tau=0.7, collision="trt", dtype="float64", solid=solid,
+                              wall_velocity=wall, force=(1e-5, -2e-6))
+    _, pull = run_pair(params, "hip-pull", steps=6)
+    _, lds = run_pair(params, "hip-lds", steps=6, block=block)
+    np.testing.assert_array_equal(lds.populations()[:, params.fluid], pull.populations()[:, params.fluid])
+
+
 @pytest.mark.parametrize("backend", ["hip-pull", "hip-aa", "hip-lds"])
 @pytest.mark.parametrize("cap", [1, 2, 4])
 def test_results_do_not_depend_on_the_occupancy_cap(backend, cap):

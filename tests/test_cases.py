@@ -91,7 +91,24 @@ def test_duct_sphere_geometry():
     assert len(links.node) > 0 and len(params.link_rules) > 0
 
 
+def test_convergence_table_reports_observed_order():
+    from lbm.cases.taylor_green import DecayResult, convergence_table
+
+    table = convergence_table("t", [DecayResult(16, 100, 1e-3, 1.0, 1.01), DecayResult(32, 400, 2.5e-4, 1.0, 1.001)])
+    assert table.splitlines()[-1].split()[3] == "2.00"
+
+
 # ---------------------------------------------------------------- GPU
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case, sizes", [("taylor_green", ["16", "32"]), ("beltrami", ["8", "16"])])
+def test_decay_case_clis_show_second_order(case, sizes, capsys):
+    import importlib
+
+    importlib.import_module(f"lbm.cases.{case}").main(["--sizes", *sizes])
+    last = capsys.readouterr().out.strip().splitlines()[-1].split()
+    assert 1.8 < float(last[3]) < 2.3
 
 
 def _states_close(params, ref, gpu, rtol):

@@ -54,31 +54,11 @@ def abc_fields(n: int, amplitude: float = 0.02, tau: float = 0.6):
     return exact_fields(n, amplitude, CS2 * (tau - 0.5), 0.0)
 
 
-def copy_peak_gbs(torch, nbytes: int = 2 * 2**30, repeats: int = 10) -> dict[str, float]:
-    """Read + write GB/s of two device copies of 2 GiB: the library copy kernel (one 16-byte
-    nontemporal access per lane) and the runtime's device-to-device copy (torch ``copy_``,
-    the analogue of the reference's ``cp.copyto`` probe). The measured peak is the larger."""
-    from lbm.backends._native import library
+def copy_peak_gbs(torch) -> dict[str, float]:
+    """The device-copy peak on 2 GiB buffers (``lbm.bench_mlups.copy_peak``)."""
+    from lbm.bench_mlups import copy_peak
 
-    lib = library()
-    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
-    stream = torch.cuda.current_stream()
-
-    def rate(copy) -> float:
-        copy()
-        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        start.record(stream)
-        for _ in range(repeats):
-            copy()
-        stop.record(stream)
-        stop.synchronize()
-        return 2 * nbytes * repeats / (start.elapsed_time(stop) / 1e3) / 1e9
-
-    kernel = rate(lambda: lib.call("lbm_device_copy", dst.data_ptr(), src.data_ptr(), nbytes, stream.cuda_stream))
-    runtime = rate(lambda: dst.copy_(src))
-    del src, dst
-    return {"kernel": kernel, "runtime": runtime, "peak": max(kernel, runtime)}
+    return copy_peak(torch)
 
 
 def kernel_times_ms(torch, solver, pairs: int = 20) -> dict[str, float]:

@@ -37,7 +37,7 @@ def candidate_blocks(nx: int, variant: str = "pull"):
     return [b for b in BLOCKS if b[0] <= limit]
 
 
-SOLVER_CAPS = (0, 2, 4)
+SOLVER_CAPS = (0, 1, 2, 3, 4)
 """Resident-workgroup caps per compute unit tried with every block shape (0: none)."""
 
 

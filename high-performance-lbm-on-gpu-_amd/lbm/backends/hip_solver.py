@@ -50,25 +50,40 @@ def check_block(block) -> tuple[int, int]:
     return bx, by
 
 
+# D3Q19 launch shape per (variant, dtype): lanes per workgroup and resident workgroups per
+# CU, measured at 256^3 (MLUPS against the former common default 256 x 2, cap 2):
+#   fp32 AA    256 x 2, cap 2 (16 wavefronts/CU)   37.5 K
+#   fp32 pull  256 x 1, cap 4 (16 wavefronts/CU)   38.2 K (was 36.1 K)
+#   fp64 both  256 x 1, cap 2 ( 8 wavefronts/CU)   AA 17.6 K (was 16.4 K), pull 17.5 K (17.0 K)
+# Fewer concurrent address streams per CU keep the 38 population planes' DRAM pages open
+# longer; float64 moves twice the bytes per lane, so half the wavefronts saturate HBM.
+_SHAPE_3D = {("aa", "float32"): (512, 2), ("pull", "float32"): (256, 4), ("aa", "float64"): (256, 2),
+             ("pull", "float64"): (256, 2)}
+
+
 def default_block(params, variant: str) -> tuple[int, int]:
     """Whole rows per workgroup where possible: bx covers nx (a multiple of the 64-lane
-    wavefront, at most 256) and by stacks rows, up to 1024 threads in 2D and 512 in 3D.
-    At 256^3 (D3Q19 fp32 AA) 256 x 2 with two resident groups per CU (see
-    ``default_groups_per_cu``) moves 5.61 TB/s per AA pair against 5.36 TB/s for 256 x 4
+    wavefront, at most 256) and by stacks rows up to 1024 lanes in 2D, or the lanes of the
+    measured D3Q19 shape. At 256^3 (D3Q19 fp32 AA) 256 x 2 with two resident groups per CU
+    (see ``default_groups_per_cu``) moves 5.61 TB/s per AA pair against 5.36 TB/s for 256 x 4
     uncapped; 256 x 4 was 4 % above 128 x 2 and 27 % above 64 x 4. The LDS-tiled kernel
-    takes the same shapes: its halo ring costs 2 (bx + 2) + 2 by cells per bx * by, so
+    takes the 2D shapes: its halo ring costs 2 (bx + 2) + 2 by cells per bx * by, so
     1024-lane tiles win (D2Q9 fp32 4096^2: 256 x 4 and 128 x 8 at 76 K MLUPS against 62 K
     for 64 x 4 and 55 K for 32 x 8)."""
     bx = min(256, 64 * math.ceil(params.nx / 64))
-    lanes = 1024 if params.d == 2 else 512
+    if params.d == 2:
+        lanes = 1024
+    else:
+        lanes = _SHAPE_3D[("aa" if variant == "aa" else "pull", params.dtype)][0]
     return bx, max(1, min(params.ny, lanes // bx))
 
 
 def default_groups_per_cu(params, variant: str) -> int:
-    """Resident workgroups per compute unit for the stream-collide kernels (0: no cap).
-    D3Q19 runs 16 wavefronts per CU (two 512-lane groups): fewer in-flight streams per
-    CU keep the 38 population planes' DRAM pages open longer (measured at 256^3)."""
-    return 2 if params.d == 3 and variant != "lds" else 0
+    """Resident workgroups per compute unit for the stream-collide kernels (0: no cap):
+    the measured D3Q19 caps above; 2D lattices run uncapped."""
+    if params.d == 2:
+        return 0
+    return _SHAPE_3D[("aa" if variant == "aa" else "pull", params.dtype)][1]
 
 
 def node_kinds(lattice: VelocitySet, solid: np.ndarray, forced_nodes=None) -> np.ndarray:
@@ -137,7 +152,7 @@ class HipSolver(Solver):
         self.step_multiple = 2 if variant == "aa" else 1
         self._block = block
         self.graphs = graphs
-        self._graph = None
+        self._graphs = {}
         p = params
         self._np_dtype = np.dtype(p.dtype)
         self._torch_dtype = {"float32": torch.float32, "float64": torch.float64}[p.dtype]
@@ -194,7 +209,7 @@ class HipSolver(Solver):
     @block.setter
     def block(self, value) -> None:
         self._block = check_block(value)
-        self._graph = None
+        self._graphs.clear()
 
     @property
     def groups_per_cu(self) -> int:
@@ -205,7 +220,7 @@ class HipSolver(Solver):
         if int(value) < 0:
             raise ValueError(f"groups_per_cu must be >= 0; got {value}")
         self._groups_per_cu = int(value)
-        self._graph = None
+        self._graphs.clear()
 
     # ------------------------------------------------------------------ plumbing
 
@@ -246,12 +261,12 @@ class HipSolver(Solver):
         if field is None:
             self._body = None
             self._force_mode = FORCE_FIELD if self._ibm is not None else FORCE_UNIFORM
-            self._graph = None
+            self._graphs.clear()
             return
         t = self._torch.as_tensor(np.asarray(field), dtype=self._torch_dtype, device=self.device)
         self._body = t.reshape(self.params.d, self._n).contiguous()
         self._force_mode = FORCE_DENSE
-        self._graph = None
+        self._graphs.clear()
 
     # ------------------------------------------------------------------ Solver API
 
@@ -277,7 +292,7 @@ class HipSolver(Solver):
         )
         if self._f_next is not None:
             self._f_next.copy_(self._f)
-        self._graph = None
+        self._graphs.clear()
         if self._field is not None:
             self._field.zero_()
             self._body_force.zero_()
@@ -302,17 +317,21 @@ class HipSolver(Solver):
         self._advance_eager(steps)
 
     def _advance_pair_graphed(self) -> None:
-        """Two steps from one captured HIP graph (buffers return to their places)."""
+        """Two steps from a captured HIP graph (buffers return to their places). The
+        two-lattice variants keep one graph per buffer holding the state, since an odd
+        number of steps run in between swaps the roles of the two buffers."""
         torch = self._torch
-        if self._graph is None:
+        key = self._f.data_ptr()
+        graph = self._graphs.get(key)
+        if graph is None:
             self._advance_eager(2)
             torch.cuda.synchronize()
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.graph(graph):
                 self._advance_eager(2)
-            self._graph = graph
+            self._graphs[key] = graph
         else:
-            self._graph.replay()
+            graph.replay()
 
     def _advance_eager(self, steps: int) -> None:
         L, D = self._lattice_code, self._dtype_code

@@ -133,16 +133,17 @@ __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T*
             if (kind[nbr<L, I, -1>(x, y, z, g)] & KIND_SOLID) blocked |= 1u << I;
         });
     }
+    // Selected slots instead of branches keep all loads (and below, all stores) of the
+    // node in flight together; moving-wall terms of blocked links are added afterwards.
     T fs[L::Q], lam[L::Q];
     static_for<0, L::Q>([&](auto I) {
         constexpr int i = I;
         const uint32_t up = nbr<L, i, -1>(x, y, z, g);
-        if (MASK && ((blocked >> i) & 1u))
-            fs[i] = f[(size_t)L::opp[i] * g.n + node] + wall_term<L, i>(wall_u, up, g.n);
-        else
-            fs[i] = f[(size_t)i * g.n + up];
+        const bool bounce = MASK && ((blocked >> i) & 1u);
+        fs[i] = f[bounce ? (size_t)L::opp[i] * g.n + node : (size_t)i * g.n + up];
         lam[i] = lam_next[(size_t)i * g.n + node];
     });
+    if (MASK && blocked) add_wall_terms<L, -1>(fs, blocked, wall_u, Coord{x, y, z}, g);
     T Fx = r.force[0], Fy = r.force[1], Fz = r.force[2];
     if constexpr (FM == ForceMode::DENSE) {
         Fx = Fx + body[node];
@@ -161,10 +162,8 @@ __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T*
     collide_adjoint<L, FM>(fs, lam, r, Fx, Fy, Fz, mu, Fbx, Fby, Fbz);
     static_for<0, L::Q>([&](auto I) {
         constexpr int i = I;
-        if (MASK && ((blocked >> i) & 1u))
-            lam_out[(size_t)L::opp[i] * g.n + node] = mu[i];
-        else
-            lam_out[(size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)] = mu[i];
+        const bool bounce = MASK && ((blocked >> i) & 1u);
+        lam_out[bounce ? (size_t)L::opp[i] * g.n + node : (size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)] = mu[i];
     });
     if (grad_body) {
         grad_body[node] += (double)Fbx;

@@ -32,7 +32,7 @@ struct TileDesc {
 };
 
 template <class L, class T, bool MASK>
-__global__ __launch_bounds__(1024) void k_batched_pull(const T* __restrict__ src, T* __restrict__ dst,
+__global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_batched_pull(const T* __restrict__ src, T* __restrict__ dst,
                                                        const uint8_t* __restrict__ kind, const T* __restrict__ wall,
                                                        const LatticeDesc<T>* __restrict__ lattices,
                                                        const TileDesc* __restrict__ tiles, int n_tiles)
@@ -106,7 +106,7 @@ LBM_API int lbm_batched_pull_step(int lattice, int dtype, const void* src_pool, 
         require(groups > 0, "groups must be positive");
         require(groups_per_cu >= 0, "groups_per_cu must be >= 0 (0: no occupancy cap)");
         const size_t cap = occupancy_lds(groups_per_cu, 0);
-        const dim3 block = block_of(bx, by);
+        const dim3 block = block_of(bx, by, step_max_threads_of(lattice, dtype));
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
             using L = decltype(Lt);
             using T = decltype(Tt);

@@ -92,9 +92,21 @@ inline Grid make_grid(int lattice, int nx, int ny, int nz)
     return Grid{nx, ny, nz, (uint32_t)n};
 }
 
-inline dim3 block_of(int bx, int by)
+// Lanes per workgroup the stream-collide kernels are compiled for: 1024, except float64
+// D3Q19, whose 19 double populations (plus the bounce-back reloads next to walls) need
+// more than the 128 VGPRs per lane that a 1024-lane workgroup leaves.
+constexpr int step_max_threads(int d, size_t real) { return (d == 3 && real == 8) ? 512 : 1024; }
+
+inline int step_max_threads_of(int lattice, int dtype)
 {
-    require(bx > 0 && by > 0 && bx * by <= 1024, "block must be positive with at most 1024 threads");
+    return step_max_threads(lattice == LBM_D3Q19 ? 3 : 2, dtype == LBM_F64 ? 8 : 4);
+}
+
+inline dim3 block_of(int bx, int by, int max_threads = 1024)
+{
+    require(bx > 0 && by > 0 && bx * by <= max_threads,
+            max_threads == 1024 ? "block must be positive with at most 1024 threads"
+                                : "block must be positive with at most 512 threads for float64 D3Q19");
     return dim3(bx, by, 1);
 }
 

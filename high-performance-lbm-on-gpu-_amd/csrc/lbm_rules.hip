@@ -15,6 +15,8 @@
 // above that, an evaluate launch into scratch followed by a write launch.
 
 #include "lbm_affine.h"
+
+#include <algorithm>
 #include "lbm_common.h"
 
 namespace lbm {
@@ -63,29 +65,71 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_rules_small(T* __restrict__ F
 }
 
 constexpr int FORCE_THREADS = 256;
+constexpr int64_t FORCE_ONE_GROUP_ROWS = 2048;
+constexpr int FORCE_MAX_GROUPS = 1024;
 
-// Force = sum_r weight[r] * value[r] (d components), one workgroup, fixed order.
-template <class T>
-__global__ __launch_bounds__(FORCE_THREADS) void k_affine_force(const T* __restrict__ F,
-                                                                const int64_t* __restrict__ slot,
-                                                                const T* __restrict__ coeff,
-                                                                const T* __restrict__ constant,
-                                                                const double* __restrict__ weight, int64_t n, int m,
-                                                                int d, double* __restrict__ out)
+// Workgroups of the force reduction: one up to FORCE_ONE_GROUP_ROWS rows (the result is
+// written directly), else one per FORCE_THREADS rows up to FORCE_MAX_GROUPS, whose
+// partial sums a second one-workgroup pass adds up. Depends on n only, so the summation
+// order of a given row set never changes.
+inline int force_groups(int64_t n)
 {
-    __shared__ double part[3][FORCE_THREADS];
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int64_t r = threadIdx.x; r < n; r += FORCE_THREADS) {
-        const double v = (double)affine_row(F, slot, coeff, constant, n, m, r);
-        for (int a = 0; a < d; ++a) acc[a] += weight[r * d + a] * v;
-    }
-    for (int a = 0; a < 3; ++a) part[a][threadIdx.x] = acc[a];
+    if (n <= FORCE_ONE_GROUP_ROWS) return 1;
+    return (int)std::min<int64_t>(FORCE_MAX_GROUPS, (n + FORCE_THREADS - 1) / FORCE_THREADS);
+}
+
+// Sum of part[a][0..FORCE_THREADS) into part[a][0] by a fixed tree.
+__device__ __forceinline__ void force_tree(double (&part)[3][FORCE_THREADS])
+{
     __syncthreads();
     for (int width = FORCE_THREADS / 2; width > 0; width /= 2) {
         if ((int)threadIdx.x < width)
             for (int a = 0; a < 3; ++a) part[a][threadIdx.x] += part[a][threadIdx.x + width];
         __syncthreads();
     }
+}
+
+// Force = sum_r weight[r] * value[r] (d components). Lane t of workgroup b takes rows
+// b * FORCE_THREADS + t + k * G * FORCE_THREADS (G workgroups) in increasing k, then a
+// fixed tree; with one workgroup the sum goes to out, else to partial[b][3].
+template <class T>
+__global__ __launch_bounds__(FORCE_THREADS) void k_affine_force(const T* __restrict__ F,
+                                                                const int64_t* __restrict__ slot,
+                                                                const T* __restrict__ coeff,
+                                                                const T* __restrict__ constant,
+                                                                const double* __restrict__ weight, int64_t n, int m,
+                                                                int d, double* __restrict__ out,
+                                                                double* __restrict__ partial)
+{
+    __shared__ double part[3][FORCE_THREADS];
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int64_t stride = (int64_t)gridDim.x * FORCE_THREADS;
+    for (int64_t r = (int64_t)blockIdx.x * FORCE_THREADS + threadIdx.x; r < n; r += stride) {
+        const double v = (double)affine_row(F, slot, coeff, constant, n, m, r);
+        for (int a = 0; a < d; ++a) acc[a] += weight[r * d + a] * v;
+    }
+    for (int a = 0; a < 3; ++a) part[a][threadIdx.x] = acc[a];
+    force_tree(part);
+    if (threadIdx.x == 0) {
+        if (gridDim.x == 1) {
+            for (int a = 0; a < d; ++a) out[a] = part[a][0];
+        } else {
+            for (int a = 0; a < 3; ++a) partial[3 * blockIdx.x + a] = part[a][0];
+        }
+    }
+}
+
+// out[a] = sum over the groups' partial sums, lane-strided in increasing group order,
+// then the same fixed tree.
+__global__ __launch_bounds__(FORCE_THREADS) void k_force_total(const double* __restrict__ partial, int groups, int d,
+                                                               double* __restrict__ out)
+{
+    __shared__ double part[3][FORCE_THREADS];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < groups; b += FORCE_THREADS)
+        for (int a = 0; a < 3; ++a) acc[a] += partial[3 * b + a];
+    for (int a = 0; a < 3; ++a) part[a][threadIdx.x] = acc[a];
+    force_tree(part);
     if (threadIdx.x == 0)
         for (int a = 0; a < d; ++a) out[a] = part[a][0];
 }
@@ -161,19 +205,36 @@ LBM_API int lbm_affine_values(int dtype, const void* f, const int64_t* slot, con
     });
 }
 
+LBM_API int lbm_affine_force_scratch_size(int64_t n_rows, int64_t* bytes)
+{
+    return guarded("lbm_affine_force_scratch_size", [&] {
+        require(n_rows >= 0 && bytes != nullptr, "bad arguments");
+        const int groups = force_groups(n_rows);
+        *bytes = groups == 1 ? 0 : (int64_t)sizeof(double) * 3 * groups;
+    });
+}
+
 LBM_API int lbm_affine_force(int dtype, const void* f, const int64_t* slot, const void* coeff, const void* constant,
-                             const double* weight, int64_t n_rows, int m, int d, double* out, void* stream)
+                             const double* weight, int64_t n_rows, int m, int d, double* out, void* scratch,
+                             void* stream)
 {
     return guarded("lbm_affine_force", [&] {
         require(n_rows >= 0 && m >= 1 && m <= 128, "affine rows need 1 <= m <= 128 terms");
         require(d >= 1 && d <= 3, "d must be 1, 2 or 3");
         require(out != nullptr, "null output");
         require(n_rows == 0 || (f && slot && coeff && constant && weight), "null pointer");
+        const int groups = force_groups(n_rows);
+        require(groups == 1 || scratch != nullptr,
+                "more than 2048 rows need a scratch buffer of lbm_affine_force_scratch_size bytes");
+        double* partial = static_cast<double*>(scratch);
         dispatch_real(dtype, [&](auto Tt) {
             using T = decltype(Tt);
-            k_affine_force<T><<<1, FORCE_THREADS, 0, as_stream(stream)>>>(
-                static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), weight,
-                n_rows, m, d, out);
+            const hipStream_t s = as_stream(stream);
+            k_affine_force<T><<<groups, FORCE_THREADS, 0, s>>>(static_cast<const T*>(f), slot,
+                                                               static_cast<const T*>(coeff),
+                                                               static_cast<const T*>(constant), weight, n_rows, m, d,
+                                                               out, partial);
+            if (groups > 1) k_force_total<<<1, FORCE_THREADS, 0, s>>>(partial, groups, d, out);
             check_launch("affine force");
         });
     });

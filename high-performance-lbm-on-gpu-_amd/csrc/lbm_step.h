@@ -59,6 +59,20 @@ struct Coord {
     int x, y, z;
 };
 
+// f_i += the moving-wall term of link i for every set bit i of `bits`, with u_w read at
+// x + S c_i. All wall-velocity loads of the node issue together (the links' terms are
+// evaluated unconditionally, then selected), so a near-wall wavefront waits once.
+template <class L, int S, class T>
+__device__ __forceinline__ void add_wall_terms(T (&f)[L::Q], uint32_t bits, const T* __restrict__ wall_u,
+                                               const Coord& c, const Grid& g)
+{
+    T w[L::Q];
+    static_for<0, L::Q>([&](auto I) { w[I] = wall_term<L, I>(wall_u, neighbour<L, I, S>(c.x, c.y, c.z, g), g.n); });
+    static_for<0, L::Q>([&](auto I) {
+        if ((bits >> I) & 1u) f[I] = f[I] + w[I];
+    });
+}
+
 __device__ __forceinline__ bool locate(Coord& c, const Grid& g)
 {
     c.x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,20 +97,27 @@ __device__ __forceinline__ void pull_update(const T* __restrict__ src, T* __rest
                                             const Relaxation<T>& r, const Coord& c)
 {
     const uint32_t node = flat(c, g);
+    // The node's kind byte and all Q pulled populations are loaded together (the pulled
+    // slots are valid addresses whatever the node is; a solid node discards them). Only a
+    // node with a solid upstream neighbour waits for the neighbour kinds, then reloads
+    // each population from its selected slot (bounce-back or pulled) and adds the
+    // moving-wall terms, all Q loads of each pass in flight together.
     const uint8_t k = MASK ? kind[node] : 0;
-    if (MASK && (k & KIND_SOLID)) return;
-    const uint32_t blocked = (MASK && (k & KIND_NEAR_SOLID)) ? blocked_upstream<L>(kind, c.x, c.y, c.z, g) : 0u;
-
     T f[L::Q];
-    static_for<0, L::Q>([&](auto I) {
-        constexpr int i = I;
-        const uint32_t up = neighbour<L, i, -1>(c.x, c.y, c.z, g);
-        if (MASK && ((blocked >> i) & 1u)) {
-            f[i] = src[(size_t)L::opp[i] * g.n + node] + wall_term<L, i>(wall_u, up, g.n);
-        } else {
-            f[i] = src[(size_t)i * g.n + up];
+    static_for<0, L::Q>([&](auto I) { f[I] = src[(size_t)I * g.n + neighbour<L, I, -1>(c.x, c.y, c.z, g)]; });
+    if (MASK && (k & KIND_SOLID)) return;
+    if (MASK && (k & KIND_NEAR_SOLID)) {
+        const uint32_t blocked = blocked_upstream<L>(kind, c.x, c.y, c.z, g);
+        if (blocked) {
+            static_for<0, L::Q>([&](auto I) {
+                constexpr int i = I;
+                const bool bounce = (blocked >> i) & 1u;
+                f[i] = src[bounce ? (size_t)L::opp[i] * g.n + node
+                                  : (size_t)i * g.n + neighbour<L, i, -1>(c.x, c.y, c.z, g)];
+            });
+            add_wall_terms<L, -1>(f, blocked, wall_u, c, g);
         }
-    });
+    }
     T Fx = T(0), Fy = T(0), Fz = T(0);
     if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
     collide<L, FM>(f, r, Fx, Fy, Fz);

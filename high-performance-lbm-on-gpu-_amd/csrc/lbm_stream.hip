@@ -22,7 +22,7 @@ namespace lbm {
 
 // Two-lattice fused pull step.
 template <class L, class T, ForceMode FM, bool MASK>
-__global__ __launch_bounds__(1024) void k_pull(const T* __restrict__ src, T* __restrict__ dst,
+__global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_pull(const T* __restrict__ src, T* __restrict__ dst,
                                                const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
                                                const T* __restrict__ field, const T* __restrict__ body, Grid g,
                                                Relaxation<T> r)
@@ -34,17 +34,18 @@ __global__ __launch_bounds__(1024) void k_pull(const T* __restrict__ src, T* __r
 
 // AA even half-step: node-local, slot reversal.
 template <class L, class T, ForceMode FM, bool MASK>
-__global__ __launch_bounds__(1024) void k_aa_even(T* __restrict__ F, const uint8_t* __restrict__ kind,
+__global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_aa_even(T* __restrict__ F, const uint8_t* __restrict__ kind,
                                                   const T* __restrict__ field, const T* __restrict__ body,
                                                   Grid g, Relaxation<T> r)
 {
     Coord c;
     if (!locate(c, g)) return;
     const uint32_t node = flat(c, g);
+    // The kind byte and the Q populations are loaded together; a solid node discards them.
     const uint8_t k = MASK ? kind[node] : 0;
-    if (MASK && (k & KIND_SOLID)) return;
     T f[L::Q];
     static_for<0, L::Q>([&](auto I) { f[I] = F[(size_t)I * g.n + node]; });
+    if (MASK && (k & KIND_SOLID)) return;
     T Fx = T(0), Fy = T(0), Fz = T(0);
     if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
     collide<L, FM>(f, r, Fx, Fy, Fz);
@@ -55,40 +56,56 @@ __global__ __launch_bounds__(1024) void k_aa_even(T* __restrict__ F, const uint8
 // read and written by the same thread only (src/lbm/backends/kernels/stream_collide_aa.cu:15-31),
 // and every thread loads all Q values before it stores any.
 template <class L, class T, ForceMode FM, bool MASK>
-__global__ __launch_bounds__(1024) void k_aa_odd(T* __restrict__ F, const uint8_t* __restrict__ kind,
+__global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_aa_odd(T* __restrict__ F, const uint8_t* __restrict__ kind,
                                                  const T* __restrict__ wall_u, const T* __restrict__ field,
                                                  const T* __restrict__ body, Grid g, Relaxation<T> r)
 {
     Coord c;
     if (!locate(c, g)) return;
     const uint32_t node = flat(c, g);
+    // The kind byte and the Q pulled populations are loaded together (a solid node
+    // discards them; it writes nothing). A node with a solid upstream neighbour then
+    // reloads each population from its selected slot (bounce-back or pulled) and adds the
+    // moving-wall terms. Loads and stores go to selected slots rather than branches, so
+    // all Q of each pass stay in flight together.
     const uint8_t k = MASK ? kind[node] : 0;
-    if (MASK && (k & KIND_SOLID)) return;
-    const uint32_t blocked = (MASK && (k & KIND_NEAR_SOLID)) ? blocked_upstream<L>(kind, c.x, c.y, c.z, g) : 0u;
-
     T f[L::Q];
     static_for<0, L::Q>([&](auto I) {
-        constexpr int i = I;
-        const uint32_t up = neighbour<L, i, -1>(c.x, c.y, c.z, g);
-        if (MASK && ((blocked >> i) & 1u)) {
-            f[i] = F[(size_t)i * g.n + node] + wall_term<L, i>(wall_u, up, g.n);
-        } else {
-            f[i] = F[(size_t)L::opp[i] * g.n + up];
-        }
+        f[I] = F[(size_t)L::opp[I] * g.n + neighbour<L, I, -1>(c.x, c.y, c.z, g)];
     });
+    if (MASK && (k & KIND_SOLID)) return;
+    const uint32_t blocked = (MASK && (k & KIND_NEAR_SOLID)) ? blocked_upstream<L>(kind, c.x, c.y, c.z, g) : 0u;
+    if (MASK && blocked) {
+        static_for<0, L::Q>([&](auto I) {
+            constexpr int i = I;
+            const bool bounce = (blocked >> i) & 1u;
+            f[i] = F[bounce ? (size_t)i * g.n + node
+                            : (size_t)L::opp[i] * g.n + neighbour<L, i, -1>(c.x, c.y, c.z, g)];
+        });
+        add_wall_terms<L, -1>(f, blocked, wall_u, c, g);
+    }
     T Fx = T(0), Fy = T(0), Fz = T(0);
     if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
     collide<L, FM>(f, r, Fx, Fy, Fz);
+    // x + c_i is solid exactly when x - c_opp(i) is: then f*_i bounces back into slot
+    // opp(i) of this node, plus the wall term read at x + c_i.
+    if (MASK && blocked) {
+        T w[L::Q];
+        static_for<0, L::Q>([&](auto I) {
+            constexpr int i = I;
+            w[i] = wall_term<L, L::opp[i]>(wall_u, neighbour<L, i, +1>(c.x, c.y, c.z, g), g.n);
+        });
+        static_for<0, L::Q>([&](auto I) {
+            constexpr int i = I;
+            if ((blocked >> L::opp[i]) & 1u) f[i] = f[i] + w[i];
+        });
+    }
     static_for<0, L::Q>([&](auto I) {
         constexpr int i = I;
         constexpr int o = L::opp[i];
         const uint32_t down = neighbour<L, i, +1>(c.x, c.y, c.z, g);
-        // x + c_i is solid exactly when x - c_opp(i) is.
-        if (MASK && ((blocked >> o) & 1u)) {
-            F[(size_t)o * g.n + node] = f[i] + wall_term<L, o>(wall_u, down, g.n);
-        } else {
-            F[(size_t)i * g.n + down] = f[i];
-        }
+        const bool bounce = MASK && ((blocked >> o) & 1u);
+        F[bounce ? (size_t)o * g.n + node : (size_t)i * g.n + down] = f[i];
     });
 }
 
@@ -159,17 +176,25 @@ __global__ __launch_bounds__(1024) void k_lds(const T* __restrict__ src, T* __re
     if (x >= g.nx || y >= g.ny || (MASK && (k & KIND_SOLID))) return;
     const uint32_t node = (uint32_t)y * (uint32_t)g.nx + (uint32_t)x;
     T f[L::Q];
+    uint32_t blocked = 0;
     static_for<0, L::Q>([&](auto I) {
         constexpr int i = I;
         using Di = Dir<L, i>;
         const int e = (ty + 1 - Di::y) * hx + (tx + 1 - Di::x);
-        if (MASK && (kind_tile[e] & KIND_SOLID)) {
-            const uint32_t up = (uint32_t)wrap(y - Di::y, g.ny) * (uint32_t)g.nx + (uint32_t)wrap(x - Di::x, g.nx);
-            f[i] = tile[L::opp[i] * cells + centre] + wall_term<L, i>(wall_u, up, g.n);
-        } else {
-            f[i] = tile[i * cells + e];
-        }
+        const bool bounce = MASK && (kind_tile[e] & KIND_SOLID);
+        if (bounce) blocked |= 1u << i;
+        f[i] = tile[bounce ? L::opp[i] * cells + centre : i * cells + e];
     });
+    if (MASK && blocked) {
+        static_for<0, L::Q>([&](auto I) {
+            constexpr int i = I;
+            using Di = Dir<L, i>;
+            if ((blocked >> i) & 1u) {
+                const uint32_t up = (uint32_t)wrap(y - Di::y, g.ny) * (uint32_t)g.nx + (uint32_t)wrap(x - Di::x, g.nx);
+                f[i] = f[i] + wall_term<L, i>(wall_u, up, g.n);
+            }
+        });
+    }
     T Fx = T(0), Fy = T(0), Fz = T(0);
     if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
     collide<L, FM>(f, r, Fx, Fy, Fz);
@@ -318,7 +343,7 @@ int run_step(const char* name, Variant v, int lattice, int dtype, const void* a,
 {
     return guarded(name, [&] {
         const Grid g = make_grid(lattice, nx, ny, nz);
-        const dim3 block = block_of(bx, by);
+        const dim3 block = block_of(bx, by, step_max_threads_of(lattice, dtype));
         require(groups_per_cu >= 0, "groups_per_cu must be >= 0 (0: no occupancy cap)");
         require(b != nullptr && (v == Variant::EVEN || v == Variant::ODD || a != nullptr), "null population buffer");
         require(mask_mode == 0 || kind != nullptr, "mask_mode 1 needs the kind array");

@@ -87,7 +87,10 @@ def tune_batched(batched, reps: int = 4, caps=(0, 1, 2, 4)):
             batched.groups = per_cu * 256
             for cap in caps:
                 batched.groups_per_cu = cap
-                ms = _time(torch, batched._pair, reps)
+                try:
+                    ms = _time(torch, batched._pair, reps)
+                except (ValueError, RuntimeError):  # a shape the kernels were not compiled for
+                    continue
                 if ms < best_ms:
                     best, best_ms = (block, batched.groups, cap), ms
     batched.graphs = graphs

@@ -433,11 +433,18 @@ class HipSolver(Solver):
         pack = self._affine_pack(compiled)
         weight = getattr(pack, "weight", None)
         if weight is None:
+            import ctypes
+
             weight = self._torch.as_tensor(np.ascontiguousarray(probe.weight, dtype=np.float64), device=self.device)
             pack.weight = weight
+            size = ctypes.c_int64()
+            self._call("lbm_affine_force_scratch_size", pack.n, ctypes.byref(size))
+            pack.force_scratch = (self._torch.empty(size.value // 8, dtype=self._torch.float64, device=self.device)
+                                  if size.value else None)
         self._call(
             "lbm_affine_force", self._dtype_code, pointer(self._f), pointer(pack.slot), pointer(pack.coeff),
-            pointer(pack.constant), pointer(weight), pack.n, pack.m, self.params.d, pointer(out), self._stream,
+            pointer(pack.constant), pointer(weight), pack.n, pack.m, self.params.d, pointer(out),
+            pointer(pack.force_scratch), self._stream,
         )
 
     @property

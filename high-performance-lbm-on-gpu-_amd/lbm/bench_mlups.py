@@ -146,12 +146,11 @@ def main(argv=None) -> None:
 
     print(f"{torch.cuda.get_device_name(0)}: {'D2Q9' if args.dim == 2 else 'D3Q19'} {args.dtype}, "
           f"Infinity Cache {INFINITY_CACHE_BYTES // 2**20} MiB")
-    for backend in backends:
-        tiny = make_solver(grid_params(8, args.dim, args.dtype), backend=backend)
-        for name, res in tiny.kernel_resources().items():
-            print(f"  {name:>22}: {res['registers']:3d} VGPRs/lane, {res['local_bytes']} B scratch (spill), "
-                  f"{res['shared_bytes']} B LDS/group")
-        del tiny
+    tiny = make_solver(grid_params(8, args.dim, args.dtype), backend=backends[0])
+    for name, res in tiny.kernel_resources().items():
+        print(f"  {name:>22}: {res['registers']:3d} VGPRs/lane, {res['local_bytes']} B scratch (spill), "
+              f"{res['shared_bytes']} B LDS/group")
+    del tiny
     results = []
     print(f"{'backend':>9} {'N':>6} {'MLUPS':>9} {'GB/s':>8} {'peak':>7} {'% peak':>7} {'block':>9} cap  note")
     for backend in backends:

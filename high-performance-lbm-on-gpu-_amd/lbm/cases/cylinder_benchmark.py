@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import math
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -145,6 +146,40 @@ def force_history(solver, probe: ForceProbe, steps: int) -> np.ndarray:
     return out.cpu().numpy()
 
 
+@dataclass(frozen=True)
+class ChannelRun:
+    """A drag-channel run with the force recorded after every step (pair of AA steps)."""
+
+    reynolds: float
+    steps: int
+    seconds: float
+    mlups: float
+    cd: float
+    """Mean C_D = 2 F_x / (U_mean^2 2R) over the second half of the history."""
+    cl: float
+    history: np.ndarray
+
+
+def run_drag_channel(reynolds: float = 20.0, steps: int = 50_000, backend: str = "hip-pull",
+                     dtype: str = "float32", **geometry) -> ChannelRun:
+    """The 880 x 164 Bouzidi drag channel for ``steps`` steps from the inflow profile, with
+    drag and lift reduced on the device after every step; the throughput counts every node
+    and includes the rule pre-pass and the per-step force reduction."""
+    params, links, profile = drag_channel(reynolds=reynolds, dtype=dtype, **geometry)
+    probe = ForceProbe.build(params, links)
+    solver = make_solver(params, backend=backend)
+    solver.initialize(1.0, np.where(params.solid, 0.0, profile[:, None]), 0.0)
+    solver.synchronize()
+    start = time.perf_counter()
+    history = force_history(solver, probe, steps)
+    seconds = time.perf_counter() - start
+    radius = geometry.get("radius", 20.0)
+    u_mean = 2.0 * geometry.get("u_max", 0.075) / 3.0
+    coeff = history[len(history) // 2 :] / (0.5 * u_mean**2 * 2.0 * radius)
+    return ChannelRun(reynolds, steps, seconds, params.n_nodes * steps / seconds / 1e6, float(coeff[:, 0].mean()),
+                      float(coeff[:, 1].mean()), history)
+
+
 @dataclass
 class Observer:
     """Records C_D, C_L and dp of a running benchmark (``cylinder_benchmark.py:150-196``)."""
@@ -261,8 +296,19 @@ def main(argv=None) -> None:
     parser.add_argument("--sizes", type=int, nargs="+", default=[20, 40])
     parser.add_argument("--method", default="bouzidi", choices=METHODS)
     parser.add_argument("--u-max", type=float, default=None, help="peak inflow, lattice units")
+    parser.add_argument("--channel", type=float, nargs="*", default=None, metavar="RE",
+                        help="run the 880 x 164 drag channel at these Reynolds numbers instead (default 20 100)")
+    parser.add_argument("--steps", type=int, default=50_000, help="steps of a --channel run")
     add_backend_arguments(parser)
     args = parser.parse_args(argv)
+    if args.channel is not None:
+        print(f"880 x 164 Bouzidi drag channel, {args.collision}, backend = {args.backend}, {args.dtype}, "
+              f"drag and lift every step")
+        for re in args.channel or [20.0, 100.0]:
+            r = run_drag_channel(re, args.steps, args.backend, args.dtype)
+            print(f"Re = {re:5.0f}: {r.steps} steps in {r.seconds:.2f} s = {r.mlups:.0f} MLUPS; "
+                  f"C_D = {r.cd:.4f}  C_L = {r.cl:+.5f} (mean of the second half)", flush=True)
+        return
     print(f"Schaefer-Turek {args.test}, {args.method}, {args.collision}, backend = {args.backend}")
     print("accepted: " + ", ".join(f"{k} in [{lo}, {hi}]" for k, (lo, hi) in INTERVALS[args.test].items()))
     for n in args.sizes:

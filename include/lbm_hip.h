@@ -144,13 +144,20 @@ LBM_API int lbm_affine_values(int dtype, const void* f, const int64_t* slot, con
                               const void* constant, int64_t n_rows, int m, void* out, void* stream);
 
 /* Momentum-exchange force: out[a] (double) = sum_r weight[r][a] * value[r] over the
- * affine rows (value rounded to dtype, then widened), reduced by one workgroup in a fixed
- * order (lane-strided partial sums, then a tree), so repeated calls give the same bits;
- * d components. Replaces ForceProbe.evaluate's host matmul
- * (src/lbm/forces.py:98-103) so a per-step drag history stays on the device. */
+ * affine rows (value rounded to dtype, then widened), d components, reduced in a fixed
+ * order (lane-strided partial sums, then a tree) that depends on n_rows only, so repeated
+ * calls give the same bits. Up to 2048 rows one workgroup writes out directly; above, one
+ * workgroup per 256 rows (at most 1024) writes partial sums into `scratch`
+ * (lbm_affine_force_scratch_size bytes; may be NULL up to 2048 rows) and a second launch
+ * adds them. Calls that share one scratch buffer must not overlap. Replaces
+ * ForceProbe.evaluate's host matmul (src/lbm/forces.py:98-103) so a per-step drag
+ * history stays on the device. */
 LBM_API int lbm_affine_force(int dtype, const void* f, const int64_t* slot, const void* coeff,
                              const void* constant, const double* weight, int64_t n_rows, int m,
-                             int d, double* out, void* stream);
+                             int d, double* out, void* scratch, void* stream);
+
+/* Bytes of device scratch lbm_affine_force needs for n_rows rows (0 up to 2048). Host only. */
+LBM_API int lbm_affine_force_scratch_size(int64_t n_rows, int64_t* bytes);
 
 /* Bytes of device scratch lbm_ibm_force needs for n_nodes unique stencil nodes
  * and n_markers markers. Host only. */

@@ -261,3 +261,15 @@ def test_duct_sphere_512x128x128_pull_and_aa_agree_with_per_step_drag():
     fluid = params.fluid
     assert np.array_equal(aa.populations()[:, fluid], pull.populations()[:, fluid])
     assert np.all(np.isfinite(history)) and history[-1, 0] > 0.0
+
+
+@pytest.mark.gpu
+def test_channel_and_duct_runs_report_throughput_and_drag():
+    from lbm.cases.cylinder_benchmark import run_drag_channel
+    from lbm.cases.sphere_drag import run_duct
+
+    r = run_drag_channel(20.0, steps=400)
+    assert r.history.shape == (400, 2) and r.mlups > 0
+    assert np.all(np.isfinite(r.history)) and r.cd > 0
+    d = run_duct(20.0, steps=20)
+    assert d.history.shape == (10, 3) and d.mlups > 0 and np.isfinite(d.cd)

@@ -6,7 +6,7 @@ backend reads (pull; AA odd and even)."""
 import numpy as np
 import pytest
 
-from lbm import make_solver
+from lbm import SimulationParams, make_solver
 from lbm.forces import ForceProbe
 from tests.helpers import fluid_diff, random_state
 from tests.test_golden import bouzidi_case, ibm_case, open_case
@@ -59,6 +59,35 @@ def test_force_on_device_matches_the_host_evaluation(backend, d):
     out = torch.zeros(d, dtype=torch.float64, device="cuda")
     gpu.force_on_device(probe, out)
     np.testing.assert_allclose(out.cpu().numpy(), probe.evaluate(gpu), rtol=1e-12, atol=1e-18)
+
+
+@pytest.mark.parametrize("backend", ["hip-pull", "hip-aa"])
+def test_force_of_a_large_probe_is_reduced_by_many_groups_deterministically(backend):
+    """A sphere probe with more rows than one workgroup reduces (partial sums per group,
+    then a fixed-order total): equal to the host evaluation, and bit for bit the same on
+    every call."""
+    import torch
+
+    from lbm.geometry import sphere, voxelize, wall_links
+    from lbm.lattice import D3Q19
+    from lbm.link_rules import bouzidi_rules
+
+    shape = (28, 28, 36)
+    body = sphere(17.3, 13.6, 14.2, 7.4)
+    solid = voxelize(body, shape)
+    links = wall_links(D3Q19, body, solid)
+    params = SimulationParams(nx=36, ny=28, nz=28, tau=0.7, collision="trt", solid=solid, dtype="float64",
+                              link_rules=bouzidi_rules(D3Q19, links, solid), force=(1e-5, 0.0, 0.0))
+    probe = ForceProbe.build(params, links)
+    assert len(links) > 2048  # 16 groups of 256 rows
+    _, gpu = both(params, backend)
+    outs = [torch.zeros(3, dtype=torch.float64, device="cuda") for _ in range(3)]
+    for out in outs:
+        gpu.force_on_device(probe, out)
+    host = probe.evaluate(gpu)
+    np.testing.assert_allclose(outs[0].cpu().numpy(), host, rtol=1e-11, atol=1e-15 * np.abs(host).max())
+    for out in outs[1:]:
+        assert np.array_equal(out.cpu().numpy(), outs[0].cpu().numpy())
 
 
 @pytest.mark.parametrize(("backend", "d"), CASES)

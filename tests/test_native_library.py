@@ -76,6 +76,13 @@ def test_invalid_block_and_grid_are_rejected_by_the_library_before_launch():
         lib.call("lbm_lds_step", 1, 0, 1, 2, None, None, None, None, 8, 8, 8, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 32, 8, 0, None)
     with pytest.raises(ValueError, match="terms"):
         lib.call("lbm_rules_apply", 0, 1, 1, 1, 1, 1, 5, 200, None, None)
+    # float64 D3Q19 kernels are compiled for at most 512 lanes per workgroup
+    d3 = (1, 1, 1, 2, None, None, None, None, 8, 8, 8, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0)
+    with pytest.raises(ValueError, match="512 threads for float64 D3Q19"):
+        lib.call("lbm_pull_step", *d3, 256, 4, 0, None)
+    with pytest.raises(ValueError, match="512 threads for float64 D3Q19"):
+        lib.call("lbm_aa_odd", 1, 1, 1, None, None, None, None, 8, 8, 8, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 1024, 1, 0,
+                 None)
 
 
 def test_unknown_variant_is_rejected_before_touching_the_gpu():
@@ -120,3 +127,20 @@ def test_hip_backend_without_gpu_raises_clear_error():
 def test_cpu_backends_are_not_part_of_the_package():
     with pytest.raises(BackendUnavailable, match="HIP only"):
         make_solver(SimulationParams(nx=8, ny=8, tau=0.8), backend="numpy")
+
+
+def test_force_scratch_size_covers_the_partial_sums():
+    import ctypes
+
+    from lbm.backends._native import library
+
+    lib = library()
+    size = ctypes.c_int64()
+    lib.call("lbm_affine_force_scratch_size", 2048, ctypes.byref(size))
+    assert size.value == 0  # one workgroup writes the result directly
+    lib.call("lbm_affine_force_scratch_size", 2049, ctypes.byref(size))
+    assert size.value == 3 * 8 * 9
+    lib.call("lbm_affine_force_scratch_size", 10**7, ctypes.byref(size))
+    assert size.value == 3 * 8 * 1024
+    with pytest.raises(ValueError, match="bad arguments"):
+        lib.call("lbm_affine_force_scratch_size", -1, ctypes.byref(size))

@@ -62,28 +62,36 @@ _SHAPE_3D = {("aa", "float32"): (512, 2), ("pull", "float32"): (256, 4), ("aa", 
 
 
 def default_block(params, variant: str) -> tuple[int, int]:
-    """Whole rows per workgroup where possible: bx covers nx (a multiple of the 64-lane
-    wavefront, at most 256) and by stacks rows up to 1024 lanes in 2D, or the lanes of the
-    measured D3Q19 shape. At 256^3 (D3Q19 fp32 AA) 256 x 2 with two resident groups per CU
-    (see ``default_groups_per_cu``) moves 5.61 TB/s per AA pair against 5.36 TB/s for 256 x 4
-    uncapped; 256 x 4 was 4 % above 128 x 2 and 27 % above 64 x 4. The LDS-tiled kernel
-    takes the 2D shapes: its halo ring costs 2 (bx + 2) + 2 by cells per bx * by, so
-    1024-lane tiles win (D2Q9 fp32 4096^2: 256 x 4 and 128 x 8 at 76 K MLUPS against 62 K
-    for 64 x 4 and 55 K for 32 x 8)."""
-    bx = min(256, 64 * math.ceil(params.nx / 64))
+    """Whole rows per workgroup where possible. D2Q9: bx covers nx up to 1024 lanes (a
+    multiple of the 64-lane wavefront) and by stacks rows up to 1024 lanes; 1024 x 1 was
+    1.5-4 % above 256 x 4 on periodic 2048^2 / 4096^2 and Bouzidi channels 2048 x 512 /
+    8192 x 2048. The LDS-tiled kernel keeps bx <= 256: its halo ring costs
+    2 (bx + 2) + 2 by cells per bx * by, so 256 x 4 and 128 x 8 win (D2Q9 fp32 4096^2:
+    76 K MLUPS against 62 K for 64 x 4 and 55 K for 32 x 8). D3Q19: bx up to 256 with the
+    measured lanes per workgroup of ``_SHAPE_3D``."""
     if params.d == 2:
-        lanes = 1024
-    else:
-        lanes = _SHAPE_3D[("aa" if variant == "aa" else "pull", params.dtype)][0]
+        bx = min(256 if variant == "lds" else 1024, 64 * math.ceil(params.nx / 64))
+        return bx, max(1, min(params.ny, 1024 // bx))
+    bx = min(256, 64 * math.ceil(params.nx / 64))
+    lanes = _SHAPE_3D[("aa" if variant == "aa" else "pull", params.dtype)][0]
     return bx, max(1, min(params.ny, lanes // bx))
+
+
+INFINITY_CACHE_BYTES = 256 * 2**20
 
 
 def default_groups_per_cu(params, variant: str) -> int:
     """Resident workgroups per compute unit for the stream-collide kernels (0: no cap):
-    the measured D3Q19 caps above; 2D lattices run uncapped."""
-    if params.d == 2:
+    the measured D3Q19 caps of ``_SHAPE_3D``. D2Q9 runs uncapped except all-fluid lattices
+    whose state is more than twice the 256 MB Infinity Cache, where one 1024-lane group per
+    CU moved 4096^2 from 76.2 K to 81.4 K MLUPS (AA, 604 MB) and 81.5 K to 84.8 K (pull,
+    1.2 GB); the same cap cost 6-15 % on Bouzidi channels and at 2048^2 (151 / 302 MB)."""
+    if params.d == 3:
+        return _SHAPE_3D[("aa" if variant == "aa" else "pull", params.dtype)][1]
+    if variant == "lds" or params.solid.any() or params.immersed is not None:
         return 0
-    return _SHAPE_3D[("aa" if variant == "aa" else "pull", params.dtype)][1]
+    state = params.lattice.q * params.n_nodes * np.dtype(params.dtype).itemsize * (1 if variant == "aa" else 2)
+    return 1 if state > 2 * INFINITY_CACHE_BYTES else 0
 
 
 def node_kinds(lattice: VelocitySet, solid: np.ndarray, forced_nodes=None) -> np.ndarray:

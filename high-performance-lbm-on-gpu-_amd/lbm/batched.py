@@ -36,7 +36,7 @@ class BatchedSolver:
     Args:
         params_list: ``SimulationParams`` of every lattice (same lattice and dtype, no IBM).
         block: workgroup shape (bx, by); a work item is bx * by consecutive nodes of one
-            lattice; default (256, 4), (256, 2) for float64 D3Q19 (512-lane kernels).
+            lattice; default (1024, 1), (256, 2) for float64 D3Q19 (512-lane kernels).
         groups: persistent workgroups; default 8 per compute unit.
         graphs: capture the two-step cycle into a HIP graph.
         autotune: pick block and groups by timing (``lbm.backends.autotune``).
@@ -66,7 +66,7 @@ class BatchedSolver:
         self.nodes = [p.n_nodes for p in params_list]
         self.total_nodes = int(sum(self.nodes))
         if block is None:  # float64 D3Q19 kernels are compiled for at most 512 lanes
-            block = (256, 2) if (d == 3 and first.dtype == "float64") else (256, 4)
+            block = (256, 2) if (d == 3 and first.dtype == "float64") else (1024, 1)
         self.block = check_block(block)
         self.groups = int(groups) if groups else DEFAULT_GROUPS_PER_CU * COMPUTE_UNITS
         self.groups_per_cu = 0  # cap on resident workgroups per CU (0: none); set by the autotuner

@@ -125,7 +125,18 @@ __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T*
                                               int y, int z)
 {
     const uint32_t node = ((uint32_t)z * (uint32_t)g.ny + (uint32_t)y) * (uint32_t)g.nx + (uint32_t)x;
+    // The kind byte, the Q pulled populations and the Q adjoint populations are loaded
+    // together (valid addresses for any node; a solid node discards them). A node with a
+    // solid upstream neighbour then reloads each population from its selected slot
+    // (bounce-back or pulled) and adds the moving-wall terms; stores below also go to
+    // selected slots, so all loads and stores of a pass stay in flight together.
     const uint8_t k = MASK ? kind[node] : 0;
+    T fs[L::Q], lam[L::Q];
+    static_for<0, L::Q>([&](auto I) {
+        constexpr int i = I;
+        fs[i] = f[(size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)];
+        lam[i] = lam_next[(size_t)i * g.n + node];
+    });
     if (MASK && (k & KIND_SOLID)) return;
     uint32_t blocked = 0;
     if (MASK && (k & KIND_NEAR_SOLID)) {
@@ -133,17 +144,14 @@ __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T*
             if (kind[nbr<L, I, -1>(x, y, z, g)] & KIND_SOLID) blocked |= 1u << I;
         });
     }
-    // Selected slots instead of branches keep all loads (and below, all stores) of the
-    // node in flight together; moving-wall terms of blocked links are added afterwards.
-    T fs[L::Q], lam[L::Q];
-    static_for<0, L::Q>([&](auto I) {
-        constexpr int i = I;
-        const uint32_t up = nbr<L, i, -1>(x, y, z, g);
-        const bool bounce = MASK && ((blocked >> i) & 1u);
-        fs[i] = f[bounce ? (size_t)L::opp[i] * g.n + node : (size_t)i * g.n + up];
-        lam[i] = lam_next[(size_t)i * g.n + node];
-    });
-    if (MASK && blocked) add_wall_terms<L, -1>(fs, blocked, wall_u, Coord{x, y, z}, g);
+    if (MASK && blocked) {
+        static_for<0, L::Q>([&](auto I) {
+            constexpr int i = I;
+            const bool bounce = (blocked >> i) & 1u;
+            fs[i] = f[bounce ? (size_t)L::opp[i] * g.n + node : (size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)];
+        });
+        add_wall_terms<L, -1>(fs, blocked, wall_u, Coord{x, y, z}, g);
+    }
     T Fx = r.force[0], Fy = r.force[1], Fz = r.force[2];
     if constexpr (FM == ForceMode::DENSE) {
         Fx = Fx + body[node];

@@ -59,7 +59,9 @@ LBM_API int lbm_lattice_tables_device(int lattice, int dtype, int* c, int* opp, 
 /* Stream-collide entry points (lbm_pull_step, lbm_aa_even, lbm_aa_odd, lbm_lds_step):
  * groups_per_cu > 0 caps the resident workgroups per compute unit (requested as unused
  * LDS: the bandwidth of the AA pair at 256^3 peaks with 2 groups of 256 x 2 per CU);
- * 0 leaves occupancy to the hardware. */
+ * 0 leaves occupancy to the hardware. Blocks have at most 1024 threads, at most 512 for
+ * float64 D3Q19 (those kernels are compiled for 512 lanes to keep all populations in
+ * registers); larger blocks are rejected with LBM_ERR_INVALID before any launch. */
 
 /* Fused pull stream + halfway bounce-back (Ladd) + TRT/Guo collision, two lattices.
  * Replaces stream_collide_naive (src/lbm/backends/kernels/stream_collide_naive.cu:15-49),

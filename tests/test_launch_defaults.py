@@ -1,0 +1,52 @@
+"""Default launch shapes (host logic, no GPU): the measured D3Q19 shapes per variant and
+precision, D2Q9 row-wide workgroups, the occupancy-cap rule, and the lane limits the
+kernels are compiled for."""
+
+import numpy as np
+import pytest
+
+from lbm import SimulationParams
+from lbm.backends.hip_solver import default_block, default_groups_per_cu
+
+
+def box(n, d, dtype="float32", solid=False):
+    sizes = {"nx": n, "ny": n} | ({"nz": n} if d == 3 else {})
+    shape = (n,) * d
+    mask = np.zeros(shape, dtype=bool)
+    if solid:
+        mask[(0,) * d] = True
+    return SimulationParams(**sizes, tau=0.6, dtype=dtype, solid=mask)
+
+
+@pytest.mark.parametrize(("variant", "dtype", "block", "cap"), [
+    ("aa", "float32", (256, 2), 2), ("pull", "float32", (256, 1), 4),
+    ("aa", "float64", (256, 1), 2), ("pull", "float64", (256, 1), 2),
+])
+def test_d3q19_shapes_at_256(variant, dtype, block, cap):
+    p = box(256, 3, dtype)
+    assert default_block(p, variant) == block
+    assert default_groups_per_cu(p, variant) == cap
+
+
+def test_d2q9_rows_and_the_cap_rule():
+    big = box(4096, 2)  # 604 MB (AA) / 1.2 GB (pull): beyond twice the Infinity Cache
+    assert default_block(big, "pull") == (1024, 1) and default_groups_per_cu(big, "pull") == 1
+    assert default_groups_per_cu(big, "aa") == 1
+    mid = box(2048, 2)  # 151 MB / 302 MB: uncapped
+    assert default_groups_per_cu(mid, "aa") == 0 and default_groups_per_cu(mid, "pull") == 0
+    walls = box(4096, 2, solid=True)  # masked lattices run uncapped
+    assert default_groups_per_cu(walls, "pull") == 0
+    assert default_block(big, "lds") == (256, 4) and default_groups_per_cu(big, "lds") == 0
+    narrow = SimulationParams(nx=45, ny=29, tau=0.6)
+    assert default_block(narrow, "pull") == (64, 16)
+
+
+@pytest.mark.parametrize("n", [8, 20, 64, 100, 256, 300])
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_default_blocks_respect_the_kernel_lane_limits(n, dtype):
+    for d in (2, 3):
+        p = box(n, d, dtype)
+        for variant in ("pull", "aa") + (("lds",) if d == 2 else ()):
+            bx, by = default_block(p, variant)
+            limit = 512 if (d == 3 and dtype == "float64") else 1024
+            assert bx % 64 == 0 and 1 <= bx * by <= limit

@@ -106,8 +106,12 @@ class LinkRules:
         if not (len(self.node) == len(self.direction) == len(self.values)):
             raise ValueError("node, direction and values must have the same length")
         if len(self.node):
-            keys = np.stack([np.asarray(self.node), np.asarray(self.direction)])
-            if np.unique(keys, axis=1).shape[1] != len(self.node):
+            node = np.asarray(self.node, dtype=np.int64)
+            direction = np.asarray(self.direction, dtype=np.int64)
+            if direction.min() < 0 or node.min() < 0:
+                raise ValueError("link nodes and directions must be non-negative")
+            key = np.sort(node * (int(direction.max()) + 1) + direction)  # one int64 per link
+            if np.any(key[1:] == key[:-1]):
                 raise ValueError("a link has more than one rule")
 
     def __len__(self) -> int:

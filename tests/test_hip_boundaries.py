@@ -41,6 +41,21 @@ def test_bouzidi_matches_the_oracle(backend, d, dtype, tol):
     assert fluid_diff(params, ref, gpu) < tol
 
 
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("d", [2, 3])
+def test_pull_with_link_rules_is_bit_identical_to_the_oracle(d, dtype):
+    """Zou-He inlet/outlet and Bouzidi rows evaluated on the device in NumPy's summation
+    order: the two-lattice states equal the oracle's bit for bit after 40 steps."""
+    for params in (bouzidi_case(d, dtype)[0], open_case(d)[0]):
+        if params.dtype != dtype:  # the open-channel case is built in float64
+            from dataclasses import replace
+
+            params = replace(params, dtype=dtype)
+        ref, gpu = both(params, "hip-pull", seed=2)
+        fluid = params.fluid
+        np.testing.assert_array_equal(gpu.populations()[:, fluid], ref.populations()[:, fluid])
+
+
 @pytest.mark.parametrize(("backend", "d"), CASES)
 def test_force_probe_reads_every_layout_the_same(backend, d):
     params, links, _ = bouzidi_case(d, "float64")

@@ -158,11 +158,13 @@ LBM_API int lbm_affine_force(int dtype, const void* f, const int64_t* slot, cons
                              const void* constant, const double* weight, int64_t n_rows, int m,
                              int d, double* out, void* scratch, void* stream);
 
-/* Bytes of device scratch lbm_affine_force needs for n_rows rows (0 up to 2048). Host only. */
+/* Bytes of device scratch lbm_affine_force needs for n_rows rows (0 up to 2048). No
+ * counterpart in the reference (its force sum is a host matmul). Host only. */
 LBM_API int lbm_affine_force_scratch_size(int64_t n_rows, int64_t* bytes);
 
 /* Bytes of device scratch lbm_ibm_force needs for n_nodes unique stencil nodes
- * and n_markers markers. Host only. */
+ * and n_markers markers. No counterpart in the reference (its IBM runs as array code on
+ * CuPy temporaries, src/lbm/ibm.py:159-214). Host only. */
 LBM_API int lbm_ibm_scratch_size(int lattice, int dtype, int n_nodes, int n_markers, int64_t* bytes);
 
 /* Immersed-boundary direct forcing, all multi-direct iterations in one launch
@@ -229,7 +231,7 @@ LBM_API int lbm_adj_rules(int dtype, void* lam, const int64_t* target, int n_rul
                           const int* ustart, const int* urule, const void* ucoeff, int n_unique,
                           void* scratch, void* stream);
 
-/* Scratch bytes of lbm_ibm_adjoint. Host only. */
+/* Scratch bytes of lbm_ibm_adjoint. No counterpart in the reference. Host only. */
 LBM_API int lbm_ibm_adjoint_scratch_size(int lattice, int dtype, int n_nodes, int n_markers, int iterations,
                                          int64_t* bytes);
 

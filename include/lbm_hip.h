@@ -44,7 +44,7 @@ enum {
 
 /* Message of the last failed call on this thread ("" if none). */
 LBM_API const char* lbm_last_error(void);
-/* Library version string. */
+/* Library version string. No counterpart in the reference. */
 LBM_API const char* lbm_version(void);
 
 /* Host-side copy of the compiled lattice tables: c[3*Q] (x, y, z rows), opp[Q], w[Q].
@@ -279,7 +279,8 @@ LBM_API int lbm_batched_desc_size(int dtype, int64_t* bytes);
 
 /* Fill n_lattices descriptors (host memory, lbm_batched_desc_size bytes each) for lattices
  * of sizes nx/ny/nz[l], rates omega_plus/minus[l] and uniform force force[3 l .. 3 l + 2],
- * laid out back to back in the pools: populations q n_l, kind bytes n_l, wall d n_l. */
+ * laid out back to back in the pools: populations q n_l, kind bytes n_l, wall d n_l.
+ * No counterpart in the reference (batching is new). */
 LBM_API int lbm_batched_describe(int lattice, int dtype, int n_lattices, const int* nx, const int* ny,
                                  const int* nz, const double* omega_plus, const double* omega_minus,
                                  const double* force, void* host_desc);

@@ -75,6 +75,12 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_aa_odd(T*
     });
     if (MASK && (k & KIND_SOLID)) return;
     const uint32_t blocked = (MASK && (k & KIND_NEAR_SOLID)) ? blocked_upstream<L>(kind, c.x, c.y, c.z, g) : 0u;
+    // w[i]: moving-wall term of link i, u_w read at the solid node x - c_i (meaningful for
+    // the set bits of `blocked` only). It serves twice: the pulled f_i bounces back with
+    // +w[i], and after the collision f*_opp(i), which would stream into that same solid node
+    // x + c_opp(i) = x - c_i, bounces back into slot i of this node with +w[i] as well (the
+    // term of direction i at that node) — so the wall velocities are loaded once.
+    T w[L::Q];
     if (MASK && blocked) {
         static_for<0, L::Q>([&](auto I) {
             constexpr int i = I;
@@ -82,22 +88,18 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_aa_odd(T*
             f[i] = F[bounce ? (size_t)i * g.n + node
                             : (size_t)L::opp[i] * g.n + neighbour<L, i, -1>(c.x, c.y, c.z, g)];
         });
-        add_wall_terms<L, -1>(f, blocked, wall_u, c, g);
+        static_for<0, L::Q>([&](auto I) { w[I] = wall_term<L, I>(wall_u, neighbour<L, I, -1>(c.x, c.y, c.z, g), g.n); });
+        static_for<0, L::Q>([&](auto I) {
+            if ((blocked >> I) & 1u) f[I] = f[I] + w[I];
+        });
     }
     T Fx = T(0), Fy = T(0), Fz = T(0);
     if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
     collide<L, FM>(f, r, Fx, Fy, Fz);
-    // x + c_i is solid exactly when x - c_opp(i) is: then f*_i bounces back into slot
-    // opp(i) of this node, plus the wall term read at x + c_i.
     if (MASK && blocked) {
-        T w[L::Q];
         static_for<0, L::Q>([&](auto I) {
             constexpr int i = I;
-            w[i] = wall_term<L, L::opp[i]>(wall_u, neighbour<L, i, +1>(c.x, c.y, c.z, g), g.n);
-        });
-        static_for<0, L::Q>([&](auto I) {
-            constexpr int i = I;
-            if ((blocked >> L::opp[i]) & 1u) f[i] = f[i] + w[i];
+            if ((blocked >> i) & 1u) f[L::opp[i]] = f[L::opp[i]] + w[i];
         });
     }
     static_for<0, L::Q>([&](auto I) {

@@ -251,7 +251,7 @@ def main(argv=None) -> int:
     parser.add_argument("--no-two-lattice", action="store_true", help="skip the two-lattice pull timing")
     args = parser.parse_args(argv)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (got --gpus {args.gpus})", file=sys.stderr)
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (got --gpus {args.gpus})", file=sys.stderr)
         return 2
     if args.steps % 2 or args.warmup % 2:
         print("steps and warmup must be even (the AA pattern advances in pairs)", file=sys.stderr)

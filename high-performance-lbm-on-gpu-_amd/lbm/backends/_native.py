@@ -92,10 +92,21 @@ class Library:
     def call(self, name: str, *args) -> None:
         code = self._functions[name](*args)
         if code != 0:
-            message = self._lib.lbm_last_error().decode()
-            if code == -1:
-                raise ValueError(message)
-            raise NativeError(f"{message} (code {code})")
+            self._raise(code)
+
+    def call_all(self, calls) -> None:
+        """Run a prepared launch list [(entry point name, argument tuple), ...] in order."""
+        functions = self._functions
+        for name, args in calls:
+            code = functions[name](*args)
+            if code != 0:
+                self._raise(code)
+
+    def _raise(self, code: int) -> None:
+        message = self._lib.lbm_last_error().decode()
+        if code == -1:
+            raise ValueError(message)
+        raise NativeError(f"{message} (code {code})")
 
 
 @cache

@@ -306,13 +306,13 @@ class HipSolver(Solver):
             self._body_force.zero_()
         self.time_step = 0
 
-    def _step_args(self):
+    def _step_args(self, stream=None):
         p = self.params
         wp, wm = self._rates
         fx, fy, fz = self._force3
         bx, by = self.block
         return (p.nx, p.ny, self._nz, wp, wm, fx, fy, fz, self._force_mode, self._mask_mode, bx, by,
-                self._groups_per_cu, self._stream)
+                self._groups_per_cu, self._stream if stream is None else stream)
 
     def _advance(self, steps: int) -> None:
         if self.graphs and steps >= 2:
@@ -342,47 +342,65 @@ class HipSolver(Solver):
             graph.replay()
 
     def _advance_eager(self, steps: int) -> None:
-        L, D = self._lattice_code, self._dtype_code
-        kind = pointer(self._kind)
-        field = pointer(self._field)
-        body = pointer(self._body)
-        args = self._step_args()
+        """Launch ``steps`` steps from a launch list prepared once per call (one per buffer
+        parity of the two-lattice variants), so the per-step host work is the launches."""
+        stream = self._stream
+        sequences = self._step_calls(stream)
+        lib = self._lib
         if self.variant == "aa":
             for _ in range(steps // 2):
-                self._pre_pass("aa_odd")
-                self._call("lbm_aa_odd", L, D, pointer(self._f), kind, pointer(self._wall), field, body, *args)
-                self._pre_pass("aa_even")
-                self._call("lbm_aa_even", L, D, pointer(self._f), kind, field, body, *args)
+                lib.call_all(sequences[0])
             return
-        name = "lbm_lds_step" if self.variant == "lds" else "lbm_pull_step"
-        for _ in range(steps):
-            self._pre_pass("pull")
-            self._call(name, L, D, pointer(self._f), pointer(self._f_next), kind, pointer(self._wall), field, body, *args)
+        for s in range(steps):
+            lib.call_all(sequences[s % 2])
+        if steps % 2:
             self._f, self._f_next = self._f_next, self._f
 
-    def _pre_pass(self, layout: str) -> None:
-        """The IBM force field, then the link rules, for the layout the next kernel reads.
+    def _step_calls(self, stream) -> list[list[tuple]]:
+        """Launch lists of one AA pair, or of one two-lattice step per buffer parity (the
+        state in ``_f``, then in ``_f_next``): pre-pass and stream-collide, in order."""
+        L, D = self._lattice_code, self._dtype_code
+        kind, field, body, wall = pointer(self._kind), pointer(self._field), pointer(self._body), pointer(self._wall)
+        args = self._step_args(stream)
+        if self.variant == "aa":
+            f = pointer(self._f)
+            return [self._pre_pass_calls("aa_odd", f, stream)
+                    + [("lbm_aa_odd", (L, D, f, kind, wall, field, body, *args))]
+                    + self._pre_pass_calls("aa_even", f, stream)
+                    + [("lbm_aa_even", (L, D, f, kind, field, body, *args))]]
+        name = "lbm_lds_step" if self.variant == "lds" else "lbm_pull_step"
+        a, b = pointer(self._f), pointer(self._f_next)
+        return [self._pre_pass_calls("pull", src, stream) + [(name, (L, D, src, dst, kind, wall, field, body, *args))]
+                for src, dst in ((a, b), (b, a))]
 
-        The IBM moments read the state before any rule overwrote a target slot, as the
-        NumPy step does (it evaluates the IBM on the stored state, src/lbm/solver.py:263-271).
-        """
+    def _pre_pass(self, layout: str) -> None:
+        """The IBM force field, then the link rules, for the layout the next kernel reads."""
+        self._lib.call_all(self._pre_pass_calls(layout, pointer(self._f), self._stream))
+
+    def _pre_pass_calls(self, layout: str, f: int, stream) -> list[tuple]:
+        """Launches of the pre-pass on the state at address ``f``: the IBM force field, then
+        the link rules. The IBM moments read the state before any rule overwrote a target
+        slot, as the NumPy step does (it evaluates the IBM on the stored state,
+        src/lbm/solver.py:263-271)."""
+        calls = []
         if self._ibm is not None:
             ibm = self._ibm
             mp = self._ibm_pack(layout)
             fx, fy, fz = self._force3
-            self._call(
-                "lbm_ibm_force", self._lattice_code, self._dtype_code, pointer(self._f), pointer(mp.slot),
-                pointer(mp.coeff), pointer(mp.constant), mp.m, pointer(self._ibm_nodes), ibm.n_nodes,
-                pointer(self._ibm_inverse), pointer(self._ibm_weights), pointer(self._ibm_areas), ibm.n_markers,
-                pointer(self._ibm_start), pointer(self._ibm_entry), ibm.iterations, fx, fy, fz, self._n,
-                pointer(self._field), pointer(self._body_force), pointer(self._ibm_scratch), None, self._stream,
-            )
+            calls.append(("lbm_ibm_force", (
+                self._lattice_code, self._dtype_code, f, pointer(mp.slot), pointer(mp.coeff), pointer(mp.constant),
+                mp.m, pointer(self._ibm_nodes), ibm.n_nodes, pointer(self._ibm_inverse), pointer(self._ibm_weights),
+                pointer(self._ibm_areas), ibm.n_markers, pointer(self._ibm_start), pointer(self._ibm_entry),
+                ibm.iterations, fx, fy, fz, self._n, pointer(self._field), pointer(self._body_force),
+                pointer(self._ibm_scratch), None, stream,
+            )))
         pack = self._rule_passes.get(layout)
         if pack is not None:
-            self._call(
-                "lbm_rules_apply", self._dtype_code, pointer(self._f), pointer(pack.slot), pointer(pack.coeff),
-                pointer(pack.constant), pointer(pack.target), pack.n, pack.m, pointer(pack.scratch), self._stream,
-            )
+            calls.append(("lbm_rules_apply", (
+                self._dtype_code, f, pointer(pack.slot), pointer(pack.coeff), pointer(pack.constant),
+                pointer(pack.target), pack.n, pack.m, pointer(pack.scratch), stream,
+            )))
+        return calls
 
     def synchronize(self) -> None:
         self._torch.cuda.synchronize()
@@ -435,25 +453,58 @@ class HipSolver(Solver):
             self._affine_cache[key] = pack
         return pack
 
-    def force_on_device(self, probe, out) -> None:
-        """Momentum-exchange force of ``probe`` into the float64 device tensor ``out`` (d,), no host sync."""
-        compiled = probe.compiled(self.state_layout)
-        pack = self._affine_pack(compiled)
-        weight = getattr(pack, "weight", None)
-        if weight is None:
+    def _force_pack(self, probe) -> _AffinePack:
+        """The probe's rows for the current layout on the device, with their weights and the
+        reduction's scratch (uploaded on first use)."""
+        pack = self._affine_pack(probe.compiled(self.state_layout))
+        if getattr(pack, "weight", None) is None:
             import ctypes
 
-            weight = self._torch.as_tensor(np.ascontiguousarray(probe.weight, dtype=np.float64), device=self.device)
-            pack.weight = weight
+            pack.weight = self._torch.as_tensor(np.ascontiguousarray(probe.weight, dtype=np.float64),
+                                                device=self.device)
             size = ctypes.c_int64()
             self._call("lbm_affine_force_scratch_size", pack.n, ctypes.byref(size))
             pack.force_scratch = (self._torch.empty(size.value // 8, dtype=self._torch.float64, device=self.device)
                                   if size.value else None)
-        self._call(
-            "lbm_affine_force", self._dtype_code, pointer(self._f), pointer(pack.slot), pointer(pack.coeff),
-            pointer(pack.constant), pointer(weight), pack.n, pack.m, self.params.d, pointer(out),
-            pointer(pack.force_scratch), self._stream,
-        )
+        return pack
+
+    def _force_call(self, pack: _AffinePack, f: int, out: int, stream) -> tuple:
+        return ("lbm_affine_force", (
+            self._dtype_code, f, pointer(pack.slot), pointer(pack.coeff), pointer(pack.constant),
+            pointer(pack.weight), pack.n, pack.m, self.params.d, out, pointer(pack.force_scratch), stream,
+        ))
+
+    def force_on_device(self, probe, out) -> None:
+        """Momentum-exchange force of ``probe`` into the float64 device tensor ``out`` (d,), no host sync."""
+        pack = self._force_pack(probe)
+        self._lib.call_all([self._force_call(pack, pointer(self._f), pointer(out), self._stream)])
+
+    def record_forces(self, probe, steps: int) -> np.ndarray:
+        """Advance ``steps`` steps and return the force after each ``step_multiple`` steps,
+        shape (steps // step_multiple, d), float64: the launches of ``_advance`` followed by
+        ``force_on_device`` into row k of one device tensor, prepared once; one host copy and
+        one health check at the end."""
+        m = self.step_multiple
+        if steps <= 0 or steps % m:
+            raise ValueError(f"steps must be a positive multiple of {m}")
+        torch = self._torch
+        d = self.params.d
+        out = torch.zeros((steps // m, d), dtype=torch.float64, device=self.device)
+        pack = self._force_pack(probe)
+        stream = self._stream
+        sequences = self._step_calls(stream)
+        # the state after a step: the AA buffer, or the destination of that parity's step
+        states = [pointer(self._f)] if self.variant == "aa" else [pointer(self._f_next), pointer(self._f)]
+        base, row = out.data_ptr(), 8 * d
+        lib = self._lib
+        for k in range(steps // m):
+            parity = k % len(sequences)
+            lib.call_all(sequences[parity] + [self._force_call(pack, states[parity], base + k * row, stream)])
+        if self.variant != "aa" and steps % 2:
+            self._f, self._f_next = self._f_next, self._f
+        self.time_step += steps
+        self._check_health()
+        return out.cpu().numpy()
 
     @property
     def immersed_force(self) -> np.ndarray:

@@ -273,3 +273,33 @@ def test_channel_and_duct_runs_report_throughput_and_drag():
     assert np.all(np.isfinite(r.history)) and r.cd > 0
     d = run_duct(20.0, steps=20)
     assert d.history.shape == (10, 3) and d.mlups > 0 and np.isfinite(d.cd)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize(("backend", "steps"), [("hip-pull", 7), ("hip-pull", 8), ("hip-aa", 8)])
+def test_prepared_force_recording_equals_the_step_by_step_loop(backend, steps):
+    """``record_forces`` (launch lists prepared once) against ``_advance`` + ``force_on_device``
+    per step: the same forces and the same final state, bit for bit, odd step counts too."""
+    import torch
+
+    from lbm import make_solver
+    from lbm.forces import ForceProbe
+
+    params, links, profile = drag_channel(reynolds=20.0, nx=220, ny=42, radius=5.0, centre=(50.0, 20.0))
+    probe = ForceProbe.build(params, links)
+    ux = np.where(params.solid, 0.0, profile[:, None])
+    fast, slow = make_solver(params, backend=backend), make_solver(params, backend=backend)
+    for s in (fast, slow):
+        s.initialize(1.0, ux, 0.0)
+    recorded = fast.record_forces(probe, steps)
+    m = slow.step_multiple
+    expected = torch.zeros((steps // m, 2), dtype=torch.float64, device="cuda")
+    for k in range(steps // m):
+        slow._advance(m)
+        slow.time_step += m
+        slow.force_on_device(probe, expected[k])
+    assert np.array_equal(recorded, expected.cpu().numpy())
+    assert np.array_equal(fast.populations(), slow.populations()) and fast.time_step == slow.time_step
+    fast.run(2 * m)  # the buffers are where the solver expects them afterwards
+    slow.run(2 * m)
+    assert np.array_equal(fast.populations(), slow.populations())

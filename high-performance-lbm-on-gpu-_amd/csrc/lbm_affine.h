@@ -87,17 +87,16 @@ __device__ __forceinline__ T affine_row(const T* __restrict__ F, const int64_t* 
     return c + affine_sum(F, slot, coeff, stride, r, m);
 }
 
-// The link-rule pre-pass by one workgroup: rows r = tid + k * nthreads (k < PER, so at most
-// PER * nthreads rules) are evaluated into registers, then, after a barrier, written to
-// F[target[r]], so every rule reads pre-rule populations. Call from all lanes of the group.
+// The link-rule pre-pass by one workgroup, in two halves so a kernel can do other reads of
+// the state between them: rules_values evaluates rows r = tid + k * nthreads (k < PER, so
+// at most PER * nthreads rules) into registers and fetches their targets; after a barrier,
+// rules_store writes F[target[r]], so every rule reads pre-rule populations.
 template <class T, int PER>
-__device__ __forceinline__ void rules_in_one_group(T* __restrict__ F, const int64_t* __restrict__ slot,
-                                                   const T* __restrict__ coeff, const T* __restrict__ constant,
-                                                   const int64_t* __restrict__ target, int64_t n, int m, int tid,
-                                                   int nthreads)
+__device__ __forceinline__ void rules_values(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                             const T* __restrict__ coeff, const T* __restrict__ constant,
+                                             const int64_t* __restrict__ target, int64_t n, int m, int tid,
+                                             int nthreads, T (&v)[PER], int64_t (&t)[PER])
 {
-    T v[PER];
-    int64_t t[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int64_t r = (int64_t)k * nthreads + tid;
@@ -108,12 +107,31 @@ __device__ __forceinline__ void rules_in_one_group(T* __restrict__ F, const int6
         const int64_t r = (int64_t)k * nthreads + tid;
         v[k] = r < n ? affine_row(F, slot, coeff, constant, n, m, r) : T(0);
     }
-    __syncthreads();
+}
+
+template <class T, int PER>
+__device__ __forceinline__ void rules_store(T* __restrict__ F, int64_t n, int tid, int nthreads, const T (&v)[PER],
+                                            const int64_t (&t)[PER])
+{
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int64_t r = (int64_t)k * nthreads + tid;
         if (r < n) F[t[k]] = v[k];
     }
+}
+
+// The whole pre-pass by one workgroup. Call from all lanes of the group.
+template <class T, int PER>
+__device__ __forceinline__ void rules_in_one_group(T* __restrict__ F, const int64_t* __restrict__ slot,
+                                                   const T* __restrict__ coeff, const T* __restrict__ constant,
+                                                   const int64_t* __restrict__ target, int64_t n, int m, int tid,
+                                                   int nthreads)
+{
+    T v[PER];
+    int64_t t[PER];
+    rules_values<T, PER>(F, slot, coeff, constant, target, n, m, tid, nthreads, v, t);
+    __syncthreads();
+    rules_store<T, PER>(F, n, tid, nthreads, v, t);
 }
 
 }  // namespace lbm

@@ -64,11 +64,12 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_rules_small(T* __restrict__ F
     rules_in_one_group<T, SMALL_PER_THREAD>(F, slot, coeff, constant, target, n, m, threadIdx.x, SMALL_THREADS);
 }
 
-constexpr int FORCE_THREADS = 256;
+constexpr int FORCE_THREADS = 1024;
 constexpr int64_t FORCE_ONE_GROUP_ROWS = 2048;
 constexpr int FORCE_MAX_GROUPS = 1024;
 
-// Workgroups of the force reduction: one up to FORCE_ONE_GROUP_ROWS rows (the result is
+// Workgroups of the force reduction (FORCE_THREADS lanes each, so a probe of up to 1024
+// rows costs one row per lane): one up to FORCE_ONE_GROUP_ROWS rows (the result is
 // written directly), else one per FORCE_THREADS rows up to FORCE_MAX_GROUPS, whose
 // partial sums a second one-workgroup pass adds up. Depends on n only, so the summation
 // order of a given row set never changes.
@@ -131,6 +132,39 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_force_total(const double* __r
     for (int a = 0; a < 3; ++a) part[a][threadIdx.x] = acc[a];
     force_tree(part);
     if (threadIdx.x == 0)
+        for (int a = 0; a < d; ++a) out[a] = part[a][0];
+}
+
+// The force of one step and the link-rule pre-pass of the next, in one launch of one
+// workgroup (they are consecutive and read the same state). Lanes below FORCE_THREADS take
+// the force rows exactly as k_affine_force does with one workgroup (rows tid + k *
+// FORCE_THREADS, the same accumulation and tree), all lanes take the rules as
+// k_rules_small does; every read of the state precedes the barrier, the rule writes follow
+// it, so both give the bits of the two separate launches.
+template <class T>
+__global__ __launch_bounds__(SMALL_THREADS) void k_force_rules(
+    T* __restrict__ F, const int64_t* __restrict__ fslot, const T* __restrict__ fcoeff,
+    const T* __restrict__ fconstant, const double* __restrict__ weight, int64_t nf, int fm, int d,
+    double* __restrict__ out, const int64_t* __restrict__ rslot, const T* __restrict__ rcoeff,
+    const T* __restrict__ rconstant, const int64_t* __restrict__ rtarget, int64_t nr, int rm)
+{
+    __shared__ double part[3][FORCE_THREADS];
+    const int tid = threadIdx.x;
+    if (tid < FORCE_THREADS) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int64_t r = tid; r < nf; r += FORCE_THREADS) {
+            const double v = (double)affine_row(F, fslot, fcoeff, fconstant, nf, fm, r);
+            for (int a = 0; a < d; ++a) acc[a] += weight[r * d + a] * v;
+        }
+        for (int a = 0; a < 3; ++a) part[a][tid] = acc[a];
+    }
+    T v[SMALL_PER_THREAD];
+    int64_t t[SMALL_PER_THREAD];
+    rules_values<T, SMALL_PER_THREAD>(F, rslot, rcoeff, rconstant, rtarget, nr, rm, tid, SMALL_THREADS, v, t);
+    __syncthreads();
+    rules_store<T, SMALL_PER_THREAD>(F, nr, tid, SMALL_THREADS, v, t);
+    force_tree(part);
+    if (tid == 0)
         for (int a = 0; a < d; ++a) out[a] = part[a][0];
 }
 
@@ -201,6 +235,32 @@ LBM_API int lbm_affine_values(int dtype, const void* f, const int64_t* slot, con
                 static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), n_rows,
                 m, static_cast<T*>(out));
             check_launch("affine values");
+        });
+    });
+}
+
+LBM_API int lbm_affine_force_rules(int dtype, void* f, const int64_t* force_slot, const void* force_coeff,
+                                   const void* force_constant, const double* weight, int64_t n_rows, int force_m,
+                                   int d, double* out, const int64_t* rule_slot, const void* rule_coeff,
+                                   const void* rule_constant, const int64_t* rule_target, int64_t n_rules,
+                                   int rule_m, void* stream)
+{
+    return guarded("lbm_affine_force_rules", [&] {
+        require(n_rows >= 0 && n_rows <= FORCE_ONE_GROUP_ROWS, "the fused force takes at most 2048 rows");
+        require(n_rules >= 0 && n_rules <= (int64_t)SMALL_THREADS * SMALL_PER_THREAD,
+                "the fused pre-pass takes at most 4096 rules");
+        require(force_m >= 1 && force_m <= 128 && rule_m >= 1 && rule_m <= 128, "affine rows need 1 <= m <= 128 terms");
+        require(d >= 1 && d <= 3, "d must be 1, 2 or 3");
+        require(f && out, "null pointer");
+        require(n_rows == 0 || (force_slot && force_coeff && force_constant && weight), "null force rows");
+        require(n_rules == 0 || (rule_slot && rule_coeff && rule_constant && rule_target), "null rules");
+        dispatch_real(dtype, [&](auto Tt) {
+            using T = decltype(Tt);
+            k_force_rules<T><<<1, SMALL_THREADS, 0, as_stream(stream)>>>(
+                static_cast<T*>(f), force_slot, static_cast<const T*>(force_coeff),
+                static_cast<const T*>(force_constant), weight, n_rows, force_m, d, out, rule_slot,
+                static_cast<const T*>(rule_coeff), static_cast<const T*>(rule_constant), rule_target, n_rules, rule_m);
+            check_launch("fused force and pre-pass");
         });
     });
 }

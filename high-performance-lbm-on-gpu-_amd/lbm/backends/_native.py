@@ -35,6 +35,7 @@ SIGNATURES: dict[str, list] = {
     "lbm_affine_values": [_i, _p, _p, _p, _p, _i64, _i, _p, _p],
     "lbm_affine_force": [_i, _p, _p, _p, _p, _p, _i64, _i, _i, _p, _p, _p],
     "lbm_affine_force_scratch_size": [_i64, POINTER(c_int64)],
+    "lbm_affine_force_rules": [_i, _p, _p, _p, _p, _p, _i64, _i, _i, _p, _p, _p, _p, _p, _i64, _i, _p],
     "lbm_kernel_resources": [_i, _i, POINTER(c_int), _i, c_char_p, _i, POINTER(c_int)],
     "lbm_device_copy": [_p, _p, _i64, _p],
     "lbm_rules_apply_widths": [_i, _p, _p, _p, _p, _p, _p, _i64, _i, _p, _p],

@@ -497,9 +497,28 @@ class HipSolver(Solver):
         states = [pointer(self._f)] if self.variant == "aa" else [pointer(self._f_next), pointer(self._f)]
         base, row = out.data_ptr(), 8 * d
         lib = self._lib
-        for k in range(steps // m):
+        # Without an immersed body, a step's pre-pass is its link rules alone; when they and the
+        # force rows fit one workgroup, the force of step k - 1 and the rules of step k (same
+        # state) share one launch (lbm_affine_force_rules), replacing the first call of step k.
+        rules = self._rule_passes.get("aa_odd" if self.variant == "aa" else "pull")
+        fuse = self._ibm is None and rules is not None and rules.n <= 4096 and pack.n <= 2048
+        count = steps // m
+        for k in range(count):
             parity = k % len(sequences)
-            lib.call_all(sequences[parity] + [self._force_call(pack, states[parity], base + k * row, stream)])
+            calls = sequences[parity]
+            if fuse and k > 0:
+                state = states[(k - 1) % len(sequences)]  # = the source of step k
+                calls = [("lbm_affine_force_rules", (
+                    self._dtype_code, state, pointer(pack.slot), pointer(pack.coeff), pointer(pack.constant),
+                    pointer(pack.weight), pack.n, pack.m, d, base + (k - 1) * row, pointer(rules.slot),
+                    pointer(rules.coeff), pointer(rules.constant), pointer(rules.target), rules.n, rules.m, stream,
+                ))] + calls[1:]
+            if not fuse:
+                calls = calls + [self._force_call(pack, states[parity], base + k * row, stream)]
+            lib.call_all(calls)
+        if fuse:
+            lib.call_all([self._force_call(pack, states[(count - 1) % len(sequences)], base + (count - 1) * row,
+                                           stream)])
         if self.variant != "aa" and steps % 2:
             self._f, self._f_next = self._f_next, self._f
         self.time_step += steps

@@ -148,8 +148,8 @@ LBM_API int lbm_affine_values(int dtype, const void* f, const int64_t* slot, con
 /* Momentum-exchange force: out[a] (double) = sum_r weight[r][a] * value[r] over the
  * affine rows (value rounded to dtype, then widened), d components, reduced in a fixed
  * order (lane-strided partial sums, then a tree) that depends on n_rows only, so repeated
- * calls give the same bits. Up to 2048 rows one workgroup writes out directly; above, one
- * workgroup per 256 rows (at most 1024) writes partial sums into `scratch`
+ * calls give the same bits. Workgroups have 1024 lanes; up to 2048 rows one writes out
+ * directly; above, one workgroup per 1024 rows (at most 1024) writes partial sums into `scratch`
  * (lbm_affine_force_scratch_size bytes; may be NULL up to 2048 rows) and a second launch
  * adds them. Calls that share one scratch buffer must not overlap. Replaces
  * ForceProbe.evaluate's host matmul (src/lbm/forces.py:98-103) so a per-step drag
@@ -157,6 +157,19 @@ LBM_API int lbm_affine_values(int dtype, const void* f, const int64_t* slot, con
 LBM_API int lbm_affine_force(int dtype, const void* f, const int64_t* slot, const void* coeff,
                              const void* constant, const double* weight, int64_t n_rows, int m,
                              int d, double* out, void* scratch, void* stream);
+
+/* The force of one step (lbm_affine_force with one workgroup: n_rows <= 2048) and the
+ * link-rule pre-pass of the next step (lbm_rules_apply's one-workgroup form:
+ * n_rules <= 4096) in one launch: both read the same state, every read precedes the rule
+ * writes, and each gives the bits of its separate launch. No counterpart in the reference
+ * (it evaluates ForceProbe on the host, src/lbm/forces.py:98-103, and runs _RulePass,
+ * src/lbm/backends/cuda_solver.py:157-176, as its own launches); saves a launch per step
+ * for per-step drag histories of small lattices. */
+LBM_API int lbm_affine_force_rules(int dtype, void* f, const int64_t* force_slot, const void* force_coeff,
+                                   const void* force_constant, const double* weight, int64_t n_rows,
+                                   int force_m, int d, double* out, const int64_t* rule_slot,
+                                   const void* rule_coeff, const void* rule_constant,
+                                   const int64_t* rule_target, int64_t n_rules, int rule_m, void* stream);
 
 /* Bytes of device scratch lbm_affine_force needs for n_rows rows (0 up to 2048). No
  * counterpart in the reference (its force sum is a host matmul). Host only. */

@@ -94,7 +94,7 @@ def test_force_of_a_large_probe_is_reduced_by_many_groups_deterministically(back
     params = SimulationParams(nx=36, ny=28, nz=28, tau=0.7, collision="trt", solid=solid, dtype="float64",
                               link_rules=bouzidi_rules(D3Q19, links, solid), force=(1e-5, 0.0, 0.0))
     probe = ForceProbe.build(params, links)
-    assert len(links) > 2048  # 16 groups of 256 rows
+    assert len(links) > 2048  # 4 groups of 1024 rows
     _, gpu = both(params, backend)
     outs = [torch.zeros(3, dtype=torch.float64, device="cuda") for _ in range(3)]
     for out in outs:

@@ -139,8 +139,17 @@ def test_force_scratch_size_covers_the_partial_sums():
     lib.call("lbm_affine_force_scratch_size", 2048, ctypes.byref(size))
     assert size.value == 0  # one workgroup writes the result directly
     lib.call("lbm_affine_force_scratch_size", 2049, ctypes.byref(size))
-    assert size.value == 3 * 8 * 9
+    assert size.value == 3 * 8 * 3  # one group per 1024 rows
     lib.call("lbm_affine_force_scratch_size", 10**7, ctypes.byref(size))
     assert size.value == 3 * 8 * 1024
     with pytest.raises(ValueError, match="bad arguments"):
         lib.call("lbm_affine_force_scratch_size", -1, ctypes.byref(size))
+
+
+def test_fused_force_and_pre_pass_refuse_sizes_beyond_one_workgroup():
+    lib = library()
+    common = (0, 1, 1, 1, 1, 1)
+    with pytest.raises(ValueError, match="2048 rows"):
+        lib.call("lbm_affine_force_rules", *common, 2049, 4, 2, 1, 1, 1, 1, 1, 10, 4, None)
+    with pytest.raises(ValueError, match="4096 rules"):
+        lib.call("lbm_affine_force_rules", *common, 100, 4, 2, 1, 1, 1, 1, 1, 4097, 4, None)

@@ -78,3 +78,38 @@ def test_autotuned_solver_keeps_its_state():
         s.initialize(*random_state(params))
         s.run(10)
     np.testing.assert_array_equal(a.populations(), b.populations())
+
+
+def batch_3d(dtype):
+    """Uneven D3Q19 lattices: a duct with a Bouzidi sphere and Zou-He ends, a walled box
+    with a moving lid, and a periodic box with a body force."""
+    from lbm.boundaries import cavity_walls
+    from lbm.cases.sphere_drag import duct_sphere
+
+    duct, _, _ = duct_sphere(nx=36, ny=14, nz=12, radius=3.0, centre=(10.0, 7.0, 6.0), dtype=dtype)
+    solid, wall = cavity_walls(n=10, lid_velocity=0.04, dimensions=3)
+    box = SimulationParams(nx=12, ny=12, nz=12, tau=0.7, collision="trt", solid=solid, wall_velocity=wall,
+                           dtype=dtype)
+    periodic = SimulationParams(nx=20, ny=9, nz=7, tau=0.8, collision="bgk", force=(1e-5, 0.0, -2e-6), dtype=dtype)
+    return [duct, box, periodic]
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_batched_d3q19_equals_each_lattice_alone(dtype):
+    """D3Q19 lattices batched (float64 D3Q19 work items are at most 512 lanes; the
+    autotuner skips the shapes the kernels were not compiled for)."""
+    from lbm.batched import BatchedSolver
+    from tests.helpers import random_state
+
+    params = batch_3d(dtype)
+    states = [random_state(p, seed=i) for i, p in enumerate(params)]
+    for autotune in (False, True):
+        batch = BatchedSolver(params, autotune=autotune)
+        for i, state in enumerate(states):
+            batch.initialize(i, *state)
+        batch.run(9)
+        for i, (p, state) in enumerate(zip(params, states, strict=True)):
+            solver = make_solver(p, backend="hip-pull")
+            solver.initialize(*state)
+            solver.run(9)
+            np.testing.assert_array_equal(batch.populations(i), solver.populations())

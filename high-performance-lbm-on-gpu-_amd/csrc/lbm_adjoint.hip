@@ -680,7 +680,7 @@ void dispatch_force_mask(int force_mode, int mask_mode, Go&& go)
 void check_step_args(int force_mode, int mask_mode, const void* kind, const void* wall_u, const void* force_field,
                      const void* body_field)
 {
-    require(mask_mode == 0 || (kind && wall_u), "mask_mode 1 needs kind and wall velocity");
+    require(mask_mode == 0 || kind, "mask_mode 1 needs the kind array");
     require(force_mode >= 0 && force_mode <= 2, "unknown force mode");
     require(force_mode != 1 || (force_field && mask_mode == 1), "field force mode needs the field and kind");
     require(force_mode != 2 || body_field, "dense force mode needs a body field");

@@ -47,7 +47,7 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_batched_p
             const uint32_t row = node / (uint32_t)g.nx;
             const Coord c{(int)(node - row * (uint32_t)g.nx), (int)(row % (uint32_t)g.ny), (int)(row / (uint32_t)g.ny)};
             pull_update<L, T, ForceMode::UNIFORM, MASK>(src + d.f_offset, dst + d.f_offset, kind + d.kind_offset,
-                                                       wall + d.wall_offset, nullptr, nullptr, g, d.r, c);
+                                                       wall ? wall + d.wall_offset : nullptr, nullptr, nullptr, g, d.r, c);
         }
     }
 }
@@ -102,7 +102,7 @@ LBM_API int lbm_batched_pull_step(int lattice, int dtype, const void* src_pool, 
 {
     return guarded("lbm_batched_pull_step", [&] {
         require(n_tiles > 0 && src_pool && dst_pool && desc && tiles, "bad arguments");
-        require(mask_mode == 0 || (kind_pool && wall_pool), "mask_mode 1 needs kind and wall pools");
+        require(mask_mode == 0 || kind_pool, "mask_mode 1 needs the kind pool");
         require(groups > 0, "groups must be positive");
         require(groups_per_cu >= 0, "groups_per_cu must be >= 0 (0: no occupancy cap)");
         const size_t cap = occupancy_lds(groups_per_cu, 0);

@@ -12,7 +12,9 @@
 // Rules are evaluated before any is written: rule values read pre-rule
 // populations only, whatever the geometry (src/lbm/backends/kernels/link_rules.cu:8-11).
 // Up to 4096 rules this is one workgroup with a barrier between the phases;
-// above that, an evaluate launch into scratch followed by a write launch.
+// above that, an evaluate launch into scratch followed by a write launch. When no
+// rule reads a slot another rule writes (the host checks the compiled set), one
+// launch of one rule per lane gives the same bits at any size.
 
 #include "lbm_affine.h"
 
@@ -49,6 +51,24 @@ __global__ __launch_bounds__(256) void k_rules_write(T* __restrict__ F, const in
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r < n) F[target[r]] = value[r];
+}
+
+// The whole pre-pass in one launch for a rule set in which no rule reads a slot that
+// another rule writes (a rule may read its own target: its lane reads before it stores).
+// Every value then comes from pre-rule populations whatever order the lanes run in, so the
+// result equals the two-phase pass bit for bit. `width` (optional) gives each row its own
+// padded term count, as k_affine_values_w.
+template <class T>
+__global__ __launch_bounds__(256) void k_rules_direct(T* __restrict__ F, const int64_t* __restrict__ slot,
+                                                      const T* __restrict__ coeff, const T* __restrict__ constant,
+                                                      const int* __restrict__ width,
+                                                      const int64_t* __restrict__ target, int64_t n, int m)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t t = target[r];
+    const T v = affine_row(F, slot, coeff, constant, n, width ? width[r] : m, r);
+    F[t] = v;
 }
 
 constexpr int SMALL_THREADS = 1024;
@@ -218,6 +238,23 @@ LBM_API int lbm_rules_apply_widths(int dtype, void* f, const int64_t* slot, cons
                                                         static_cast<T*>(scratch));
             k_rules_write<T><<<blocks, 256, 0, s>>>(static_cast<T*>(f), target, static_cast<const T*>(scratch), n_rules);
             check_launch("batched rule pre-pass");
+        });
+    });
+}
+
+LBM_API int lbm_rules_apply_direct(int dtype, void* f, const int64_t* slot, const void* coeff, const void* constant,
+                                   const int* width, const int64_t* target, int64_t n_rules, int m, void* stream)
+{
+    return guarded("lbm_rules_apply_direct", [&] {
+        require(n_rules >= 0 && m >= 1 && m <= 128, "rules need 1 <= m <= 128 terms");
+        if (n_rules == 0) return;
+        require(f && slot && coeff && constant && target, "null pointer");
+        dispatch_real(dtype, [&](auto Tt) {
+            using T = decltype(Tt);
+            k_rules_direct<T><<<(unsigned)((n_rules + 255) / 256), 256, 0, as_stream(stream)>>>(
+                static_cast<T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), width, target,
+                n_rules, m);
+            check_launch("one-launch rule pre-pass");
         });
     });
 }

@@ -61,11 +61,13 @@ struct Coord {
 
 // f_i += the moving-wall term of link i for every set bit i of `bits`, with u_w read at
 // x + S c_i. All wall-velocity loads of the node issue together (the links' terms are
-// evaluated unconditionally, then selected), so a near-wall wavefront waits once.
+// evaluated unconditionally, then selected), so a near-wall wavefront waits once. A null
+// wall array means every wall is at rest: each term is an exact zero and nothing is read.
 template <class L, int S, class T>
 __device__ __forceinline__ void add_wall_terms(T (&f)[L::Q], uint32_t bits, const T* __restrict__ wall_u,
                                                const Coord& c, const Grid& g)
 {
+    if (wall_u == nullptr) return;
     T w[L::Q];
     static_for<0, L::Q>([&](auto I) { w[I] = wall_term<L, I>(wall_u, neighbour<L, I, S>(c.x, c.y, c.z, g), g.n); });
     static_for<0, L::Q>([&](auto I) {

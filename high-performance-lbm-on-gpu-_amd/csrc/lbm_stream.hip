@@ -80,6 +80,8 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_aa_odd(T*
     // +w[i], and after the collision f*_opp(i), which would stream into that same solid node
     // x + c_opp(i) = x - c_i, bounces back into slot i of this node with +w[i] as well (the
     // term of direction i at that node) — so the wall velocities are loaded once.
+    // A null wall array means every wall is at rest: the terms are exact zeros, not read.
+    const bool moving = MASK && blocked && wall_u != nullptr;
     T w[L::Q];
     if (MASK && blocked) {
         static_for<0, L::Q>([&](auto I) {
@@ -88,6 +90,8 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_aa_odd(T*
             f[i] = F[bounce ? (size_t)i * g.n + node
                             : (size_t)L::opp[i] * g.n + neighbour<L, i, -1>(c.x, c.y, c.z, g)];
         });
+    }
+    if (moving) {
         static_for<0, L::Q>([&](auto I) { w[I] = wall_term<L, I>(wall_u, neighbour<L, I, -1>(c.x, c.y, c.z, g), g.n); });
         static_for<0, L::Q>([&](auto I) {
             if ((blocked >> I) & 1u) f[I] = f[I] + w[I];
@@ -96,7 +100,7 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_aa_odd(T*
     T Fx = T(0), Fy = T(0), Fz = T(0);
     if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
     collide<L, FM>(f, r, Fx, Fy, Fz);
-    if (MASK && blocked) {
+    if (moving) {
         static_for<0, L::Q>([&](auto I) {
             constexpr int i = I;
             if ((blocked >> i) & 1u) f[L::opp[i]] = f[L::opp[i]] + w[i];
@@ -187,7 +191,7 @@ __global__ __launch_bounds__(1024) void k_lds(const T* __restrict__ src, T* __re
         if (bounce) blocked |= 1u << i;
         f[i] = tile[bounce ? L::opp[i] * cells + centre : i * cells + e];
     });
-    if (MASK && blocked) {
+    if (MASK && blocked && wall_u != nullptr) {  // null: every wall at rest, zero terms
         static_for<0, L::Q>([&](auto I) {
             constexpr int i = I;
             using Di = Dir<L, i>;
@@ -353,7 +357,6 @@ int run_step(const char* name, Variant v, int lattice, int dtype, const void* a,
         require(force_mode == 0 || mask_mode == 1 || force_mode == 2, "the IBM field needs the kind array");
         require(force_mode != 2 || body != nullptr, "dense force mode needs a body field");
         require(force_mode != 1 || field != nullptr, "field force mode needs a force field");
-        require(mask_mode == 0 || v == Variant::EVEN || wall != nullptr, "wall velocity array missing");
         const double force[3] = {fx, fy, fz};
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
             using L = decltype(Lt);

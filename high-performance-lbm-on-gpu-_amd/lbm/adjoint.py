@@ -220,13 +220,13 @@ class DragAdjoint:
         s = self.solver
         pack = s._rule_passes.get("pull")
         if pack is not None:
-            s._call("lbm_rules_apply", s._dtype_code, pointer(f), pointer(pack.slot), pointer(pack.coeff),
-                    pointer(pack.constant), pointer(pack.target), pack.n, pack.m, pointer(pack.scratch), s._stream)
+            name, args = s._rules_call(pack, pointer(f), s._stream)
+            s._call(name, *args)
 
     def _pull(self, src, dst, field) -> None:
         s = self.solver
         s._call("lbm_pull_step", s._lattice_code, s._dtype_code, pointer(src), pointer(dst), pointer(s._kind),
-                pointer(s._wall), pointer(field), pointer(s._body), *s._step_args())
+                s._wall_ptr, pointer(field), pointer(s._body), *s._step_args())
 
     def _forward_core(self, length: int, drag, fields, records=None) -> None:
         """``length`` steps from solver._f (ending there: length is even or a copy closes it).
@@ -285,7 +285,7 @@ class DragAdjoint:
                 buf_p, fields_p, _ = partner
                 if not self.store_fields:
                     self._ibm_forward(buf_p[jp], fields_p[jp], self._bf_scratch.data_ptr())
-            adj = (L, D, pointer(lam_next), pointer(lam_out), pointer(buf[j]), pointer(s._kind), pointer(s._wall),
+            adj = (L, D, pointer(lam_next), pointer(lam_out), pointer(buf[j]), pointer(s._kind), s._wall_ptr,
                    pointer(fields[j]), pointer(s._body), nx, ny, nz, wp, wm, fx, fy, fz, FORCE_DENSE, s._mask_mode,
                    pointer(self._grad_body), pointer(self._nu), bx, by)
             if with_partner and fuse_rules:
@@ -306,7 +306,7 @@ class DragAdjoint:
                         None if records is None else pointer(records[j]))
             if with_partner:
                 s._call("lbm_ibm_adjoint_pull", *ibm_args, pointer(buf_p[jp]), pointer(buf_p[jp + 1]),
-                        pointer(s._kind), pointer(s._wall), pointer(fields_p[jp]), pointer(s._body), nx, ny, nz,
+                        pointer(s._kind), s._wall_ptr, pointer(fields_p[jp]), pointer(s._body), nx, ny, nz,
                         wp, wm, FORCE_DENSE, s._mask_mode, s._stream)
             else:
                 s._call("lbm_ibm_adjoint", *ibm_args, s._stream)

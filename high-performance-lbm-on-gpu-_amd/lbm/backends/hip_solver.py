@@ -28,7 +28,7 @@ from lbm.backends._native import DTYPE_CODES, FORCE_DENSE, FORCE_FIELD, FORCE_UN
 from lbm.diagnostics import MacroscopicFields
 from lbm.geometry import neighbour_index
 from lbm.lattice import VelocitySet
-from lbm.link_rules import CompiledAffine, compile_rules
+from lbm.link_rules import CompiledAffine, compile_rules, reads_other_targets
 from lbm.solver import SimulationDiverged, Solver
 
 VARIANTS = ("pull", "aa", "lds")
@@ -187,6 +187,9 @@ class HipSolver(Solver):
         self._wall = torch.as_tensor(
             np.ascontiguousarray(p.wall_velocity.reshape(p.d, -1).astype(self._np_dtype)), device=self.device
         )
+        # walls at rest (the usual case) give exact-zero wall terms: the kernels get no
+        # wall array and read no wall velocity
+        self._wall_ptr = pointer(self._wall) if np.any(p.wall_velocity) else None
         self._force_mode = FORCE_FIELD if self._ibm is not None else FORCE_UNIFORM
         self._field = None
         if self._ibm is not None:
@@ -200,6 +203,7 @@ class HipSolver(Solver):
                 pack = _AffinePack(torch, compiled.values, self._np_dtype, self.device)
                 pack.target = torch.as_tensor(np.asarray(compiled.target, dtype=np.int64), device=self.device)
                 pack.scratch = torch.empty(max(1, pack.n), dtype=self._torch_dtype, device=self.device)
+                pack.direct = not reads_other_targets(compiled.target, compiled.values.slots, compiled.values.coeff)
                 self._rule_passes[layout] = pack
         self._affine_cache: dict[int, _AffinePack] = {}
         self._f = torch.empty((p.lattice.q, self._n), dtype=self._torch_dtype, device=self.device)
@@ -360,7 +364,7 @@ class HipSolver(Solver):
         """Launch lists of one AA pair, or of one two-lattice step per buffer parity (the
         state in ``_f``, then in ``_f_next``): pre-pass and stream-collide, in order."""
         L, D = self._lattice_code, self._dtype_code
-        kind, field, body, wall = pointer(self._kind), pointer(self._field), pointer(self._body), pointer(self._wall)
+        kind, field, body, wall = pointer(self._kind), pointer(self._field), pointer(self._body), self._wall_ptr
         args = self._step_args(stream)
         if self.variant == "aa":
             f = pointer(self._f)
@@ -396,11 +400,22 @@ class HipSolver(Solver):
             )))
         pack = self._rule_passes.get(layout)
         if pack is not None:
-            calls.append(("lbm_rules_apply", (
-                self._dtype_code, f, pointer(pack.slot), pointer(pack.coeff), pointer(pack.constant),
-                pointer(pack.target), pack.n, pack.m, pointer(pack.scratch), stream,
-            )))
+            calls.append(self._rules_call(pack, f, stream))
         return calls
+
+    def _rules_call(self, pack: _AffinePack, f: int, stream) -> tuple:
+        """The launch of one compiled rule set on the state at address ``f``: one launch of
+        one rule per lane when no rule reads a slot another rule writes, else the two-phase
+        pass (one workgroup up to 4096 rules, else evaluate into scratch, then write)."""
+        if pack.direct:
+            return ("lbm_rules_apply_direct", (
+                self._dtype_code, f, pointer(pack.slot), pointer(pack.coeff), pointer(pack.constant), None,
+                pointer(pack.target), pack.n, pack.m, stream,
+            ))
+        return ("lbm_rules_apply", (
+            self._dtype_code, f, pointer(pack.slot), pointer(pack.coeff), pointer(pack.constant),
+            pointer(pack.target), pack.n, pack.m, pointer(pack.scratch), stream,
+        ))
 
     def synchronize(self) -> None:
         self._torch.cuda.synchronize()

@@ -5,8 +5,9 @@ simulation (``SimulationParams``: walls, link rules, uniform force; no immersed 
 of the same velocity set and precision. All lattices share three device pools
 (populations x2, kind bytes, wall velocities); a step is
 
-1. one rule pre-pass over the concatenated rules of every lattice
-   (``lbm_rules_apply_widths``, rows keep their own padded width so they round as alone);
+1. one rule pre-pass over the concatenated rules of every lattice, rows keeping their
+   own padded width so they round as alone (``lbm_rules_apply_direct``, one launch, when
+   no rule reads a slot another rule writes; else ``lbm_rules_apply_widths``);
 2. one persistent launch of ``lbm_batched_pull_step`` over the (lattice, tile) work table.
 
 Two steps (the pool ping-pong) are captured in a HIP graph and replayed. Each lattice's
@@ -24,7 +25,7 @@ import numpy as np
 from lbm.backends import _require_device, load_library
 from lbm.backends._native import DTYPE_CODES, pointer
 from lbm.backends.hip_solver import check_block, node_kinds
-from lbm.link_rules import compile_rules
+from lbm.link_rules import compile_rules, reads_other_targets
 
 DEFAULT_GROUPS_PER_CU = 8
 COMPUTE_UNITS = 256
@@ -95,6 +96,7 @@ class BatchedSolver:
         walls = np.concatenate([p.wall_velocity.reshape(-1).astype(self._np_dtype) for p in params_list])
         self._kind = torch.as_tensor(kinds, device=self.device)
         self._wall = torch.as_tensor(walls, device=self.device)
+        self._wall_ptr = pointer(self._wall) if np.any(walls) else None  # none moving: no wall reads
         self._mask_mode = 1 if any(p.solid.any() for p in params_list) else 0
         self._fa = torch.zeros(int(self.f_offset[-1]), dtype=self._tdt, device=self.device)
         self._fb = torch.zeros_like(self._fa)
@@ -138,6 +140,8 @@ class BatchedSolver:
         self._rule_target = torch.as_tensor(np.concatenate(targets), device=dev)
         self._rule_width = torch.as_tensor(np.concatenate(widths), device=dev)
         self._rule_scratch = torch.empty(self._n_rules, dtype=self._tdt, device=dev)
+        # lattices share no slots, so the concatenation is one-launch safe when each set is
+        self._rule_direct = not reads_other_targets(np.concatenate(targets), slot.T, coeff.T)
 
     def _build_tiles(self) -> None:
         """Work items (lattice, first node): runs of bx * by consecutive nodes of one lattice."""
@@ -177,13 +181,17 @@ class BatchedSolver:
 
     def _step(self, src, dst) -> None:
         stream = self._torch.cuda.current_stream().cuda_stream
-        if self._n_rules:
+        if self._n_rules and self._rule_direct:
+            self._lib.call("lbm_rules_apply_direct", self._dcode, pointer(src), pointer(self._rule_slot),
+                           pointer(self._rule_coeff), pointer(self._rule_const), pointer(self._rule_width),
+                           pointer(self._rule_target), self._n_rules, self._rule_m, stream)
+        elif self._n_rules:
             self._lib.call("lbm_rules_apply_widths", self._dcode, pointer(src), pointer(self._rule_slot),
                            pointer(self._rule_coeff), pointer(self._rule_const), pointer(self._rule_target),
                            pointer(self._rule_width), self._n_rules, self._rule_m, pointer(self._rule_scratch), stream)
         bx, by = self.block
         self._lib.call("lbm_batched_pull_step", self._lcode, self._dcode, pointer(src), pointer(dst),
-                       pointer(self._kind), pointer(self._wall), pointer(self._desc), pointer(self._tiles),
+                       pointer(self._kind), self._wall_ptr, pointer(self._desc), pointer(self._tiles),
                        self._n_tiles, self._mask_mode, bx, by, self.groups, self.groups_per_cu, stream)
 
     def _pair(self) -> None:

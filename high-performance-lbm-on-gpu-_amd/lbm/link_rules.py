@@ -19,7 +19,9 @@ layout         where f*_k(y) is stored                          slot a rule writ
 
 The written slot is the one the stream-collide kernel's bounce-back branch
 reads, so kernels need no rule awareness. The device pre-pass evaluates every
-rule before writing any (see ``csrc/lbm_rules.hip``).
+rule before writing any (see ``csrc/lbm_rules.hip``), or, for a set in which no
+rule reads a slot another rule writes (``reads_other_targets``), evaluates and
+writes each rule in one launch with the same result.
 """
 
 from __future__ import annotations
@@ -249,3 +251,24 @@ def compile_rules(lattice: VelocitySet, rules: LinkRules, layout: str, solid, wa
         target=compile_targets(lattice, rules, layout, n),
         values=compile_affine(lattice, rules.values, layout, solid, wall_velocity),
     )
+
+
+def reads_other_targets(target, slots, coeff) -> bool:
+    """Whether some rule has a nonzero term on a slot that a *different* rule writes.
+
+    When it does not, evaluating and writing each rule in one pass, in any order, gives the
+    values of the two-phase pre-pass (every rule reads pre-rule populations; a rule that
+    reads its own target does so before it writes it), so the device can apply the set in
+    one launch. ``slots``/``coeff`` are (n_rules, m); zero coefficients are padding.
+    """
+    target = np.asarray(target, dtype=np.int64)
+    if target.size == 0:
+        return False
+    coeff = np.asarray(coeff)
+    rows, terms = np.nonzero(coeff != 0)
+    read = np.asarray(slots, dtype=np.int64)[rows, terms]
+    order = np.argsort(target, kind="stable")
+    written = target[order]
+    at = np.minimum(np.searchsorted(written, read), written.size - 1)
+    hit = written[at] == read
+    return bool(np.any(order[at[hit]] != rows[hit]))

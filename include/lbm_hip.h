@@ -13,7 +13,9 @@
  * bit 0 solid, bit 1 IBM-forced fluid node, bit 2 fluid node with a solid
  * neighbour at some x - c_i (set by the host, lets interior wavefronts skip
  * the bounce-back checks). wall_u and force fields are [d][z][y][x] arrays
- * of the population precision.
+ * of the population precision; wall_u (and the batched wall pool) may be NULL
+ * when every wall is at rest: the moving-wall terms are then exact zeros and
+ * no wall velocity is read.
  *
  * Each declaration names the reference interface it replaces
  * (ShauryaAttreya37/High-Performance-LBM-On-GPU-, paths relative to its root).
@@ -138,6 +140,17 @@ LBM_API int lbm_rules_apply(int dtype, void* f, const int64_t* slot, const void*
 LBM_API int lbm_rules_apply_widths(int dtype, void* f, const int64_t* slot, const void* coeff,
                                    const void* constant, const int64_t* target, const int* width,
                                    int64_t n_rules, int m, void* scratch, void* stream);
+
+/* The pre-pass of lbm_rules_apply in one launch, one rule per lane, no scratch, for a rule
+ * set in which no rule reads a slot that another rule writes (a rule may read its own
+ * target). The caller checks that property on the compiled set (the Python host does it
+ * once per layout); under it every rule reads pre-rule populations in any lane order, so
+ * the result equals lbm_rules_apply's bit for bit. width (may be NULL) gives row r its own
+ * padded term count, as in lbm_rules_apply_widths. Same role as _RulePass.__call__
+ * (src/lbm/backends/cuda_solver.py:157-176), which always runs its two kernels. */
+LBM_API int lbm_rules_apply_direct(int dtype, void* f, const int64_t* slot, const void* coeff,
+                                   const void* constant, const int* width, const int64_t* target,
+                                   int64_t n_rules, int m, void* stream);
 
 /* Values of compiled affine expressions on the state: out[r] (dtype), same
  * arithmetic as the rule evaluation. Replaces Solver.evaluate_affine

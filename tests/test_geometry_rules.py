@@ -14,7 +14,7 @@ from lbm.forces import ForceProbe
 from lbm.geometry import Pipe, circle, sphere, voxelize, wall_links
 from lbm.ibm import ImmersedBoundary, ImmersedForcing, roma
 from lbm.lattice import D2Q9, D3Q19, LATTICES, VelocitySet, default_lattice
-from lbm.link_rules import AffineSet, LinkRules, bouzidi_rules, compile_affine
+from lbm.link_rules import AffineSet, LinkRules, bouzidi_rules, compile_affine, compile_rules, reads_other_targets
 from lbm.open_boundaries import open_channel
 
 # ---------------------------------------------------------------- shapes
@@ -127,6 +127,35 @@ def test_rules_must_sit_on_fluid_to_resting_solid_links():
         rules.validate(D2Q9, solid, moving)
     with pytest.raises(ValueError, match="more than one rule"):
         LinkRules.concat(rules, rules)
+
+
+def test_rules_reading_another_rules_target_are_detected():
+    target = np.array([10, 20])
+    coeff = np.array([[0.5, 0.5], [1.0, 0.0]])
+    # rule 0 reads slot 20, which rule 1 writes: the two-phase pass is needed
+    assert reads_other_targets(target, np.array([[20, 3], [4, 0]]), coeff)
+    # a rule reading its own target is no dependency (its lane reads before it writes)
+    assert not reads_other_targets(target, np.array([[10, 3], [4, 0]]), coeff)
+    # a zero coefficient is padding, its slot never read
+    assert not reads_other_targets(target, np.array([[3, 20], [4, 0]]), np.array([[1.0, 0.0], [1.0, 0.0]]))
+    assert not reads_other_targets(np.zeros(0, np.int64), np.zeros((0, 2), np.int64), np.zeros((0, 2)))
+
+
+def test_body_and_open_end_rules_are_one_launch_safe_in_every_layout():
+    body, solid, links = disc_channel()
+    ny, nx = solid.shape
+    y = np.arange(ny) - 0.5
+    inflow = np.stack([np.where((y > 0) & (y < ny - 2), 0.01, 0.0), np.zeros(ny)])
+    solid, open_rules = open_channel(D2Q9, solid, inflow)
+    links = wall_links(D2Q9, body, solid)
+    rules = LinkRules.concat(open_rules, bouzidi_rules(D2Q9, links, solid))
+    walls = np.zeros((2, ny, nx))
+    for layout in ("pull", "aa_odd", "aa_even"):
+        c = compile_rules(D2Q9, rules, layout, solid, walls)
+        assert not reads_other_targets(c.target, c.values.slots, c.values.coeff), layout
+    # in the pull layout a Bouzidi row with q < 1/2 reads f*_opp(j)(x), the slot it writes
+    c = compile_rules(D2Q9, rules, "pull", solid, walls)
+    assert ((c.values.slots == c.target[:, None]) & (c.values.coeff != 0)).any()
 
 
 def test_unknown_layout_is_rejected():

@@ -140,3 +140,63 @@ def test_body_at_rest_in_fluid_at_rest_feels_no_force():
     solver = make_solver(params, backend="hip-aa")
     solver.run(10)
     np.testing.assert_allclose(solver.immersed_force, 0.0, atol=1e-16)
+
+
+def rule_sets():
+    """Rule sets of the case geometries: a D2Q9 channel with a Bouzidi cylinder and Zou-He
+    ends (under 4096 rules: the one-workgroup two-phase pass; over: evaluate + write) and a
+    D3Q19 duct with a Bouzidi sphere."""
+    from lbm.cases.cylinder_benchmark import drag_channel
+    from lbm.cases.sphere_drag import duct_sphere
+
+    small = drag_channel(nx=220, ny=82, radius=10.0, centre=(50.0, 40.0))[0]
+    large = drag_channel(nx=1024, ny=600, radius=40.0, centre=(256.0, 300.0))[0]
+    duct = duct_sphere(nx=64, ny=30, nz=30, radius=5.0, centre=(20.0, 15.0, 15.0))[0]
+    return {"d2q9-small": small, "d2q9-large": large, "d3q19-duct": duct}
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("layout", ["pull", "aa_odd", "aa_even"])
+def test_one_launch_pre_pass_equals_the_two_phase_pass(layout, dtype):
+    """lbm_rules_apply_direct (evaluate and write each rule in one launch) leaves the state
+    bit for bit as lbm_rules_apply (every value before any write) on sets in which no rule
+    reads another rule's target, also with per-row widths against lbm_rules_apply_widths."""
+    import torch
+
+    from lbm.backends._native import DTYPE_CODES, library, pointer
+    from lbm.backends.hip_solver import _AffinePack
+    from lbm.link_rules import compile_rules, reads_other_targets
+
+    lib = library()
+    code = DTYPE_CODES[dtype]
+    tdt = getattr(torch, dtype)
+    sets = rule_sets()
+    assert len(sets["d2q9-small"].link_rules) <= 4096 < len(sets["d2q9-large"].link_rules)
+    for name, params in sets.items():
+        compiled = compile_rules(params.lattice, params.link_rules, layout, params.solid, params.wall_velocity)
+        assert not reads_other_targets(compiled.target, compiled.values.slots, compiled.values.coeff), name
+        pack = _AffinePack(torch, compiled.values, np.dtype(dtype), "cuda")
+        target = torch.as_tensor(np.asarray(compiled.target, dtype=np.int64), device="cuda")
+        scratch = torch.empty(pack.n, dtype=tdt, device="cuda")
+        gen = torch.Generator(device="cuda").manual_seed(7)
+        f = torch.rand((params.lattice.q, params.n_nodes), generator=gen, dtype=tdt, device="cuda")
+        two, one = f.clone(), f.clone()
+        lib.call("lbm_rules_apply", code, pointer(two), pointer(pack.slot), pointer(pack.coeff),
+                 pointer(pack.constant), pointer(target), pack.n, pack.m, pointer(scratch), None)
+        lib.call("lbm_rules_apply_direct", code, pointer(one), pointer(pack.slot), pointer(pack.coeff),
+                 pointer(pack.constant), None, pointer(target), pack.n, pack.m, None)
+        torch.cuda.synchronize()
+        assert not torch.equal(two, f), name  # the rules changed the state
+        assert torch.equal(one, two), name
+        # the same set with every row declared one term narrower than the arrays' width
+        width = torch.full((pack.n,), pack.m, dtype=torch.int32, device="cuda")
+        wide = _AffinePack(torch, type(compiled.values)(
+            np.pad(compiled.values.slots, ((0, 0), (0, 1))), np.pad(compiled.values.coeff, ((0, 0), (0, 1))),
+            compiled.values.constant), np.dtype(dtype), "cuda")
+        two, one = f.clone(), f.clone()
+        lib.call("lbm_rules_apply_widths", code, pointer(two), pointer(wide.slot), pointer(wide.coeff),
+                 pointer(wide.constant), pointer(target), pointer(width), wide.n, wide.m, pointer(scratch), None)
+        lib.call("lbm_rules_apply_direct", code, pointer(one), pointer(wide.slot), pointer(wide.coeff),
+                 pointer(wide.constant), pointer(width), pointer(target), wide.n, wide.m, None)
+        torch.cuda.synchronize()
+        assert torch.equal(one, two), name

@@ -76,6 +76,10 @@ def test_invalid_block_and_grid_are_rejected_by_the_library_before_launch():
         lib.call("lbm_lds_step", 1, 0, 1, 2, None, None, None, None, 8, 8, 8, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 32, 8, 0, None)
     with pytest.raises(ValueError, match="terms"):
         lib.call("lbm_rules_apply", 0, 1, 1, 1, 1, 1, 5, 200, None, None)
+    with pytest.raises(ValueError, match="terms"):
+        lib.call("lbm_rules_apply_direct", 0, 1, 1, 1, 1, None, 1, 5, 0, None)
+    with pytest.raises(ValueError, match="null pointer"):
+        lib.call("lbm_rules_apply_direct", 0, 1, 1, 1, 1, None, None, 5, 4, None)
     # float64 D3Q19 kernels are compiled for at most 512 lanes per workgroup
     d3 = (1, 1, 1, 2, None, None, None, None, 8, 8, 8, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0)
     with pytest.raises(ValueError, match="512 threads for float64 D3Q19"):

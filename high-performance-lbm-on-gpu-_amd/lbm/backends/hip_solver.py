@@ -9,7 +9,8 @@ Per time step (``pull``/``lds``) or pair of half-steps (``aa``):
 
 1. pre-pass for the layout the next kernel reads (``pull``, ``aa_odd``,
    ``aa_even``): the fused IBM force (``lbm_ibm_force``), then the link rules
-   (``lbm_rules_apply``);
+   (``lbm_rules_apply_direct``, or ``lbm_rules_apply`` when a rule reads a slot another
+   rule writes);
 2. the stream-collide kernel (``lbm_pull_step``, ``lbm_aa_odd`` /
    ``lbm_aa_even``, ``lbm_lds_step``).
 
@@ -59,22 +60,41 @@ def check_block(block) -> tuple[int, int]:
 # longer; float64 moves twice the bytes per lane, so half the wavefronts saturate HBM.
 _SHAPE_3D = {("aa", "float32"): (512, 2), ("pull", "float32"): (256, 4), ("aa", "float64"): (256, 2),
              ("pull", "float64"): (256, 2)}
+# Lattices with solid or forced nodes (the kernels read the kind bytes): every row whose
+# ends are solid has a near-solid lane at each end, which takes the longer bounce-back
+# path, so smaller workgroups let the rest of the CU move on. 512 x 128 x 128 duct with a
+# Bouzidi sphere, AA fp32: 256 x 1 with cap 4 gave 31.9 K MLUPS against 30.7 K for 256 x 2
+# with cap 2.
+_SHAPE_3D_MASKED = {("aa", "float32"): (256, 4)}
+_D2Q9_LANES, _D2Q9_MASKED_LANES = 1024, 512
+
+
+def _masked(params) -> bool:
+    return bool(params.solid.any()) or params.immersed is not None
+
+
+def _shape_3d(params, variant: str) -> tuple[int, int]:
+    key = ("aa" if variant == "aa" else "pull", params.dtype)
+    return _SHAPE_3D_MASKED.get(key, _SHAPE_3D[key]) if _masked(params) else _SHAPE_3D[key]
 
 
 def default_block(params, variant: str) -> tuple[int, int]:
     """Whole rows per workgroup where possible. D2Q9: bx covers nx up to 1024 lanes (a
     multiple of the 64-lane wavefront) and by stacks rows up to 1024 lanes; 1024 x 1 was
-    1.5-4 % above 256 x 4 on periodic 2048^2 / 4096^2 and Bouzidi channels 2048 x 512 /
-    8192 x 2048. The LDS-tiled kernel keeps bx <= 256: its halo ring costs
-    2 (bx + 2) + 2 by cells per bx * by, so 256 x 4 and 128 x 8 win (D2Q9 fp32 4096^2:
-    76 K MLUPS against 62 K for 64 x 4 and 55 K for 32 x 8). D3Q19: bx up to 256 with the
-    measured lanes per workgroup of ``_SHAPE_3D``."""
+    1.5-4 % above 256 x 4 on periodic 2048^2 / 4096^2. Lattices with solid or forced nodes
+    take rows of up to 512 lanes: Bouzidi/Zou-He channels 4096^2, 8192 x 2048 and 2048^2
+    and a 4096^2 lid-driven cavity ran 1-8 % faster at 512 x 1 than at 1024 x 1 (a
+    near-solid lane at each end of every row holds its workgroup longer). The LDS-tiled
+    kernel keeps bx <= 256: its halo ring costs 2 (bx + 2) + 2 by cells per bx * by, so
+    256 x 4 and 128 x 8 win (D2Q9 fp32 4096^2: 76 K MLUPS against 62 K for 64 x 4 and 55 K
+    for 32 x 8). D3Q19: bx up to 256 with the measured lanes per workgroup of
+    ``_SHAPE_3D`` (``_SHAPE_3D_MASKED`` for lattices with solid or forced nodes)."""
     if params.d == 2:
-        bx = min(256 if variant == "lds" else 1024, 64 * math.ceil(params.nx / 64))
-        return bx, max(1, min(params.ny, 1024 // bx))
+        lanes = _D2Q9_MASKED_LANES if (_masked(params) and variant != "lds") else _D2Q9_LANES
+        bx = min(256 if variant == "lds" else lanes, 64 * math.ceil(params.nx / 64))
+        return bx, max(1, min(params.ny, lanes // bx))
     bx = min(256, 64 * math.ceil(params.nx / 64))
-    lanes = _SHAPE_3D[("aa" if variant == "aa" else "pull", params.dtype)][0]
-    return bx, max(1, min(params.ny, lanes // bx))
+    return bx, max(1, min(params.ny, _shape_3d(params, variant)[0] // bx))
 
 
 INFINITY_CACHE_BYTES = 256 * 2**20
@@ -82,13 +102,13 @@ INFINITY_CACHE_BYTES = 256 * 2**20
 
 def default_groups_per_cu(params, variant: str) -> int:
     """Resident workgroups per compute unit for the stream-collide kernels (0: no cap):
-    the measured D3Q19 caps of ``_SHAPE_3D``. D2Q9 runs uncapped except all-fluid lattices
-    whose state is more than twice the 256 MB Infinity Cache, where one 1024-lane group per
-    CU moved 4096^2 from 76.2 K to 81.4 K MLUPS (AA, 604 MB) and 81.5 K to 84.8 K (pull,
+    the measured D3Q19 caps of ``_SHAPE_3D`` / ``_SHAPE_3D_MASKED``. D2Q9 runs uncapped
+    except all-fluid lattices whose state is more than twice the 256 MB Infinity Cache,
+    where one 1024-lane group per CU moved 4096^2 from 76.2 K to 81.4 K MLUPS (AA, 604 MB) and 81.5 K to 84.8 K (pull,
     1.2 GB); the same cap cost 6-15 % on Bouzidi channels and at 2048^2 (151 / 302 MB)."""
     if params.d == 3:
-        return _SHAPE_3D[("aa" if variant == "aa" else "pull", params.dtype)][1]
-    if variant == "lds" or params.solid.any() or params.immersed is not None:
+        return _shape_3d(params, variant)[1]
+    if variant == "lds" or _masked(params):
         return 0
     state = params.lattice.q * params.n_nodes * np.dtype(params.dtype).itemsize * (1 if variant == "aa" else 2)
     return 1 if state > 2 * INFINITY_CACHE_BYTES else 0

@@ -28,14 +28,26 @@ def test_d3q19_shapes_at_256(variant, dtype, block, cap):
     assert default_groups_per_cu(p, variant) == cap
 
 
+@pytest.mark.parametrize(("variant", "dtype", "block", "cap"), [
+    ("aa", "float32", (256, 1), 4), ("pull", "float32", (256, 1), 4),
+    ("aa", "float64", (256, 1), 2), ("pull", "float64", (256, 1), 2),
+])
+def test_d3q19_shapes_with_solid_nodes(variant, dtype, block, cap):
+    p = box(256, 3, dtype, solid=True)
+    assert default_block(p, variant) == block
+    assert default_groups_per_cu(p, variant) == cap
+
+
 def test_d2q9_rows_and_the_cap_rule():
     big = box(4096, 2)  # 604 MB (AA) / 1.2 GB (pull): beyond twice the Infinity Cache
     assert default_block(big, "pull") == (1024, 1) and default_groups_per_cu(big, "pull") == 1
     assert default_groups_per_cu(big, "aa") == 1
     mid = box(2048, 2)  # 151 MB / 302 MB: uncapped
     assert default_groups_per_cu(mid, "aa") == 0 and default_groups_per_cu(mid, "pull") == 0
-    walls = box(4096, 2, solid=True)  # masked lattices run uncapped
+    walls = box(4096, 2, solid=True)  # masked lattices run uncapped, in rows of 512 lanes
     assert default_groups_per_cu(walls, "pull") == 0
+    assert default_block(walls, "pull") == default_block(walls, "aa") == (512, 1)
+    assert default_block(walls, "lds") == (256, 4)
     assert default_block(big, "lds") == (256, 4) and default_groups_per_cu(big, "lds") == 0
     narrow = SimulationParams(nx=45, ny=29, tau=0.6)
     assert default_block(narrow, "pull") == (64, 16)

@@ -14,11 +14,20 @@ each timed run, and
 Grids whose population state fits in the 256 MB Infinity Cache can exceed HBM bandwidth;
 the table flags them.
 
+``--bodies`` instead times each lattice with a body against the same lattice empty (the
+reference's "bodies cost little" comparison, README.md:70-72: a Bouzidi cylinder with a
+Zou-He inlet and outlet kept 96 % of the empty-domain throughput at D2Q9 1024^2, 77-83 %
+in 3D): a Bouzidi cylinder of radius n/16 in an n x n channel, or a Bouzidi sphere in an
+n x n/4 x n/4 duct, with a parabolic Zou-He inlet and a pressure outlet, rule pre-pass
+included, against the fully periodic lattice of the same size and tau.
+
 Usage::
 
     python -m lbm.bench_mlups                                   # D2Q9, all backends, 256..4096
     python -m lbm.bench_mlups --dim 3 --backends hip-pull hip-aa --sizes 64 128 256
     python -m lbm.bench_mlups --dtype float64 --csv results.csv
+    python -m lbm.bench_mlups --bodies --backends hip-pull hip-aa            # 1024^2, 4096^2
+    python -m lbm.bench_mlups --bodies --dim 3 --backends hip-pull hip-aa    # 512 x 128 x 128
 """
 
 from __future__ import annotations
@@ -122,6 +131,63 @@ def measure(torch, backend: str, size: int, dim: int, dtype: str, target_seconds
                        state <= INFINITY_CACHE_BYTES, tuple(block), cap)
 
 
+@dataclass(frozen=True)
+class BodiesMeasurement:
+    backend: str
+    case: str
+    nodes: int
+    rules: int
+    empty_mlups: float
+    bodies_mlups: float
+    ratio: float
+
+
+def bodies_case(size: int, dim: int, dtype: str = "float32"):
+    """(name, params, initial x velocity) of the with-body lattice of size ``size``: a
+    Bouzidi cylinder (radius n/16 at (n/4, n/2)) in an n x n channel, or a Bouzidi sphere
+    (radius 12 n/512 at (n/4, n/8, n/8)) in an n x n/4 x n/4 duct, each with a parabolic
+    Zou-He inlet, a pressure outlet and no-slip side walls, started from the inflow. The
+    Reynolds number grows with the size (20 at 1024^2 and at 512 x 128 x 128) so that tau is
+    the same at every size: 1.46 in 2D, 0.62 in 3D."""
+    if dim == 2:
+        from lbm.cases.cylinder_benchmark import drag_channel
+
+        params, _, profile = drag_channel(reynolds=20.0 * size / 1024, nx=size, ny=size, radius=size / 16,
+                                          centre=(size / 4, size / 2), dtype=dtype)
+        return f"channel {size}x{size}", params, np.where(params.solid, 0.0, profile[:, None])
+    from lbm.cases.sphere_drag import duct_sphere
+
+    side = size // 4
+    params, _, inflow = duct_sphere(reynolds=20.0 * size / 512, nx=size, ny=side, nz=side,
+                                    radius=12.0 * size / 512, centre=(size / 4, side / 2, side / 2), dtype=dtype)
+    return f"duct {size}x{side}x{side}", params, np.where(params.solid, 0.0, inflow[0][:, :, None])
+
+
+def _rate(torch, params, backend: str, ux, target_seconds: float) -> float:
+    solver = make_solver(params, backend=backend)
+    solver.initialize(1.0, ux, *([0.0] * (params.d - 1)))
+    probe = 2 * solver.step_multiple
+    timed_run(solver, probe)
+    per_step = timed_run(solver, probe) / probe
+    steps = solver.step_multiple * max(1, round(target_seconds / per_step / solver.step_multiple))
+    best = max(params.n_nodes * steps / timed_run(solver, steps) / 1e6 for _ in range(2))
+    del solver
+    torch.cuda.empty_cache()
+    return best
+
+
+def measure_bodies(torch, backend: str, size: int, dim: int, dtype: str = "float32",
+                   target_seconds: float = TARGET_SECONDS) -> BodiesMeasurement:
+    """MLUPS of the with-body lattice of ``bodies_case`` (best of two timed runs, every node
+    counted, rule pre-pass included) and of the fully periodic lattice of the same size."""
+    name, params, ux = bodies_case(size, dim, dtype)
+    sizes = {"nx": params.nx, "ny": params.ny} | ({"nz": params.nz} if dim == 3 else {})
+    empty = SimulationParams(**sizes, tau=params.tau, collision=params.collision, dtype=dtype)
+    e = _rate(torch, empty, backend, 0.0, target_seconds)
+    b = _rate(torch, params, backend, ux, target_seconds)
+    return BodiesMeasurement(backend, name, params.n_nodes, len(params.link_rules), e, b, b / e)
+
+
 def main(argv=None) -> None:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--backends", nargs="+", default=list(BACKENDS), choices=BACKENDS)
@@ -130,7 +196,11 @@ def main(argv=None) -> None:
     parser.add_argument("--dtype", default="float32", choices=("float32", "float64"))
     parser.add_argument("--seconds", type=float, default=TARGET_SECONDS, help="length of each timed run")
     parser.add_argument("--csv", default=None, help="write the table to this CSV file")
+    parser.add_argument("--bodies", action="store_true",
+                        help="time lattices with a body and open ends against the same lattices empty")
     args = parser.parse_args(argv)
+    if args.sizes is None and args.bodies:
+        args.sizes = [1024, 4096] if args.dim == 2 else [512]
     if args.sizes is None:
         args.sizes = [256, 512, 1024, 2048, 4096] if args.dim == 2 else [64, 96, 128, 192, 256]
     backends = args.backends
@@ -144,6 +214,8 @@ def main(argv=None) -> None:
 
     import torch
 
+    if args.bodies:
+        return _main_bodies(torch, args, backends)
     print(f"{torch.cuda.get_device_name(0)}: {'D2Q9' if args.dim == 2 else 'D3Q19'} {args.dtype}, "
           f"Infinity Cache {INFINITY_CACHE_BYTES // 2**20} MiB")
     tiny = make_solver(grid_params(8, args.dim, args.dtype), backend=backends[0])
@@ -166,6 +238,25 @@ def main(argv=None) -> None:
             print(f"{m.backend:>9} {m.size:6d} {m.mlups:9.0f} {m.bandwidth_gbs:8.0f} {m.peak_gbs:7.0f} "
                   f"{100 * m.fraction_of_peak:6.1f}% {m.block[0]:>4}x{m.block[1]:<4} {m.groups_per_cu:>3}  {note}",
                   flush=True)
+    if args.csv and results:
+        with open(args.csv, "w", newline="") as handle:
+            writer = csv.DictWriter(handle, fieldnames=list(asdict(results[0])))
+            writer.writeheader()
+            writer.writerows(asdict(r) for r in results)
+        print(f"table written to {args.csv}")
+
+
+def _main_bodies(torch, args, backends) -> None:
+    print(f"{torch.cuda.get_device_name(0)}: {'D2Q9' if args.dim == 2 else 'D3Q19'} {args.dtype}, "
+          "with a body and open ends against empty")
+    print(f"{'backend':>9} {'case':>22} {'rules':>8} {'empty':>9} {'bodies':>9} {'ratio':>6}")
+    results = []
+    for backend in backends:
+        for size in args.sizes:
+            m = measure_bodies(torch, backend, size, args.dim, args.dtype, args.seconds)
+            results.append(m)
+            print(f"{m.backend:>9} {m.case:>22} {m.rules:8d} {m.empty_mlups:9.0f} {m.bodies_mlups:9.0f} "
+                  f"{m.ratio:6.3f}", flush=True)
     if args.csv and results:
         with open(args.csv, "w", newline="") as handle:
             writer = csv.DictWriter(handle, fieldnames=list(asdict(results[0])))

@@ -200,9 +200,13 @@ __global__ __launch_bounds__(256) void k_adj_pull(const T* __restrict__ lam_next
 }
 
 // The adjoint step plus an independent link-rule pre-pass on another state buffer, in one
-// launch: the grid has one extra z layer whose block (0, 0) runs the pre-pass
-// (rules_in_one_group, at most 4 rules per lane). Used to overlap the forward recompute
-// of one checkpoint segment with the backward sweep of the next. Blocks may have up to 1024
+// launch: the grid has one extra z layer, the first, which runs the pre-pass. With
+// `direct`, for a rule set in which no rule reads a slot another rule writes, each lane of
+// the layer evaluates and writes one rule (as k_rules_direct), so the adjoint keeps its own
+// block shape; otherwise block (0, 0) runs the two-phase pass (rules_in_one_group, at most
+// 4 rules per lane). Workgroups are dispatched in block order, so the pre-pass starts with
+// the first adjoint workgroups. Used to overlap the forward recompute of one checkpoint
+// segment with the backward sweep of the next. Blocks may have up to 1024
 // lanes (D2Q9 float32), so the pre-pass gets one lane per rule; the float64 and D3Q19
 // adjoints need more than the 128 VGPRs a 1024-lane block leaves, so their blocks stay at
 // 512 and 256 lanes.
@@ -217,10 +221,20 @@ __global__ __launch_bounds__(adj_rules_max_threads(L::D, sizeof(T))) void k_adj_
                                                         T* __restrict__ nu_forced, Grid g, Relaxation<T> r,
                                                         T* __restrict__ rf, const int64_t* __restrict__ rslot,
                                                         const T* __restrict__ rcoeff, const T* __restrict__ rconstant,
-                                                        const int64_t* __restrict__ rtarget, int64_t n_rules, int rm)
+                                                        const int64_t* __restrict__ rtarget, int64_t n_rules, int rm,
+                                                        int direct)
 {
-    if ((int)blockIdx.z == g.nz) {
-        if (blockIdx.x == 0 && blockIdx.y == 0)
+    if (blockIdx.z == 0) {
+        if (direct) {
+            const int64_t lanes = (int64_t)blockDim.x * blockDim.y;
+            const int64_t rr = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * lanes + threadIdx.y * blockDim.x +
+                               threadIdx.x;
+            if (rr < n_rules) {
+                const int64_t t = rtarget[rr];
+                const T v = affine_row(rf, rslot, rcoeff, rconstant, n_rules, rm, rr);
+                rf[t] = v;
+            }
+        } else if (blockIdx.x == 0 && blockIdx.y == 0)
             rules_in_one_group<T, ADJ_RULES_PER_LANE>(rf, rslot, rcoeff, rconstant, rtarget, n_rules, rm,
                                                       threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
         return;
@@ -229,7 +243,7 @@ __global__ __launch_bounds__(adj_rules_max_threads(L::D, sizeof(T))) void k_adj_
     const int y = blockIdx.y * blockDim.y + threadIdx.y;
     if (x < g.nx && y < g.ny)
         adj_pull_node<L, T, FM, MASK>(lam_next, lam_out, f, kind, wall_u, field, body, grad_body, nu_forced, g, r, x,
-                                      y, blockIdx.z);
+                                      y, blockIdx.z - 1);
 }
 
 // Transposed rows: lam[slot[t]] += sum_{e in [start[t], start[t+1])} coeff[e] * x[row[e]]
@@ -760,7 +774,8 @@ LBM_API int lbm_adj_pull_step_rules(int lattice, int dtype, const void* lam_next
                                     double omega_minus, double fx, double fy, double fz, int force_mode,
                                     int mask_mode, double* grad_body, void* nu_forced, int bx, int by, void* rule_f,
                                     const int64_t* rule_slot, const void* rule_coeff, const void* rule_constant,
-                                    const int64_t* rule_target, int64_t n_rules, int rule_m, void* stream)
+                                    const int64_t* rule_target, int64_t n_rules, int rule_m, int rules_direct,
+                                    void* stream)
 {
     return guarded("lbm_adj_pull_step_rules", [&] {
         const Grid g = make_grid(lattice, nx, ny, nz);
@@ -769,15 +784,17 @@ LBM_API int lbm_adj_pull_step_rules(int lattice, int dtype, const void* lam_next
                 "block must be positive with at most 1024 (D2Q9 f32), 512 (D2Q9 f64) or 256 (D3Q19) threads");
         require(lam_next && lam_out && f && lam_next != lam_out, "adjoint buffers must be distinct and non-null");
         check_step_args(force_mode, mask_mode, kind, wall_u, force_field, body_field);
-        require(n_rules >= 1 && n_rules <= (int64_t)ADJ_RULES_PER_LANE * bx * by,
-                "the fused pre-pass takes 1 .. 4 * bx * by rules");
+        const dim3 layer = grid_of(g, dim3(bx, by, 1));
+        const int64_t layer_lanes = (int64_t)layer.x * layer.y * bx * by;
+        require(n_rules >= 1 && n_rules <= (rules_direct ? layer_lanes : (int64_t)ADJ_RULES_PER_LANE * bx * by),
+                "the fused pre-pass takes 1 .. 4 * bx * by rules, or one per lane of a grid layer when direct");
         require(rule_m >= 1 && rule_m <= 128, "rules need 1 <= m <= 128 terms");
         require(rule_f && rule_slot && rule_coeff && rule_constant && rule_target, "null rule pointer");
         require(rule_f != f && rule_f != lam_out, "the pre-pass state must be another buffer");
         const double force[3] = {fx, fy, fz};
         const dim3 block(bx, by, 1);
         dim3 grid = grid_of(g, block);
-        grid.z += 1;  // the pre-pass layer
+        grid.z += 1;  // the pre-pass layer (z = 0)
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
             using L = decltype(Lt);
             using T = decltype(Tt);
@@ -789,7 +806,7 @@ LBM_API int lbm_adj_pull_step_rules(int lattice, int dtype, const void* lam_next
                         static_cast<const T*>(wall_u), static_cast<const T*>(force_field),
                         static_cast<const T*>(body_field), grad_body, static_cast<T*>(nu_forced), g, r,
                         static_cast<T*>(rule_f), rule_slot, static_cast<const T*>(rule_coeff),
-                        static_cast<const T*>(rule_constant), rule_target, n_rules, rule_m);
+                        static_cast<const T*>(rule_constant), rule_target, n_rules, rule_m, rules_direct);
             });
             check_launch("adjoint step with pre-pass");
         });

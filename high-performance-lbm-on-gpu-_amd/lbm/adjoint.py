@@ -273,8 +273,13 @@ class DragAdjoint:
         bx, by = self.adjoint_block
         nx, ny, nz = s.params.nx, s.params.ny, s._nz
         pack = s._rule_passes.get("pull")
-        fbx, fby = self.fused_block
-        fuse_rules = pack is not None and pack.n <= 4 * fbx * fby
+        # the recompute's pre-pass rides in the adjoint step's launch: one rule per lane of an
+        # extra grid layer at the adjoint's own block shape when no rule reads another's
+        # target, else one workgroup of up to 4 rules per lane
+        layer = -(-nx // bx) * -(-ny // by) * bx * by
+        direct = pack is not None and pack.direct and pack.n <= layer
+        fbx, fby = (bx, by) if direct else self.fused_block
+        fuse_rules = pack is not None and (direct or pack.n <= 4 * fbx * fby)
         r = self._rules or {"target": None, "n": 0, "uslot": None, "ustart": None, "urule": None,
                             "ucoeff": None, "nu": 0, "scratch": None}
         for j in reversed(range(length)):
@@ -290,7 +295,8 @@ class DragAdjoint:
                    pointer(self._grad_body), pointer(self._nu), bx, by)
             if with_partner and fuse_rules:
                 s._call("lbm_adj_pull_step_rules", *adj[:-2], fbx, fby, pointer(buf_p[jp]), pointer(pack.slot),
-                        pointer(pack.coeff), pointer(pack.constant), pointer(pack.target), pack.n, pack.m, s._stream)
+                        pointer(pack.coeff), pointer(pack.constant), pointer(pack.target), pack.n, pack.m,
+                        int(direct), s._stream)
             else:
                 if with_partner:
                     self._rules_forward(buf_p[jp])

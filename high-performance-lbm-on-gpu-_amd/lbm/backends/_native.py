@@ -45,7 +45,7 @@ SIGNATURES: dict[str, list] = {
     "lbm_batched_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p],
     "lbm_adj_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _p, _p, _i, _i, _p],
     "lbm_adj_pull_step_rules": [_i, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _p, _p,
-                                _i, _i, _p, _p, _p, _p, _p, _i64, _i, _p],
+                                _i, _i, _p, _p, _p, _p, _p, _i64, _i, _i, _p],
     "lbm_adj_rules": [_i, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p],
     "lbm_ibm_adjoint_scratch_size": [_i, _i, _i, _i, _i, POINTER(c_int64)],
     "lbm_ibm_adjoint": [_i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _i, _p, _p, _i, _d, _d, _d, _i64, _p, _p, _p,

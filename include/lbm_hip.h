@@ -236,11 +236,14 @@ LBM_API int lbm_adj_pull_step(int lattice, int dtype, const void* lam_next, void
                               double omega_minus, double fx, double fy, double fz, int force_mode,
                               int mask_mode, double* grad_body, void* nu_forced, int bx, int by, void* stream);
 
-/* lbm_adj_pull_step plus, in the same launch, an independent link-rule pre-pass (as
- * lbm_rules_apply, 1 <= n_rules <= 4 * bx * by) on another state buffer rule_f; bx * by up to
- * 1024 for D2Q9 float32, 512 for D2Q9 float64 and 256 for D3Q19. No counterpart
- * in the reference: the checkpointed adjoint uses it to run the forward recompute of one
- * segment under the backward sweep of the next, saving a launch per step. */
+/* lbm_adj_pull_step plus, in the same launch, an independent link-rule pre-pass on another
+ * state buffer rule_f: as lbm_rules_apply in one workgroup (1 <= n_rules <= 4 * bx * by), or
+ * with rules_direct = 1 as lbm_rules_apply_direct, one rule per lane of an extra grid layer
+ * (n_rules up to the lanes of one z layer of the grid; only for sets in which no rule reads a
+ * slot another rule writes). bx * by up to 1024 for D2Q9 float32, 512 for D2Q9 float64 and 256
+ * for D3Q19. No counterpart in the reference: the checkpointed adjoint uses it to run the
+ * forward recompute of one segment under the backward sweep of the next, saving a launch per
+ * step. */
 LBM_API int lbm_adj_pull_step_rules(int lattice, int dtype, const void* lam_next, void* lam_out, const void* f,
                                     const uint8_t* kind, const void* wall_u, const void* force_field,
                                     const void* body_field, int nx, int ny, int nz, double omega_plus,
@@ -248,7 +251,7 @@ LBM_API int lbm_adj_pull_step_rules(int lattice, int dtype, const void* lam_next
                                     int mask_mode, double* grad_body, void* nu_forced, int bx, int by,
                                     void* rule_f, const int64_t* rule_slot, const void* rule_coeff,
                                     const void* rule_constant, const int64_t* rule_target, int64_t n_rules,
-                                    int rule_m, void* stream);
+                                    int rule_m, int rules_direct, void* stream);
 
 /* Transposed link-rule pre-pass (no counterpart in the reference): a_r = lam[target_r],
  * lam[target_r] = 0, then lam[uslot[u]] += sum_{e in [ustart[u], ustart[u+1])} ucoeff[e] a[urule[e]]

@@ -234,3 +234,39 @@ def test_recorded_forcing_equals_recomputed_forcing(dtype):
     assert results[0][0] == results[1][0]
     np.testing.assert_array_equal(results[0][1], results[1][1])
     np.testing.assert_array_equal(results[0][2], results[1][2])
+
+
+@pytest.mark.gpu
+def test_fused_adjoint_step_gives_the_same_bits_with_either_pre_pass():
+    """lbm_adj_pull_step_rules with the one-workgroup two-phase pre-pass and with one rule
+    per lane of the extra grid layer (rules_direct): the adjoint outputs and the pre-passed
+    state agree bit for bit on the Zou-He set, in which no rule reads another's target."""
+    from lbm import make_solver
+    from lbm.backends._native import FORCE_DENSE, library, pointer
+
+    params = case(dtype="float64", rules=True)
+    s = make_solver(params, backend="hip-pull")
+    pack = s._rule_passes["pull"]
+    assert pack.direct and 0 < pack.n <= 4 * 256
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    q, n = params.lattice.q, params.n_nodes
+    rand = lambda *shape: torch.rand(shape, generator=gen, dtype=torch.float64, device="cuda")  # noqa: E731
+    lam_next, f, rf0 = rand(q, n), rand(q, n), rand(q, n)
+    field, body = rand(2, n) * 1e-3, rand(2, n) * 1e-6
+    wp, wm = params.relaxation_rates
+    outs = []
+    for direct in (0, 1):
+        lam_out = torch.zeros(q, n, dtype=torch.float64, device="cuda")
+        grad = torch.zeros(2, n, dtype=torch.float64, device="cuda")
+        nu = torch.zeros(2, n, dtype=torch.float64, device="cuda")
+        rf = rf0.clone()
+        library().call("lbm_adj_pull_step_rules", 0, 1, pointer(lam_next), pointer(lam_out), pointer(f),
+                       pointer(s._kind), s._wall_ptr, pointer(field), pointer(body), params.nx, params.ny, 1, wp, wm,
+                       0.0, 0.0, 0.0, FORCE_DENSE, 1, pointer(grad), pointer(nu), 64, 4, pointer(rf),
+                       pointer(pack.slot), pointer(pack.coeff), pointer(pack.constant), pointer(pack.target), pack.n,
+                       pack.m, direct, None)
+        torch.cuda.synchronize()
+        outs.append((lam_out, grad, nu, rf))
+    assert not torch.equal(outs[0][3], rf0)  # the pre-pass wrote its targets
+    for a, b in zip(*outs, strict=True):
+        assert torch.equal(a, b)

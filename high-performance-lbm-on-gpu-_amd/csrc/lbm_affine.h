@@ -87,6 +87,71 @@ __device__ __forceinline__ T affine_row(const T* __restrict__ F, const int64_t* 
     return c + affine_sum(F, slot, coeff, stride, r, m);
 }
 
+// One row's coefficients, slots and constant fetched ahead (rows of at most M terms, 8 <= M
+// <= 16), so a kernel can issue those loads before other independent work and gather
+// from F afterwards; affine_finish then sums in affine_row's order (8 interleaved partial
+// sums over whole 8-term chunks, their fixed tree, then the rest one by one), giving the
+// same bits.
+template <class T, int M>
+struct AffineAhead {
+    static_assert(M >= 8 && M <= 16, "rows of 8 to 16 terms");
+    T a[M];
+    int64_t s[M];
+    T c;
+};
+
+template <class T, int M>
+__device__ __forceinline__ void affine_fetch(const int64_t* __restrict__ slot, const T* __restrict__ coeff,
+                                             const T* __restrict__ constant, int64_t stride, int m, int64_t r,
+                                             AffineAhead<T, M>& p)
+{
+    p.c = constant[r];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        if (k < m) {
+            const int64_t e = (int64_t)k * stride + r;
+            p.a[k] = coeff[e];
+            p.s[k] = slot[e];
+        }
+    }
+}
+
+template <class T, int M>
+__device__ __forceinline__ T affine_finish(const T* __restrict__ F, const AffineAhead<T, M>& p, int m)
+{
+    T t[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        if (k < m) {
+            const bool zero = p.a[k] == T(0);
+            const T v = F[zero ? 0 : p.s[k]];
+            t[k] = zero ? T(0) : p.a[k] * v;
+        }
+    }
+    T s = T(0);
+    if (m < 8) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < m) s = s + t[k];
+        return p.c + s;
+    }
+    T acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = t[k];
+    const int rest = (M == 16 && m == 16) ? 16 : 8;  // a second whole chunk only when m == 16
+    if constexpr (M == 16) {
+        if (m == 16) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[k] = acc[k] + t[8 + k];
+        }
+    }
+    s = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+#pragma unroll
+    for (int k = 8; k < M; ++k)
+        if (k >= rest && k < m) s = s + t[k];
+    return p.c + s;
+}
+
 // The link-rule pre-pass by one workgroup, in two halves so a kernel can do other reads of
 // the state between them: rules_values evaluates rows r = tid + k * nthreads (k < PER, so
 // at most PER * nthreads rules) into registers and fetches their targets; after a barrier,

@@ -64,6 +64,15 @@ __global__ __launch_bounds__(ibm_max_threads(D)) void k_ibm(const T* __restrict_
     IbmTables<T> tb{inverse_g, weights_g, areas_g, start_g, entry_g};
     T* total;
     T* base;
+    const int rows = (D + 1) * nn;
+    // With one moment row per lane (2D), the row's coefficients and slots go out before the
+    // table copies, so its population gathers wait one latency after the tables, not two.
+    constexpr int AHEAD = 9;
+    const bool ahead = D == 2 && m <= AHEAD && rows <= nthreads;
+    AffineAhead<T, AHEAD> pre;
+    if constexpr (D == 2) {
+        if (ahead && tid < rows) affine_fetch(slot, coeff, constant, rows, m, (int64_t)tid, pre);
+    }
     if constexpr (STAGED) {
         base = reinterpret_cast<T*>(ibm_lds);
         total = base + ibm_forward_layout(base, D, nn, nm, slots, &s);
@@ -82,10 +91,13 @@ __global__ __launch_bounds__(ibm_max_threads(D)) void k_ibm(const T* __restrict_
         const size_t used = ibm_forward_layout(base, D, nn, nm, slots, &s);
         total = RECORD ? scratch : base + used;
     }
-    const int rows = (D + 1) * nn;
     const T half_force[3] = {hfx, hfy, hfz};
     // P0: affine moment rows, pairwise over the m padded terms.
-    for (int r = tid; r < rows; r += nthreads) s.moments[r] = affine_row(F, slot, coeff, constant, rows, m, r);
+    if (ahead) {
+        if (tid < rows) s.moments[tid] = affine_finish(F, pre, m);
+    } else {
+        for (int r = tid; r < rows; r += nthreads) s.moments[r] = affine_row(F, slot, coeff, constant, rows, m, r);
+    }
     __syncthreads();
     ibm_iterate<T, D>(s, tb, nn, nm, iterations, RECORD, half_force, total, tid, nthreads);
     const T* last = s.force + (RECORD ? (size_t)iterations * D * nn : 0);

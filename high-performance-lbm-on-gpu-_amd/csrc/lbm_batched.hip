@@ -20,11 +20,16 @@ template <class T>
 struct LatticeDesc {
     Grid g;
     int pad;
-    uint64_t f_offset;     // elements into the population pools (q * n per lattice)
+    uint64_t f_offset;     // elements into the population pools (q * g.ns per lattice)
     uint64_t kind_offset;  // bytes into the kind pool (n per lattice)
     uint64_t wall_offset;  // elements into the wall pool (d * n per lattice)
     Relaxation<T> r;
 };
+
+// Population planes in the pool start every POOL_PLANE_ALIGN elements (each lattice's
+// node count rounded up), so every plane of every lattice begins on a cache-line boundary
+// and a wavefront's unshifted access covers two 128-byte lines, not three.
+constexpr uint32_t POOL_PLANE_ALIGN = 64;
 
 struct TileDesc {
     int lattice;
@@ -38,17 +43,32 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_batched_p
                                                        const TileDesc* __restrict__ tiles, int n_tiles)
 {
     const uint32_t lane = threadIdx.y * blockDim.x + threadIdx.x;
-    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-        const TileDesc td = tiles[t];
-        const LatticeDesc<T>& d = lattices[td.lattice];
+    int t = blockIdx.x;
+    if (t >= n_tiles) return;
+    TileDesc td = tiles[t];
+    for (;;) {
+        // The next tile's descriptor is requested (from an index valid either way) before
+        // this tile's update, so the table read overlaps this tile's population traffic
+        // instead of stalling every wavefront of the workgroup between tiles.
+        const int tn = t + (int)gridDim.x;
+        const bool more = tn < n_tiles;
+        const TileDesc next = tiles[more ? tn : t];
+        // Tile and lattice descriptors are the same for the whole workgroup: kept in scalar
+        // registers, the lattice descriptor is one scalar read, not per-lane vector loads.
+        const int lat = __builtin_amdgcn_readfirstlane(td.lattice);
+        const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane((int)td.start);
+        const LatticeDesc<T>& d = lattices[lat];
         const Grid g = d.g;
-        const uint32_t node = td.start + lane;
+        const uint32_t node = first + lane;
         if (node < g.n) {
             const uint32_t row = node / (uint32_t)g.nx;
             const Coord c{(int)(node - row * (uint32_t)g.nx), (int)(row % (uint32_t)g.ny), (int)(row / (uint32_t)g.ny)};
             pull_update<L, T, ForceMode::UNIFORM, MASK>(src + d.f_offset, dst + d.f_offset, kind + d.kind_offset,
                                                        wall ? wall + d.wall_offset : nullptr, nullptr, nullptr, g, d.r, c);
         }
+        if (!more) break;
+        t = tn;
+        td = next;
     }
 }
 
@@ -82,11 +102,12 @@ LBM_API int lbm_batched_describe(int lattice, int dtype, int n_lattices, const i
             for (int l = 0; l < n_lattices; ++l) {
                 LatticeDesc<T> d{};
                 d.g = make_grid(lattice, nx[l], ny[l], nz[l]);
+                d.g.ns = (uint32_t)(((uint64_t)d.g.n + POOL_PLANE_ALIGN - 1) / POOL_PLANE_ALIGN * POOL_PLANE_ALIGN);
                 d.f_offset = f;
                 d.kind_offset = k;
                 d.wall_offset = w;
                 d.r = make_relaxation<L, T>(omega_plus[l], omega_minus[l], force + 3 * l);
-                f += (uint64_t)L::Q * d.g.n;
+                f += (uint64_t)L::Q * d.g.ns;
                 k += d.g.n;
                 w += (uint64_t)L::D * d.g.n;
                 out[l] = d;

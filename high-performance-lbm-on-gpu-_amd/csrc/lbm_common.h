@@ -81,6 +81,7 @@ void dispatch_real(int dtype, F&& f)
 struct Grid {
     int nx, ny, nz;
     uint32_t n;
+    uint32_t ns;  // elements from one population plane to the next: n, or more in padded pools
 };
 
 inline Grid make_grid(int lattice, int nx, int ny, int nz)
@@ -89,7 +90,7 @@ inline Grid make_grid(int lattice, int nx, int ny, int nz)
     require(lattice == LBM_D3Q19 || nz == 1, "a D2Q9 grid has nz = 1");
     const uint64_t n = (uint64_t)nx * ny * nz;
     require(n < (1ull << 31), "grids are limited to 2^31 nodes");
-    return Grid{nx, ny, nz, (uint32_t)n};
+    return Grid{nx, ny, nz, (uint32_t)n, (uint32_t)n};
 }
 
 // Lanes per workgroup the stream-collide kernels are compiled for: 1024, except float64

@@ -91,7 +91,8 @@ __device__ __forceinline__ uint32_t flat(const Coord& c, const Grid& g)
 // ------------------------------------------------------------------ kernels
 
 // One node of the two-lattice fused pull step (shared by the single-lattice and the
-// batched kernels, so both produce the same bits).
+// batched kernels, so both produce the same bits). Population planes are g.ns elements
+// apart (n for one lattice; padded in the batched pools); wall and force fields g.n.
 template <class L, class T, ForceMode FM, bool MASK>
 __device__ __forceinline__ void pull_update(const T* __restrict__ src, T* __restrict__ dst,
                                             const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
@@ -106,7 +107,7 @@ __device__ __forceinline__ void pull_update(const T* __restrict__ src, T* __rest
     // moving-wall terms, all Q loads of each pass in flight together.
     const uint8_t k = MASK ? kind[node] : 0;
     T f[L::Q];
-    static_for<0, L::Q>([&](auto I) { f[I] = src[(size_t)I * g.n + neighbour<L, I, -1>(c.x, c.y, c.z, g)]; });
+    static_for<0, L::Q>([&](auto I) { f[I] = src[(size_t)I * g.ns + neighbour<L, I, -1>(c.x, c.y, c.z, g)]; });
     if (MASK && (k & KIND_SOLID)) return;
     if (MASK && (k & KIND_NEAR_SOLID)) {
         const uint32_t blocked = blocked_upstream<L>(kind, c.x, c.y, c.z, g);
@@ -114,8 +115,8 @@ __device__ __forceinline__ void pull_update(const T* __restrict__ src, T* __rest
             static_for<0, L::Q>([&](auto I) {
                 constexpr int i = I;
                 const bool bounce = (blocked >> i) & 1u;
-                f[i] = src[bounce ? (size_t)L::opp[i] * g.n + node
-                                  : (size_t)i * g.n + neighbour<L, i, -1>(c.x, c.y, c.z, g)];
+                f[i] = src[bounce ? (size_t)L::opp[i] * g.ns + node
+                                  : (size_t)i * g.ns + neighbour<L, i, -1>(c.x, c.y, c.z, g)];
             });
             add_wall_terms<L, -1>(f, blocked, wall_u, c, g);
         }
@@ -123,7 +124,7 @@ __device__ __forceinline__ void pull_update(const T* __restrict__ src, T* __rest
     T Fx = T(0), Fy = T(0), Fz = T(0);
     if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
     collide<L, FM>(f, r, Fx, Fy, Fz);
-    static_for<0, L::Q>([&](auto I) { dst[(size_t)I * g.n + node] = f[I]; });
+    static_for<0, L::Q>([&](auto I) { dst[(size_t)I * g.ns + node] = f[I]; });
 }
 
 }  // namespace lbm

@@ -29,6 +29,14 @@ from lbm.link_rules import compile_rules, reads_other_targets
 
 DEFAULT_GROUPS_PER_CU = 8
 COMPUTE_UNITS = 256
+PLANE_ALIGN = 64
+"""Population planes of each lattice start every PLANE_ALIGN elements in the pool (the
+node count rounded up, as ``lbm_batched_describe`` lays them out): every plane begins on a
+cache-line boundary."""
+
+
+def plane_stride(n: int) -> int:
+    return -(-int(n) // PLANE_ALIGN) * PLANE_ALIGN
 
 
 class BatchedSolver:
@@ -88,7 +96,8 @@ class BatchedSolver:
         self._lib.call("lbm_batched_describe", self._lcode, self._dcode, n_l, nx.ctypes.data, ny.ctypes.data,
                        nz.ctypes.data, wp.ctypes.data, wm.ctypes.data, force.ctypes.data, host.ctypes.data)
         self._desc = torch.as_tensor(host, device=self.device)
-        self.f_offset = np.concatenate([[0], np.cumsum([q * n for n in self.nodes])]).astype(np.int64)
+        self.strides = [plane_stride(n) for n in self.nodes]
+        self.f_offset = np.concatenate([[0], np.cumsum([q * s for s in self.strides])]).astype(np.int64)
         node_offset = np.concatenate([[0], np.cumsum(self.nodes)]).astype(np.int64)
         self.node_offset = node_offset
 
@@ -119,11 +128,16 @@ class BatchedSolver:
             if p.link_rules is None or not len(p.link_rules):
                 continue
             c = compile_rules(p.lattice, p.link_rules, "pull", p.solid, p.wall_velocity)
-            base = int(self.f_offset[i])
-            slots.append(np.asarray(c.values.slots) + base)
+            base, n, stride = int(self.f_offset[i]), p.n_nodes, self.strides[i]
+
+            def pooled(flat, base=base, n=n, stride=stride):  # slot (k, y) = k n + y -> k stride + y
+                flat = np.asarray(flat, dtype=np.int64)
+                return flat // n * stride + flat % n + base
+
+            slots.append(pooled(c.values.slots))
             coeffs.append(np.asarray(c.values.coeff))
             consts.append(np.asarray(c.values.constant))
-            targets.append(np.asarray(c.target) + base)
+            targets.append(pooled(c.target))
             widths.append(np.full(len(c.target), c.values.coeff.shape[1], dtype=np.int32))
         self._n_rules = int(sum(len(t) for t in targets))
         if not self._n_rules:
@@ -159,10 +173,11 @@ class BatchedSolver:
     # ------------------------------------------------------------------ state
 
     def view(self, i: int, pool=None):
-        """Populations of lattice i as a (q, N_i) view of the current pool."""
+        """Populations of lattice i as a (q, N_i) view of the current pool (planes
+        ``strides[i]`` elements apart)."""
         pool = self._fa if pool is None else pool
         q = self.lattice.q
-        return pool[int(self.f_offset[i]) : int(self.f_offset[i + 1])].view(q, self.nodes[i])
+        return pool[int(self.f_offset[i]) : int(self.f_offset[i + 1])].view(q, self.strides[i])[:, : self.nodes[i]]
 
     def initialize(self, i: int, rho=1.0, ux=0.0, uy=0.0, uz=0.0) -> None:
         """Equilibrium of lattice i (same construction as ``Solver.initialize``)."""
@@ -173,10 +188,11 @@ class BatchedSolver:
         torch, dev = self._torch, self.device
         dr = torch.as_tensor(np.ascontiguousarray(drho).ravel(), device=dev)
         comps = [torch.as_tensor(np.ascontiguousarray(a).ravel(), device=dev) for a in u]
-        f = self.view(i)
+        f = torch.empty((p.lattice.q, p.n_nodes), dtype=self._tdt, device=dev)  # contiguous planes
         self._lib.call("lbm_init_equilibrium", self._lcode, self._dcode, pointer(f), pointer(dr), pointer(comps[0]),
                        pointer(comps[1]), pointer(comps[2] if len(comps) == 3 else None), p.n_nodes, 0,
                        torch.cuda.current_stream().cuda_stream)
+        self.view(i).copy_(f)
         self.view(i, self._fb).copy_(f)
 
     def _step(self, src, dst) -> None:

@@ -308,8 +308,10 @@ LBM_API int lbm_batched_desc_size(int dtype, int64_t* bytes);
 
 /* Fill n_lattices descriptors (host memory, lbm_batched_desc_size bytes each) for lattices
  * of sizes nx/ny/nz[l], rates omega_plus/minus[l] and uniform force force[3 l .. 3 l + 2],
- * laid out back to back in the pools: populations q n_l, kind bytes n_l, wall d n_l.
- * No counterpart in the reference (batching is new). */
+ * laid out back to back in the pools: kind bytes n_l, wall d n_l, and populations q s_l,
+ * where population plane i of lattice l starts s_l elements after plane i - 1 and s_l is
+ * n_l rounded up to a multiple of 64 (so every plane starts on a cache-line boundary; the
+ * padding is never read). No counterpart in the reference (batching is new). */
 LBM_API int lbm_batched_describe(int lattice, int dtype, int n_lattices, const int* nx, const int* ny,
                                  const int* nz, const double* omega_plus, const double* omega_minus,
                                  const double* force, void* host_desc);

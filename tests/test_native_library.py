@@ -157,3 +157,35 @@ def test_fused_force_and_pre_pass_refuse_sizes_beyond_one_workgroup():
         lib.call("lbm_affine_force_rules", *common, 2049, 4, 2, 1, 1, 1, 1, 1, 10, 4, None)
     with pytest.raises(ValueError, match="4096 rules"):
         lib.call("lbm_affine_force_rules", *common, 100, 4, 2, 1, 1, 1, 1, 1, 4097, 4, None)
+
+
+def test_batched_descriptors_lay_out_the_padded_pools_the_host_expects():
+    """lbm_batched_describe (host only) pads every lattice's population planes to the
+    stride lbm.batched.plane_stride computes, and places the lattices back to back: the
+    library's descriptors and the Python pool views agree on every offset."""
+    import struct
+
+    from lbm.batched import PLANE_ALIGN, plane_stride
+
+    lib = library()
+    size = ctypes.c_int64()
+    lib.call("lbm_batched_desc_size", 0, ctypes.byref(size))
+    sizes = [(40, 30), (77, 300), (64, 64), (1000, 3), (33, 1)]
+    nx = np.array([s[0] for s in sizes], dtype=np.int32)
+    ny = np.array([s[1] for s in sizes], dtype=np.int32)
+    nz = np.ones(len(sizes), dtype=np.int32)
+    wp, wm = np.full(len(sizes), 1.25), np.full(len(sizes), 1.1)
+    force = np.zeros(3 * len(sizes))
+    raw = np.zeros(len(sizes) * size.value, dtype=np.uint8)
+    lib.call("lbm_batched_describe", 0, 0, len(sizes), nx.ctypes.data, ny.ctypes.data, nz.ctypes.data,
+             wp.ctypes.data, wm.ctypes.data, force.ctypes.data, raw.ctypes.data)
+    data = raw.tobytes()
+    f = kind = 0
+    for lat, (x, y) in enumerate(sizes):
+        d = data[lat * size.value : (lat + 1) * size.value]
+        n, ns = struct.unpack_from("<II", d, 12)  # Grid {nx, ny, nz, n, ns}
+        f_offset, kind_offset = struct.unpack_from("<QQ", d, 24)
+        assert n == x * y and ns == plane_stride(n) and ns % PLANE_ALIGN == 0 and ns - n < PLANE_ALIGN
+        assert (f_offset, kind_offset) == (f, kind)
+        f += 9 * ns
+        kind += n

@@ -137,6 +137,10 @@ __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T*
         fs[i] = f[(size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)];
         lam[i] = lam_next[(size_t)i * g.n + node];
     });
+    if constexpr (MASK) {
+        hold_loaded(fs);
+        hold_loaded(lam);
+    }
     if (MASK && (k & KIND_SOLID)) return;
     uint32_t blocked = 0;
     if (MASK && (k & KIND_NEAR_SOLID)) {

@@ -6,6 +6,21 @@
 
 namespace lbm {
 
+// ------------------------------------------------------------------ load order
+
+// Holds the values of `v` in registers at this point of the program. The masked kernels
+// load a node's kind byte and its populations together, then leave early for a solid
+// node; without this the compiler moves the population loads below that exit (only the
+// fluid path uses them), so every wavefront would wait for its kind bytes before issuing
+// its population loads: a second memory round trip. Placed before the exit, the loads are
+// issued with the kind byte and waited for together. The statement has no instructions.
+template <class T, int N>
+__device__ __forceinline__ void hold_loaded(const T (&v)[N])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) asm volatile("" : : "v"(v[i]));
+}
+
 // ------------------------------------------------------------------ indexing
 
 __device__ __forceinline__ int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
@@ -108,6 +123,7 @@ __device__ __forceinline__ void pull_update(const T* __restrict__ src, T* __rest
     const uint8_t k = MASK ? kind[node] : 0;
     T f[L::Q];
     static_for<0, L::Q>([&](auto I) { f[I] = src[(size_t)I * g.ns + neighbour<L, I, -1>(c.x, c.y, c.z, g)]; });
+    if constexpr (MASK) hold_loaded(f);
     if (MASK && (k & KIND_SOLID)) return;
     if (MASK && (k & KIND_NEAR_SOLID)) {
         const uint32_t blocked = blocked_upstream<L>(kind, c.x, c.y, c.z, g);

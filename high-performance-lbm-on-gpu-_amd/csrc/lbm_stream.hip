@@ -45,6 +45,7 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_aa_even(T
     const uint8_t k = MASK ? kind[node] : 0;
     T f[L::Q];
     static_for<0, L::Q>([&](auto I) { f[I] = F[(size_t)I * g.n + node]; });
+    if constexpr (MASK) hold_loaded(f);
     if (MASK && (k & KIND_SOLID)) return;
     T Fx = T(0), Fy = T(0), Fz = T(0);
     if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
@@ -73,6 +74,7 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_aa_odd(T*
     static_for<0, L::Q>([&](auto I) {
         f[I] = F[(size_t)L::opp[I] * g.n + neighbour<L, I, -1>(c.x, c.y, c.z, g)];
     });
+    if constexpr (MASK) hold_loaded(f);
     if (MASK && (k & KIND_SOLID)) return;
     const uint32_t blocked = (MASK && (k & KIND_NEAR_SOLID)) ? blocked_upstream<L>(kind, c.x, c.y, c.z, g) : 0u;
     // w[i]: moving-wall term of link i, u_w read at the solid node x - c_i (meaningful for

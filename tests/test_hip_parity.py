@@ -177,3 +177,38 @@ def test_device_lattice_tables_match_the_host_tables():
                 np.testing.assert_array_equal(ci[axis], vs.component(axis))
             np.testing.assert_array_equal(opp.cpu().numpy(), vs.opposite)
             np.testing.assert_array_equal(w.cpu().numpy(), np.asarray(vs.w, dtype=w.cpu().numpy().dtype))
+
+
+@pytest.mark.parametrize(("sizes", "force"), [
+    ({"nx": 1, "ny": 12}, (1e-5, 2e-5)), ({"nx": 10, "ny": 1}, (1e-5, 2e-5)), ({"nx": 1, "ny": 1}, (1e-5, 2e-5)),
+    ({"nx": 6, "ny": 5, "nz": 1}, (1e-5, 2e-5, 3e-5)), ({"nx": 1, "ny": 1, "nz": 7}, (1e-5, 2e-5, 3e-5)),
+], ids=["1x12", "10x1", "1x1", "6x5x1", "1x1x7"])
+def test_one_node_wide_periodic_grids_are_bit_identical_to_the_oracle(sizes, force):
+    """Axes of one node wrap onto themselves (every neighbour along them is the node
+    itself): the kernels' periodic wrap must give the oracle's np.roll result, for every
+    variant the grid allows."""
+    params = SimulationParams(**sizes, tau=0.7, collision="trt", force=force, dtype="float64")
+    backends = ("hip-pull", "hip-aa", "hip-lds") if params.d == 2 else ("hip-pull", "hip-aa")
+    for backend in backends:
+        reference, gpu = run_pair(params, backend)
+        if params.n_nodes == 1:
+            # One node: the oracle's sums over the Q populations run along a contiguous axis,
+            # where NumPy switches to its pairwise (8-accumulator) order, so only the
+            # reference's own bound holds (tests/test_cuda_backend.py:186-207: < 1e-15)
+            assert np.abs(gpu.populations() - reference.populations()).max() < 1e-15, backend
+        else:
+            np.testing.assert_array_equal(gpu.populations(), reference.populations(), err_msg=backend)
+
+
+def test_an_empty_rule_set_changes_nothing():
+    """A LinkRules of no rules runs the plain kernels: the same bits as no rules at all."""
+    from dataclasses import replace
+
+    from lbm.link_rules import AffineSet, LinkRules
+
+    params = mixed_params("float64", "trt")
+    empty = LinkRules(np.zeros(0, np.int64), np.zeros(0, np.int64), AffineSet.from_lists([]))
+    for backend in ("hip-pull", "hip-aa"):
+        _, plain = run_pair(params, backend, steps=10)
+        _, with_empty = run_pair(replace(params, link_rules=empty), backend, steps=10)
+        np.testing.assert_array_equal(with_empty.populations(), plain.populations())

@@ -189,3 +189,13 @@ def test_batched_descriptors_lay_out_the_padded_pools_the_host_expects():
         assert (f_offset, kind_offset) == (f, kind)
         f += 9 * ns
         kind += n
+
+
+def test_grids_past_two_to_the_31_nodes_and_empty_rule_sets_are_handled_before_any_gpu_work():
+    lib = library()
+    huge = (1, 0, None, None, None, None, None, None, 2048, 2048, 2048, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 256, 1, 0, None)
+    with pytest.raises(ValueError, match=r"2\^31 nodes"):
+        lib.call("lbm_pull_step", *huge)
+    # no rules: nothing to read, nothing launched, success
+    lib.call("lbm_rules_apply", 0, None, None, None, None, None, 0, 4, None, None)
+    lib.call("lbm_rules_apply_direct", 0, None, None, None, None, None, None, 0, 4, None)

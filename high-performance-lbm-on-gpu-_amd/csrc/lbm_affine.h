@@ -17,6 +17,8 @@
 
 #include <cstdint>
 
+#include "lbm_hold.h"
+
 namespace lbm {
 
 template <class T>
@@ -34,14 +36,18 @@ __device__ __forceinline__ void affine_terms(const T* __restrict__ F, const int6
             s[k] = slot[e];
         }
     }
+    // The gathers go out together from their selected addresses (hold_value: otherwise
+    // each would sit behind a branch on its coefficient with a wait of its own).
+    T v[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        if (k < count) {
-            const bool zero = a[k] == T(0);
-            const T v = F[zero ? 0 : s[k]];
-            term[k] = zero ? T(0) : a[k] * v;
-        }
-    }
+    for (int k = 0; k < 8; ++k)
+        if (k < count) v[k] = F[a[k] == T(0) ? 0 : s[k]];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (k < count) hold_value(v[k]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (k < count) term[k] = a[k] == T(0) ? T(0) : a[k] * v[k];
 }
 
 // Pairwise sum of the m terms of row r (no constant).
@@ -119,15 +125,16 @@ __device__ __forceinline__ void affine_fetch(const int64_t* __restrict__ slot, c
 template <class T, int M>
 __device__ __forceinline__ T affine_finish(const T* __restrict__ F, const AffineAhead<T, M>& p, int m)
 {
-    T t[M];
+    T v[M], t[M];
 #pragma unroll
-    for (int k = 0; k < M; ++k) {
-        if (k < m) {
-            const bool zero = p.a[k] == T(0);
-            const T v = F[zero ? 0 : p.s[k]];
-            t[k] = zero ? T(0) : p.a[k] * v;
-        }
-    }
+    for (int k = 0; k < M; ++k)
+        if (k < m) v[k] = F[p.a[k] == T(0) ? 0 : p.s[k]];
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (k < m) hold_value(v[k]);
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (k < m) t[k] = p.a[k] == T(0) ? T(0) : p.a[k] * v[k];
     T s = T(0);
     if (m < 8) {
 #pragma unroll

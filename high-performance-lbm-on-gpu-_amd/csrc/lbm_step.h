@@ -3,23 +3,9 @@
 #pragma once
 
 #include "lbm_common.h"
+#include "lbm_hold.h"
 
 namespace lbm {
-
-// ------------------------------------------------------------------ load order
-
-// Holds the values of `v` in registers at this point of the program. The masked kernels
-// load a node's kind byte and its populations together, then leave early for a solid
-// node; without this the compiler moves the population loads below that exit (only the
-// fluid path uses them), so every wavefront would wait for its kind bytes before issuing
-// its population loads: a second memory round trip. Placed before the exit, the loads are
-// issued with the kind byte and waited for together. The statement has no instructions.
-template <class T, int N>
-__device__ __forceinline__ void hold_loaded(const T (&v)[N])
-{
-#pragma unroll
-    for (int i = 0; i < N; ++i) asm volatile("" : : "v"(v[i]));
-}
 
 // ------------------------------------------------------------------ indexing
 

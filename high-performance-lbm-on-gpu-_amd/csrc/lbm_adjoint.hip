@@ -256,11 +256,15 @@ __global__ __launch_bounds__(adj_rules_max_threads(L::D, sizeof(T))) void k_adj_
 // A RowBatch holds, for slots t0 + k * nthreads (k < K), the slot, the list bounds and the
 // first E entries of each list. fetch() reads only the transposed tables, which no launch
 // writes, so it can run ahead of the barrier that publishes lam; apply() then costs one
-// memory latency (the old lam values) plus the LDS reads of x. Longer lists finish in a loop.
+// memory latency (the old lam values and the x reads together). Every lane reads all E
+// entries of each list from an index valid either way (entry 0 past a list's end, slot 0
+// past the last slot), and the values are held together, so the reads go out at once
+// rather than one branch and one wait each; entries past a list's end add nothing, the
+// real ones add in list order. Longer lists finish in a loop.
 template <class T, int K, int E = 4>
 struct RowBatch {
     int64_t sl[K];
-    int b[K], end[K];
+    int b[K], end[K], nv[K];
     int rr[K][E];
     T cc[K][E];
 
@@ -271,37 +275,45 @@ struct RowBatch {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int t = t0 + k * nthreads;
-            if (t < n) {
-                sl[k] = slot[t];
-                b[k] = start[t];
-                end[k] = start[t + 1];
-            }
+            const bool in = t < n;
+            sl[k] = slot[in ? t : 0];
+            b[k] = in ? start[t] : 0;
+            end[k] = in ? start[t + 1] : 0;
         }
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (t0 + k * nthreads < n)
+        for (int k = 0; k < K; ++k) {
+            nv[k] = end[k] - b[k] < E ? end[k] - b[k] : E;
 #pragma unroll
-                for (int j = 0; j < E; ++j)
-                    if (b[k] + j < end[k]) {
-                        rr[k][j] = row[b[k] + j];
-                        cc[k][j] = coeff[b[k] + j];
-                    }
+            for (int j = 0; j < E; ++j) {
+                const int e = j < nv[k] ? b[k] + j : 0;
+                rr[k][j] = row[e];
+                cc[k][j] = coeff[e];
+            }
+        }
     }
 
     __device__ __forceinline__ void apply(T* __restrict__ lam, const int* __restrict__ row,
                                           const T* __restrict__ coeff, int n, int t0, int nthreads, const T* x)
     {
-        T old[K];
+        T old[K], xv[K][E];
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (t0 + k * nthreads < n) old[k] = lam[sl[k]];
+        for (int k = 0; k < K; ++k) {
+            old[k] = lam[sl[k]];
+#pragma unroll
+            for (int j = 0; j < E; ++j) xv[k][j] = x[rr[k][j]];
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            hold_value(old[k]);
+            hold_loaded(xv[k]);
+        }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (t0 + k * nthreads < n) {
                 T acc = T(0);
 #pragma unroll
                 for (int j = 0; j < E; ++j)
-                    if (b[k] + j < end[k]) acc += cc[k][j] * x[rr[k][j]];
+                    if (j < nv[k]) acc += cc[k][j] * xv[k][j];
                 for (int e = b[k] + E; e < end[k]; ++e) acc += coeff[e] * x[row[e]];
                 lam[sl[k]] = old[k] + acc;
             }

@@ -71,6 +71,7 @@ __device__ __forceinline__ void add_wall_terms(T (&f)[L::Q], uint32_t bits, cons
     if (wall_u == nullptr) return;
     T w[L::Q];
     static_for<0, L::Q>([&](auto I) { w[I] = wall_term<L, I>(wall_u, neighbour<L, I, S>(c.x, c.y, c.z, g), g.n); });
+    hold_loaded(w);
     static_for<0, L::Q>([&](auto I) {
         if ((bits >> I) & 1u) f[I] = f[I] + w[I];
     });

@@ -194,13 +194,18 @@ __global__ __launch_bounds__(1024) void k_lds(const T* __restrict__ src, T* __re
         f[i] = tile[bounce ? L::opp[i] * cells + centre : i * cells + e];
     });
     if (MASK && blocked && wall_u != nullptr) {  // null: every wall at rest, zero terms
+        // Every link's term from its upstream node (a valid address for any link), held
+        // together so the loads go out at once, then added on the blocked links only.
+        T w[L::Q];
         static_for<0, L::Q>([&](auto I) {
             constexpr int i = I;
             using Di = Dir<L, i>;
-            if ((blocked >> i) & 1u) {
-                const uint32_t up = (uint32_t)wrap(y - Di::y, g.ny) * (uint32_t)g.nx + (uint32_t)wrap(x - Di::x, g.nx);
-                f[i] = f[i] + wall_term<L, i>(wall_u, up, g.n);
-            }
+            const uint32_t up = (uint32_t)wrap(y - Di::y, g.ny) * (uint32_t)g.nx + (uint32_t)wrap(x - Di::x, g.nx);
+            w[i] = wall_term<L, i>(wall_u, up, g.n);
+        });
+        hold_loaded(w);
+        static_for<0, L::Q>([&](auto I) {
+            if ((blocked >> I) & 1u) f[I] = f[I] + w[I];
         });
     }
     T Fx = T(0), Fy = T(0), Fz = T(0);

@@ -144,8 +144,12 @@ __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T*
     if (MASK && (k & KIND_SOLID)) return;
     uint32_t blocked = 0;
     if (MASK && (k & KIND_NEAR_SOLID)) {
+        uint32_t nk[L::Q];  // all neighbour kinds read together, held, then tested
+        nk[0] = 0;
+        static_for<1, L::Q>([&](auto I) { nk[I] = kind[nbr<L, I, -1>(x, y, z, g)]; });
+        hold_loaded(nk);
         static_for<1, L::Q>([&](auto I) {
-            if (kind[nbr<L, I, -1>(x, y, z, g)] & KIND_SOLID) blocked |= 1u << I;
+            if (nk[I] & KIND_SOLID) blocked |= 1u << I;
         });
     }
     if (MASK && blocked) {

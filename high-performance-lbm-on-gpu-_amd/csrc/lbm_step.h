@@ -21,14 +21,19 @@ __device__ __forceinline__ uint32_t neighbour(int x, int y, int z, const Grid& g
     return ((uint32_t)zz * (uint32_t)g.ny + (uint32_t)yy) * (uint32_t)g.nx + (uint32_t)xx;
 }
 
-// Bit i set when the upstream node x - c_i is solid.
+// Bit i set when the upstream node x - c_i is solid. The Q - 1 neighbour kinds are read
+// together and held, then tested (else each read could sit behind its own test and wait).
 template <class L>
 __device__ __forceinline__ uint32_t blocked_upstream(const uint8_t* __restrict__ kind, int x, int y, int z,
                                                      const Grid& g)
 {
+    uint32_t nk[L::Q];
+    nk[0] = 0;
+    static_for<1, L::Q>([&](auto I) { nk[I] = kind[neighbour<L, I, -1>(x, y, z, g)]; });
+    hold_loaded(nk);
     uint32_t m = 0;
     static_for<1, L::Q>([&](auto I) {
-        if (kind[neighbour<L, I, -1>(x, y, z, g)] & KIND_SOLID) m |= 1u << I;
+        if (nk[I] & KIND_SOLID) m |= 1u << I;
     });
     return m;
 }

@@ -36,12 +36,17 @@ __device__ __forceinline__ void affine_terms(const T* __restrict__ F, const int6
             s[k] = slot[e];
         }
     }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (k < count) hold_value(s[k]);  // slot loads stay ahead of the gather branches
     // The gathers go out together from their selected addresses (hold_value: otherwise
-    // each would sit behind a branch on its coefficient with a wait of its own).
+    // each would sit behind a branch on its coefficient with a wait of its own). A term
+    // whose coefficient is zero on every lane of the wavefront (row padding) is not
+    // fetched at all: the branch around it is uniform and its term is zero either way.
     T v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-        if (k < count) v[k] = F[a[k] == T(0) ? 0 : s[k]];
+        if (k < count) v[k] = __any(a[k] != T(0)) ? F[a[k] == T(0) ? 0 : s[k]] : T(0);
 #pragma unroll
     for (int k = 0; k < 8; ++k)
         if (k < count) hold_value(v[k]);

@@ -99,15 +99,47 @@ inline int force_groups(int64_t n)
     return (int)std::min<int64_t>(FORCE_MAX_GROUPS, (n + FORCE_THREADS - 1) / FORCE_THREADS);
 }
 
-// Sum of part[a][0..FORCE_THREADS) into part[a][0] by a fixed tree.
-__device__ __forceinline__ void force_tree(double (&part)[3][FORCE_THREADS])
+constexpr int FORCE_WAVES = FORCE_THREADS / 64;
+
+// Sum over the 64 lanes of a wavefront by a fixed shuffle tree (lane 0 holds the result).
+__device__ __forceinline__ double wave_sum(double v)
 {
-    __syncthreads();
-    for (int width = FORCE_THREADS / 2; width > 0; width /= 2) {
-        if ((int)threadIdx.x < width)
-            for (int a = 0; a < 3; ++a) part[a][threadIdx.x] += part[a][threadIdx.x + width];
-        __syncthreads();
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset);
+    return v;
+}
+
+// Sum of every lane's acc[a] into part[a][0] (lane 0 reads it back) by a fixed tree: each
+// wavefront's shuffle tree (force_wave_sums), a barrier, then the same tree over the
+// wavefronts' sums in the first wavefront (force_waves_total). A kernel may put other work
+// between the halves as long as a workgroup barrier separates them. Call from all
+// FORCE_THREADS lanes.
+__device__ __forceinline__ void force_wave_sums(const double (&acc)[3], double (&part)[3][FORCE_WAVES])
+{
+    double s[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) s[a] = wave_sum(acc[a]);
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 3; ++a) part[a][threadIdx.x >> 6] = s[a];
+}
+
+__device__ __forceinline__ void force_waves_total(double (&part)[3][FORCE_WAVES])
+{
+    const int lane = threadIdx.x & 63;
+    if ((threadIdx.x >> 6) == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double v = wave_sum(lane < FORCE_WAVES ? part[a][lane] : 0.0);
+            if (lane == 0) part[a][0] = v;
+        }
     }
+}
+
+__device__ __forceinline__ void force_tree(const double (&acc)[3], double (&part)[3][FORCE_WAVES])
+{
+    force_wave_sums(acc, part);
+    __syncthreads();
+    force_waves_total(part);
 }
 
 // Force = sum_r weight[r] * value[r] (d components). Lane t of workgroup b takes rows
@@ -122,15 +154,14 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_affine_force(const T* __restr
                                                                 int d, double* __restrict__ out,
                                                                 double* __restrict__ partial)
 {
-    __shared__ double part[3][FORCE_THREADS];
+    __shared__ double part[3][FORCE_WAVES];
     double acc[3] = {0.0, 0.0, 0.0};
     const int64_t stride = (int64_t)gridDim.x * FORCE_THREADS;
     for (int64_t r = (int64_t)blockIdx.x * FORCE_THREADS + threadIdx.x; r < n; r += stride) {
         const double v = (double)affine_row(F, slot, coeff, constant, n, m, r);
         for (int a = 0; a < d; ++a) acc[a] += weight[r * d + a] * v;
     }
-    for (int a = 0; a < 3; ++a) part[a][threadIdx.x] = acc[a];
-    force_tree(part);
+    force_tree(acc, part);
     if (threadIdx.x == 0) {
         if (gridDim.x == 1) {
             for (int a = 0; a < d; ++a) out[a] = part[a][0];
@@ -145,12 +176,11 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_affine_force(const T* __restr
 __global__ __launch_bounds__(FORCE_THREADS) void k_force_total(const double* __restrict__ partial, int groups, int d,
                                                                double* __restrict__ out)
 {
-    __shared__ double part[3][FORCE_THREADS];
+    __shared__ double part[3][FORCE_WAVES];
     double acc[3] = {0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < groups; b += FORCE_THREADS)
         for (int a = 0; a < 3; ++a) acc[a] += partial[3 * b + a];
-    for (int a = 0; a < 3; ++a) part[a][threadIdx.x] = acc[a];
-    force_tree(part);
+    force_tree(acc, part);
     if (threadIdx.x == 0)
         for (int a = 0; a < d; ++a) out[a] = part[a][0];
 }
@@ -168,22 +198,21 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_force_rules(
     double* __restrict__ out, const int64_t* __restrict__ rslot, const T* __restrict__ rcoeff,
     const T* __restrict__ rconstant, const int64_t* __restrict__ rtarget, int64_t nr, int rm)
 {
-    __shared__ double part[3][FORCE_THREADS];
+    static_assert(SMALL_THREADS == FORCE_THREADS, "one lane per force row and per rule");
+    __shared__ double part[3][FORCE_WAVES];
     const int tid = threadIdx.x;
-    if (tid < FORCE_THREADS) {
-        double acc[3] = {0.0, 0.0, 0.0};
-        for (int64_t r = tid; r < nf; r += FORCE_THREADS) {
-            const double v = (double)affine_row(F, fslot, fcoeff, fconstant, nf, fm, r);
-            for (int a = 0; a < d; ++a) acc[a] += weight[r * d + a] * v;
-        }
-        for (int a = 0; a < 3; ++a) part[a][tid] = acc[a];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t r = tid; r < nf; r += FORCE_THREADS) {
+        const double v = (double)affine_row(F, fslot, fcoeff, fconstant, nf, fm, r);
+        for (int a = 0; a < d; ++a) acc[a] += weight[r * d + a] * v;
     }
+    force_wave_sums(acc, part);
     T v[SMALL_PER_THREAD];
     int64_t t[SMALL_PER_THREAD];
     rules_values<T, SMALL_PER_THREAD>(F, rslot, rcoeff, rconstant, rtarget, nr, rm, tid, SMALL_THREADS, v, t);
-    __syncthreads();
+    __syncthreads();  // also the force tree's barrier
     rules_store<T, SMALL_PER_THREAD>(F, nr, tid, SMALL_THREADS, v, t);
-    force_tree(part);
+    force_waves_total(part);
     if (tid == 0)
         for (int a = 0; a < d; ++a) out[a] = part[a][0];
 }

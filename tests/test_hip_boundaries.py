@@ -200,3 +200,49 @@ def test_one_launch_pre_pass_equals_the_two_phase_pass(layout, dtype):
                  pointer(wide.constant), pointer(width), pointer(target), wide.n, wide.m, None)
         torch.cuda.synchronize()
         assert torch.equal(one, two), name
+
+
+@pytest.mark.parametrize(("nf", "fm", "nr", "rm", "d"), [
+    (300, 4, 1364, 6, 2), (1024, 8, 2048, 8, 3), (1, 1, 1, 1, 2), (0, 3, 700, 5, 2), (500, 3, 0, 4, 2),
+    (300, 9, 1364, 6, 2), (1025, 4, 100, 4, 2), (100, 4, 2049, 4, 3), (2048, 12, 4096, 16, 3),
+])
+def test_fused_force_and_pre_pass_equal_the_separate_launches(nf, fm, nr, rm, d):
+    """lbm_affine_force_rules (the force of one step and the rule pre-pass of the next in one
+    workgroup) gives the same bits as lbm_affine_force followed by lbm_rules_apply: padded
+    rows, rows of up to 16 terms, one and two rules per lane, sizes at the workgroup's limits
+    and empty row sets."""
+    import torch
+
+    from lbm.backends._native import library
+
+    rng = np.random.default_rng(nf * 7 + nr)
+    n = 9 * 4096
+    f0 = torch.as_tensor(rng.uniform(0.5, 1.5, n).astype(np.float32), device="cuda")
+
+    def rows(count, m):
+        slot = rng.integers(0, n, (m, max(count, 1))).astype(np.int64)
+        coeff = rng.normal(size=(m, max(count, 1))).astype(np.float32)
+        width = rng.integers(1, m + 1, max(count, 1))
+        pad = np.arange(m)[:, None] >= width[None, :]
+        coeff[pad], slot[pad] = 0.0, 0
+        constant = rng.normal(size=max(count, 1)).astype(np.float32)
+        return [torch.as_tensor(a, device="cuda") for a in (slot, coeff, constant)]
+
+    fs, fc, fk = rows(nf, fm)
+    weight = torch.as_tensor(rng.normal(size=(max(nf, 1), d)), device="cuda")
+    rs, rc, rk = rows(nr, rm)
+    target = torch.as_tensor(rng.permutation(n)[: max(nr, 1)].astype(np.int64), device="cuda")
+    lib = library()
+    p = lambda t: t.data_ptr()  # noqa: E731
+    fused, separate = f0.clone(), f0.clone()
+    out_fused = torch.full((3,), 7.0, dtype=torch.float64, device="cuda")
+    out_separate = torch.full((3,), 7.0, dtype=torch.float64, device="cuda")
+    lib.call("lbm_affine_force_rules", 0, p(fused), p(fs), p(fc), p(fk), p(weight), nf, fm, d, p(out_fused), p(rs),
+             p(rc), p(rk), p(target), nr, rm, None)
+    lib.call("lbm_affine_force", 0, p(separate), p(fs), p(fc), p(fk), p(weight), nf, fm, d, p(out_separate), None, None)
+    lib.call("lbm_rules_apply", 0, p(separate), p(rs), p(rc), p(rk), p(target), nr, rm, None, None)
+    torch.cuda.synchronize()
+    assert torch.equal(fused, separate)
+    assert torch.equal(out_fused[:d], out_separate[:d])
+    if nr:
+        assert not torch.equal(fused, f0)

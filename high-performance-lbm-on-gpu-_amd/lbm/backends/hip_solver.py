@@ -69,8 +69,9 @@ _SHAPE_3D_MASKED = {("aa", "float32"): (256, 4)}
 _D2Q9_LANES, _D2Q9_MASKED_LANES = 1024, 512
 # Masked D2Q9 lattices up to this many nodes keep rows of at most 512 lanes but stack them
 # to 1024 lanes per workgroup: 880 x 164 Bouzidi channel 14.8 -> 15.9 K MLUPS (pull), 512^2
-# channel with a cylinder 20.6 -> 25.7 K; from 1024^2 up 512 x 1 was as fast or faster.
-_D2Q9_SMALL_MASKED_NODES = 2**19
+# channel with a cylinder 20.6 -> 25.7 K; from 640^2 to 896^2 the two were within 3 %
+# either way, and from 1024^2 up 512 x 1 was as fast or faster.
+_D2Q9_SMALL_MASKED_NODES = 2**18
 
 
 def _masked(params) -> bool:
@@ -88,7 +89,7 @@ def default_block(params, variant: str) -> tuple[int, int]:
     1.5-4 % above 256 x 4 on periodic 2048^2 / 4096^2. Lattices with solid or forced nodes
     take rows of up to 512 lanes: Bouzidi/Zou-He channels 4096^2, 8192 x 2048 and 2048^2
     and a 4096^2 lid-driven cavity ran 1-8 % faster at 512 x 1 than at 1024 x 1 (a
-    near-solid lane at each end of every row holds its workgroup longer); up to 2^19 nodes
+    near-solid lane at each end of every row holds its workgroup longer); up to 2^18 nodes
     they stack those rows to 1024 lanes (880 x 164 and 512^2 channels: 7-25 % faster). The LDS-tiled
     kernel keeps bx <= 256: its halo ring costs 2 (bx + 2) + 2 by cells per bx * by, so
     256 x 4 and 128 x 8 win (D2Q9 fp32 4096^2: 76 K MLUPS against 62 K for 64 x 4 and 55 K

@@ -49,7 +49,7 @@ def test_d2q9_rows_and_the_cap_rule():
     assert default_block(walls, "pull") == default_block(walls, "aa") == (512, 1)
     assert default_block(walls, "lds") == (256, 4)
     assert default_block(big, "lds") == (256, 4) and default_groups_per_cu(big, "lds") == 0
-    small = box(512, 2, solid=True)  # masked up to 2^19 nodes: rows of 512 lanes, two per group
+    small = box(512, 2, solid=True)  # masked up to 2^18 nodes: rows of 512 lanes, two per group
     assert default_block(small, "pull") == default_block(small, "aa") == (512, 2)
     channel = SimulationParams(nx=880, ny=164, tau=0.6, solid=np.zeros((164, 880), dtype=bool) | (np.arange(164) == 0)[:, None])
     assert default_block(channel, "pull") == (512, 2)

@@ -426,20 +426,33 @@ class DragAdjoint:
 
 
 def drag_loss(problem: DragAdjoint, body_field, markers):
-    """Differentiable mean drag: a ``torch.autograd.Function`` over (body_field, markers)."""
+    """Differentiable mean drag: a ``torch.autograd.Function`` over (body_field, markers).
+
+    ``body_field`` holds d * n_nodes values (any shape that reshapes to (d, *shape)) and
+    ``markers`` n_markers * d values; both may live on any device and be float32 or float64.
+    The loss is a 0-d float64 tensor on ``body_field``'s device. Its backward pass runs
+    ``problem.backward()`` on the checkpoints this forward pass stored, so it must run
+    before ``problem`` is run forward again. The gradients come back with the shape, dtype
+    and device of the inputs they belong to."""
     import torch
+
+    d = problem.params.d
 
     class DragLoss(torch.autograd.Function):
         @staticmethod
         def forward(ctx, g, x):
+            ctx.meta = ((g.shape, g.dtype, g.device), (x.shape, x.dtype, x.device))
             problem.set_body_force(g.detach())
-            problem.set_markers(x.detach().cpu().numpy())
-            return torch.tensor(problem.forward(), dtype=torch.float64)
+            problem.set_markers(x.detach().cpu().double().numpy().reshape(-1, d))
+            return torch.tensor(problem.forward(), dtype=torch.float64, device=g.device)
 
         @staticmethod
         def backward(ctx, grad_out):
             gb, gx = problem.backward()
-            scale = grad_out.detach().cpu().double()
-            return (torch.as_tensor(gb) * scale).to(body_field.device), (torch.as_tensor(gx) * scale).to(markers.device)
+            scale = float(grad_out.detach().cpu())
+            out = []
+            for grad, (shape, dtype, device) in zip((gb, gx), ctx.meta, strict=True):
+                out.append(torch.as_tensor(grad * scale).reshape(shape).to(dtype=dtype, device=device))
+            return tuple(out)
 
     return DragLoss.apply(body_field, markers)

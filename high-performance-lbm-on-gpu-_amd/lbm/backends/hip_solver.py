@@ -140,6 +140,44 @@ def node_kinds(lattice: VelocitySet, solid: np.ndarray, forced_nodes=None) -> np
     return kinds.reshape(shape)
 
 
+class DeviceArrays:
+    """Array namespace of a HIP solver's state: attribute lookups fall through to ``torch``
+    (``isfinite``, ``argmax``, ``where``, ...), arrays are created on the solver's device,
+    and ``cuda.get_current_stream()`` returns torch's current stream, whose
+    ``synchronize()`` the reference's benchmark calls (benchmarks/bench_mlups.py:99-101)."""
+
+    def __init__(self, torch, device):
+        self._torch = torch
+        self.device = device
+
+        class _Cuda:
+            @staticmethod
+            def get_current_stream():
+                return torch.cuda.current_stream()
+
+            @staticmethod
+            def synchronize():
+                torch.cuda.synchronize()
+
+        self.cuda = _Cuda()
+
+    def asarray(self, a, dtype=None):
+        """A device tensor from a NumPy array, scalar, sequence or tensor (no copy when
+        ``a`` already is a device tensor of that dtype)."""
+        torch = self._torch
+        if dtype is not None and not isinstance(dtype, torch.dtype):
+            dtype = getattr(torch, np.dtype(dtype).name)
+        if not isinstance(a, torch.Tensor):
+            a = np.asarray(a)
+        return torch.as_tensor(a, dtype=dtype, device=self.device)
+
+    def __getattr__(self, name):
+        return getattr(self._torch, name)
+
+    def __repr__(self) -> str:
+        return f"DeviceArrays(torch, device={self.device})"
+
+
 class _AffinePack:
     """A compiled affine set on the device: term-major slots / coefficients, constants."""
 
@@ -184,6 +222,7 @@ class HipSolver(Solver):
         torch = _require_device()
         self._lib = load_library()
         self._torch = torch
+        self._xp = None
         self.device = torch.device("cuda")
         self.variant = variant
         self.step_multiple = 2 if variant == "aa" else 1
@@ -348,6 +387,9 @@ class HipSolver(Solver):
                 self._groups_per_cu, self._stream if stream is None else stream)
 
     def _advance(self, steps: int) -> None:
+        if steps < 0 or steps % self.step_multiple:
+            raise ValueError(f"the {self.variant!r} variant advances in multiples of {self.step_multiple} steps; "
+                             f"got {steps}")
         if self.graphs and steps >= 2:
             pairs, rest = divmod(steps, 2)
             for _ in range(pairs):
@@ -463,11 +505,27 @@ class HipSolver(Solver):
         reason = "non-finite populations" if nonfinite else "non-positive density"
         raise SimulationDiverged(self.time_step, location, reason)
 
+    @property
+    def xp(self) -> "DeviceArrays":
+        """The array namespace of the state (``Solver.xp``, src/lbm/solver.py:74-77): torch
+        on the ROCm device, with the CuPy spellings the reference's callers use
+        (``xp.cuda.get_current_stream().synchronize()``, ``xp.asarray``)."""
+        if self._xp is None:
+            self._xp = DeviceArrays(self._torch, self.device)
+        return self._xp
+
     def device_populations(self):
         """Post-collision populations in natural order as a (q, N) device tensor (a copy for AA)."""
         if self.variant == "aa":
             return self._f[list(self.params.lattice.opposite)]
         return self._f
+
+    def _device_populations(self):
+        """Post-collision populations in natural direction order on the device, shape
+        (q, *shape) (``Solver._device_populations``, src/lbm/solver.py:89-92; for AA a
+        permuted copy, as src/lbm/backends/cuda_solver.py:302-305)."""
+        p = self.params
+        return self.device_populations().reshape(p.lattice.q, *p.shape)
 
     def populations(self) -> np.ndarray:
         p = self.params
@@ -610,5 +668,5 @@ class HipSolver(Solver):
         return out
 
 
-__all__ = ["HipSolver", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block",
+__all__ = ["HipSolver", "DeviceArrays", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block",
            "default_groups_per_cu"]

@@ -259,10 +259,13 @@ class ImmersedForcing:
 
 
 class SimulationDiverged(RuntimeError):
+    """The reference's exception and message (src/lbm/solver.py:39-52)."""
+
     def __init__(self, step, location, reason):
         self.step = step
         self.location = location
-        super().__init__(f"simulation diverged by step {step}: {reason} at node {location}")
+        names = "(y, x)" if len(location) == 2 else "(z, y, x)"
+        super().__init__(f"simulation diverged by step {step}: {reason} at node {names}={location}")
 
 
 class Fields:

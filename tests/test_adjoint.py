@@ -198,6 +198,55 @@ def test_adjoint_matches_central_finite_differences(dtype):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("input_dtype", [torch.float32, torch.float64])
+def test_drag_loss_autograd_function_gives_the_adjoint_gradients(input_dtype):
+    """``drag_loss`` as a torch.autograd.Function on CUDA inputs: the loss equals
+    DragAdjoint.forward, ``.grad`` equals DragAdjoint.backward (scaled by the upstream
+    gradient), with the inputs' shape (a flattened body field included), dtype and device;
+    and one central finite difference of the loss agrees to 1e-4."""
+    from lbm.adjoint import drag_loss
+
+    params = case(dtype="float64")
+    g0 = random_body_field(params)
+    x0 = params.immersed.markers.copy()
+    reference = make_problem(params)
+    reference.set_body_force(g0)
+    ref_loss = reference.forward()
+    ref_g, ref_x = reference.backward()
+
+    problem = make_problem(params)
+    g = torch.as_tensor(g0, dtype=input_dtype, device="cuda").reshape(params.d, -1).requires_grad_(True)
+    x = torch.as_tensor(x0, dtype=input_dtype, device="cuda").requires_grad_(True)
+    loss = drag_loss(problem, g, x)
+    assert loss.dtype == torch.float64 and loss.device.type == "cuda" and loss.dim() == 0
+    (3.0 * loss).backward()
+    assert g.grad.shape == g.shape and g.grad.dtype == input_dtype and g.grad.device.type == "cuda"
+    assert x.grad.shape == x.shape and x.grad.dtype == input_dtype and x.grad.device.type == "cuda"
+    if input_dtype == torch.float64:
+        # the same body force and markers: the same forward and adjoint runs, bit for bit
+        assert float(loss) == ref_loss
+        np.testing.assert_array_equal(g.grad.cpu().numpy(), 3.0 * ref_g.reshape(params.d, -1))
+        np.testing.assert_array_equal(x.grad.cpu().numpy(), 3.0 * ref_x)
+    else:
+        # float32 inputs round g and the markers once on the way in
+        assert float(loss) == pytest.approx(ref_loss, rel=1e-5)
+        np.testing.assert_allclose(g.grad.cpu().numpy(), (3.0 * ref_g.reshape(params.d, -1)).astype(np.float32),
+                                   rtol=1e-4, atol=1e-4 * np.abs(3.0 * ref_g).max())
+        np.testing.assert_allclose(x.grad.cpu().numpy(), 3.0 * ref_x, rtol=1e-3, atol=1e-4 * np.abs(3 * ref_x).max())
+
+    eps = 1e-6
+    direction = random_body_field(params, seed=11, scale=1.0)
+    fd_problem = make_problem(params)
+
+    def loss_at(field):
+        with torch.no_grad():
+            return float(drag_loss(fd_problem, torch.as_tensor(field, device="cuda"), torch.as_tensor(x0)))
+
+    fd = (loss_at(g0 + eps * direction) - loss_at(g0 - eps * direction)) / (2 * eps)
+    assert float(np.sum(ref_g * direction)) == pytest.approx(fd, rel=1e-4)
+
+
+@pytest.mark.gpu
 def test_graph_replay_equals_eager_launches():
     params = case()
     a, b = make_problem(params, graphs=True), make_problem(params, graphs=False)
@@ -313,3 +362,47 @@ def test_full_size_ibm_channel_gradients_match_central_finite_differences():
         xm[l, a] -= h
         fd_x = (loss_at(g0, xp) - loss_at(g0, xm)) / (2 * h)
         assert grad_x[l, a] == pytest.approx(fd_x, rel=1e-4, abs=1e-4 * np.abs(grad_x).max())
+
+
+@pytest.mark.gpu
+def test_config_size_float32_gradients_match_float64_finite_differences():
+    """The differentiable configuration as stated: D2Q9 float32 channel 512 x 128, IBM
+    cylinder R = 12 (3 multi-direct passes), Re 100, 4,000 steps, loss = mean drag over the
+    last 2,000, checkpoints every 100, body force N(0, 1e-7). The float32 adjoint's
+    directional derivative along a random body-force direction, and one marker coordinate,
+    agree with float64 central finite differences of the float64 forward run to 1e-4
+    relative."""
+    from lbm.adjoint import DragAdjoint
+    from lbm.cases.ibm_channel import ibm_channel_params, initial_fields
+
+    def problem(dtype):
+        params = ibm_channel_params(dtype=dtype)
+        return params, DragAdjoint(params, steps=4000, window=2000, checkpoint_every=100,
+                                   init=initial_fields(params))
+
+    params, adj = problem("float32")
+    params64, fd_problem = problem("float64")
+    g0 = np.random.default_rng(0).standard_normal((2, *params.shape)) * 1e-7 * params.fluid
+    adj.set_body_force(g0)
+    loss32 = adj.forward()
+    grad_g, grad_x = adj.backward()
+    assert np.all(np.isfinite(grad_g)) and np.all(np.isfinite(grad_x))
+
+    def loss_at(g, markers):
+        fd_problem.set_body_force(g)
+        fd_problem.set_markers(markers)
+        return fd_problem.forward()
+
+    x0 = params64.immersed.markers.copy()
+    assert loss_at(g0, x0) == pytest.approx(loss32, rel=1e-4)
+    direction = np.random.default_rng(11).standard_normal((2, *params.shape)) * params.fluid
+    eps = 1e-6
+    fd = (loss_at(g0 + eps * direction, x0) - loss_at(g0 - eps * direction, x0)) / (2 * eps)
+    assert float(np.sum(grad_g * direction)) == pytest.approx(fd, rel=1e-4)
+    l, a = 0, 0
+    h = 1e-5
+    xp, xm = x0.copy(), x0.copy()
+    xp[l, a] += h
+    xm[l, a] -= h
+    fd_x = (loss_at(g0, xp) - loss_at(g0, xm)) / (2 * h)
+    assert grad_x[l, a] == pytest.approx(fd_x, rel=1e-4, abs=1e-4 * np.abs(grad_x).max())

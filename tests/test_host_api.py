@@ -211,3 +211,46 @@ def test_hip_solver_reports_the_requested_velocity_and_returns_host_copies(backe
     assert not np.any(s.populations() == 7.0)
     with pytest.raises(ValueError, match="no z velocity"):
         s.initialize(uz=0.1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend", ["hip-pull", "hip-aa"])
+def test_reference_benchmark_pattern_drives_the_hip_solver(backend):
+    """The reference's timing helpers (benchmarks/bench_mlups.py:99-109) use ``solver.xp``
+    and ``solver.run`` only; they run unchanged on the HIP solver, and
+    ``_device_populations`` is the natural-order device state (src/lbm/solver.py:89-92)."""
+    import time
+
+    import torch
+
+    from lbm import make_solver
+    from oracle import ReferenceSolver
+
+    def _synchronize(solver) -> None:
+        if solver.xp is not np:
+            solver.xp.cuda.get_current_stream().synchronize()
+
+    def _timed_run(solver, steps: int) -> float:
+        _synchronize(solver)
+        start = time.perf_counter()
+        solver.run(steps, check_every=steps)
+        _synchronize(solver)
+        return time.perf_counter() - start
+
+    p = SimulationParams(nx=24, ny=20, nz=16, tau=0.8, collision="trt", dtype="float64")
+    s = make_solver(p, backend=backend)
+    rng = np.random.default_rng(0)
+    velocity = [0.01 * rng.standard_normal(p.shape) for _ in range(3)]
+    s.initialize(1.0, *velocity)
+    probe = 2 * s.step_multiple
+    assert _timed_run(s, probe) > 0 and s.time_step == probe
+    assert s.xp is not np and s.xp.asarray(np.arange(3)).device.type == "cuda"
+    assert bool(s.xp.isfinite(s.xp.asarray([1.0, 2.0])).all())
+
+    ref = ReferenceSolver(p)
+    ref.initialize(1.0, *velocity)
+    ref.run(probe)
+    f = s._device_populations()
+    assert isinstance(f, torch.Tensor) and f.is_cuda and tuple(f.shape) == (19, *p.shape)
+    np.testing.assert_array_equal(f.cpu().numpy(), s.populations())
+    np.testing.assert_allclose(f.cpu().numpy(), ref.populations(), rtol=0, atol=1e-15)

@@ -19,6 +19,8 @@
 #include "lbm_affine.h"
 
 #include <algorithm>
+#include <mutex>
+
 #include "lbm_common.h"
 
 namespace lbm {
@@ -217,6 +219,50 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_force_rules(
         for (int a = 0; a < d; ++a) out[a] = part[a][0];
 }
 
+// Rule values of lbm_rules_apply_noalloc: the library's own device buffer, one per GPU
+// (each device holds its own copy of this array until the process ends). Host code never
+// passes the array's name to a kernel: on the host that name is only a stand-in for the
+// symbol, so kernels get the device address that hipGetSymbolAddress returns for the
+// current device (noalloc_scratch_ptr).
+constexpr int64_t NOALLOC_MAX_RULES = int64_t(1) << 20;
+static __device__ __align__(256) double rules_noalloc_scratch[NOALLOC_MAX_RULES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+
+// Device address of rules_noalloc_scratch on the current device, looked up on the first
+// call on that device and kept. hipGetSymbolAddress allocates nothing.
+inline void* noalloc_scratch_ptr()
+{
+    static std::mutex lock;
+    static void* address[NOALLOC_MAX_DEVICES] = {};
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    require(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device index out of range");
+    std::lock_guard<std::mutex> hold(lock);
+    if (address[dev] == nullptr) {
+        void* ptr = nullptr;
+        hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(rules_noalloc_scratch)), "hipGetSymbolAddress(scratch)");
+        require(ptr != nullptr, "hipGetSymbolAddress returned no address", LBM_ERR_HIP);
+        address[dev] = ptr;
+    }
+    return address[dev];
+}
+
+// The launches of lbm_rules_apply on a given scratch buffer (used when n_rules > 4096).
+template <class T>
+void rules_apply_launch(T* f, const int64_t* slot, const T* coeff, const T* constant, const int64_t* target,
+                        int64_t n_rules, int m, T* scratch, hipStream_t s)
+{
+    if (n_rules <= (int64_t)SMALL_THREADS * SMALL_PER_THREAD) {
+        k_rules_small<T><<<1, SMALL_THREADS, 0, s>>>(f, slot, coeff, constant, target, n_rules, m);
+    } else {
+        require(scratch != nullptr, "more than 4096 rules need a scratch buffer of n_rules values");
+        const unsigned blocks = (unsigned)((n_rules + 255) / 256);
+        k_affine_values<T><<<blocks, 256, 0, s>>>(f, slot, coeff, constant, n_rules, m, scratch);
+        k_rules_write<T><<<blocks, 256, 0, s>>>(f, target, scratch, n_rules);
+    }
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -232,20 +278,32 @@ LBM_API int lbm_rules_apply(int dtype, void* f, const int64_t* slot, const void*
         require(f && slot && coeff && constant && target, "null pointer");
         dispatch_real(dtype, [&](auto Tt) {
             using T = decltype(Tt);
-            const hipStream_t s = as_stream(stream);
-            if (n_rules <= (int64_t)SMALL_THREADS * SMALL_PER_THREAD) {
-                k_rules_small<T><<<1, SMALL_THREADS, 0, s>>>(static_cast<T*>(f), slot, static_cast<const T*>(coeff),
-                                                             static_cast<const T*>(constant), target, n_rules, m);
-            } else {
-                require(scratch != nullptr, "more than 4096 rules need a scratch buffer of n_rules values");
-                const unsigned blocks = (unsigned)((n_rules + 255) / 256);
-                k_affine_values<T><<<blocks, 256, 0, s>>>(static_cast<const T*>(f), slot, static_cast<const T*>(coeff),
-                                                          static_cast<const T*>(constant), n_rules, m,
-                                                          static_cast<T*>(scratch));
-                k_rules_write<T><<<blocks, 256, 0, s>>>(static_cast<T*>(f), target, static_cast<const T*>(scratch),
-                                                        n_rules);
-            }
+            rules_apply_launch<T>(static_cast<T*>(f), slot, static_cast<const T*>(coeff),
+                                  static_cast<const T*>(constant), target, n_rules, m, static_cast<T*>(scratch),
+                                  as_stream(stream));
             check_launch("rule pre-pass");
+        });
+    });
+}
+
+LBM_API int lbm_rules_apply_noalloc(int dtype, void* f, const int64_t* slot, const void* coeff, const void* constant,
+                                    const int64_t* target, int64_t n_rules, int m, void* stream)
+{
+    return guarded("lbm_rules_apply_noalloc", [&] {
+        require(n_rules >= 0 && m >= 1 && m <= 128, "rules need 1 <= m <= 128 terms");
+        require(n_rules <= NOALLOC_MAX_RULES,
+                "at most 1048576 rules: the library's scratch holds 2^20 rule values; use lbm_rules_apply with a "
+                "caller scratch buffer for larger sets");
+        require(dtype == LBM_F32 || dtype == LBM_F64, "unknown dtype code");
+        if (n_rules == 0) return;
+        require(f && slot && coeff && constant && target, "null pointer");
+        void* scratch = n_rules > (int64_t)SMALL_THREADS * SMALL_PER_THREAD ? noalloc_scratch_ptr() : nullptr;
+        dispatch_real(dtype, [&](auto Tt) {
+            using T = decltype(Tt);
+            rules_apply_launch<T>(static_cast<T*>(f), slot, static_cast<const T*>(coeff),
+                                  static_cast<const T*>(constant), target, n_rules, m, static_cast<T*>(scratch),
+                                  as_stream(stream));
+            check_launch("rule pre-pass (library scratch)");
         });
     });
 }

@@ -33,6 +33,7 @@ SIGNATURES: dict[str, list] = {
     "lbm_health": [_i, _i, _p, _p, _i64, _i, _p, _p],
     "lbm_rules_apply": [_i, _p, _p, _p, _p, _p, _i64, _i, _p, _p],
     "lbm_rules_apply_direct": [_i, _p, _p, _p, _p, _p, _p, _i64, _i, _p],
+    "lbm_rules_apply_noalloc": [_i, _p, _p, _p, _p, _p, _i64, _i, _p],
     "lbm_affine_values": [_i, _p, _p, _p, _p, _i64, _i, _p, _p],
     "lbm_affine_force": [_i, _p, _p, _p, _p, _p, _i64, _i, _i, _p, _p, _p],
     "lbm_affine_force_scratch_size": [_i64, POINTER(c_int64)],

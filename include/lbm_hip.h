@@ -133,6 +133,17 @@ LBM_API int lbm_rules_apply(int dtype, void* f, const int64_t* slot, const void*
                             const void* constant, const int64_t* target, int64_t n_rules, int m,
                             void* scratch, void* stream);
 
+/* lbm_rules_apply without a scratch argument and without allocating: above 4096 rules the
+ * rule values go through the library's own device buffer (one per GPU, 2^20 values = 8 MiB,
+ * held until the process ends) instead of a caller buffer. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). At most 2^20 rules (LBM_ERR_INVALID above, before
+ * any GPU work). Every call on one GPU shares that buffer: calls that overlap on one GPU
+ * (other streams, other host threads) race. Same bits as lbm_rules_apply. */
+LBM_API int lbm_rules_apply_noalloc(int dtype, void* f, const int64_t* slot, const void* coeff,
+                                    const void* constant, const int64_t* target, int64_t n_rules, int m,
+                                    void* stream);
+
 /* lbm_rules_apply for a concatenation of rule sets (no counterpart in the reference): row r
  * sums only its first width[r] padded terms (its own set's width; m is the padded width of
  * the arrays), so every row rounds as when its set is applied alone. Always two launches

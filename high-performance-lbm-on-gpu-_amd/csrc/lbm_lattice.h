@@ -69,5 +69,8 @@ struct Dir {
 constexpr uint8_t KIND_SOLID = 1;
 constexpr uint8_t KIND_FORCED = 2;
 constexpr uint8_t KIND_NEAR_SOLID = 4;
+// A fluid node with link rules that lbm_pull_step_links updates in its rule layer (the
+// host sets it on exactly the nodes of the rule-node list it passes; other kernels ignore it).
+constexpr uint8_t KIND_RULES = 8;
 
 }  // namespace lbm

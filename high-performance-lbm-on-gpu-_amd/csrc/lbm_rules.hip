@@ -17,6 +17,7 @@
 // launch of one rule per lane gives the same bits at any size.
 
 #include "lbm_affine.h"
+#include "lbm_force.h"
 
 #include <algorithm>
 #include <mutex>
@@ -86,67 +87,8 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_rules_small(T* __restrict__ F
     rules_in_one_group<T, SMALL_PER_THREAD>(F, slot, coeff, constant, target, n, m, threadIdx.x, SMALL_THREADS);
 }
 
-constexpr int FORCE_THREADS = 1024;
-constexpr int64_t FORCE_ONE_GROUP_ROWS = 2048;
-constexpr int FORCE_MAX_GROUPS = 1024;
-
-// Workgroups of the force reduction (FORCE_THREADS lanes each, so a probe of up to 1024
-// rows costs one row per lane): one up to FORCE_ONE_GROUP_ROWS rows (the result is
-// written directly), else one per FORCE_THREADS rows up to FORCE_MAX_GROUPS, whose
-// partial sums a second one-workgroup pass adds up. Depends on n only, so the summation
-// order of a given row set never changes.
-inline int force_groups(int64_t n)
-{
-    if (n <= FORCE_ONE_GROUP_ROWS) return 1;
-    return (int)std::min<int64_t>(FORCE_MAX_GROUPS, (n + FORCE_THREADS - 1) / FORCE_THREADS);
-}
-
-constexpr int FORCE_WAVES = FORCE_THREADS / 64;
-
-// Sum over the 64 lanes of a wavefront by a fixed shuffle tree (lane 0 holds the result).
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_down(v, offset);
-    return v;
-}
-
-// Sum of every lane's acc[a] into part[a][0] (lane 0 reads it back) by a fixed tree: each
-// wavefront's shuffle tree (force_wave_sums), a barrier, then the same tree over the
-// wavefronts' sums in the first wavefront (force_waves_total). A kernel may put other work
-// between the halves as long as a workgroup barrier separates them. Call from all
-// FORCE_THREADS lanes.
-__device__ __forceinline__ void force_wave_sums(const double (&acc)[3], double (&part)[3][FORCE_WAVES])
-{
-    double s[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) s[a] = wave_sum(acc[a]);
-    if ((threadIdx.x & 63) == 0)
-        for (int a = 0; a < 3; ++a) part[a][threadIdx.x >> 6] = s[a];
-}
-
-__device__ __forceinline__ void force_waves_total(double (&part)[3][FORCE_WAVES])
-{
-    const int lane = threadIdx.x & 63;
-    if ((threadIdx.x >> 6) == 0) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const double v = wave_sum(lane < FORCE_WAVES ? part[a][lane] : 0.0);
-            if (lane == 0) part[a][0] = v;
-        }
-    }
-}
-
-__device__ __forceinline__ void force_tree(const double (&acc)[3], double (&part)[3][FORCE_WAVES])
-{
-    force_wave_sums(acc, part);
-    __syncthreads();
-    force_waves_total(part);
-}
-
-// Force = sum_r weight[r] * value[r] (d components). Lane t of workgroup b takes rows
-// b * FORCE_THREADS + t + k * G * FORCE_THREADS (G workgroups) in increasing k, then a
-// fixed tree; with one workgroup the sum goes to out, else to partial[b][3].
+// Force = sum_r weight[r] * value[r] (d components), the groups of the reduction as
+// workgroups; with one group the sum goes to out, else to partial[b][3].
 template <class T>
 __global__ __launch_bounds__(FORCE_THREADS) void k_affine_force(const T* __restrict__ F,
                                                                 const int64_t* __restrict__ slot,
@@ -158,11 +100,8 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_affine_force(const T* __restr
 {
     __shared__ double part[3][FORCE_WAVES];
     double acc[3] = {0.0, 0.0, 0.0};
-    const int64_t stride = (int64_t)gridDim.x * FORCE_THREADS;
-    for (int64_t r = (int64_t)blockIdx.x * FORCE_THREADS + threadIdx.x; r < n; r += stride) {
-        const double v = (double)affine_row(F, slot, coeff, constant, n, m, r);
-        for (int a = 0; a < d; ++a) acc[a] += weight[r * d + a] * v;
-    }
+    const ForceRows<T> fr{slot, coeff, constant, weight, n, m, d};
+    force_lane(F, fr, (int)gridDim.x, (int)blockIdx.x, (int)threadIdx.x, acc);
     force_tree(acc, part);
     if (threadIdx.x == 0) {
         if (gridDim.x == 1) {
@@ -173,18 +112,22 @@ __global__ __launch_bounds__(FORCE_THREADS) void k_affine_force(const T* __restr
     }
 }
 
-// out[a] = sum over the groups' partial sums, lane-strided in increasing group order,
-// then the same fixed tree.
+// out[a] = sum over the groups' partial sums.
 __global__ __launch_bounds__(FORCE_THREADS) void k_force_total(const double* __restrict__ partial, int groups, int d,
                                                                double* __restrict__ out)
 {
     __shared__ double part[3][FORCE_WAVES];
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < groups; b += FORCE_THREADS)
-        for (int a = 0; a < 3; ++a) acc[a] += partial[3 * b + a];
-    force_tree(acc, part);
-    if (threadIdx.x == 0)
-        for (int a = 0; a < d; ++a) out[a] = part[a][0];
+    force_partials_total(partial, groups, d, out, part);
+}
+
+// The totals of a history of partial sums: row s of out (d values) from partial row s
+// (groups x 3 values), one workgroup per row.
+__global__ __launch_bounds__(FORCE_THREADS) void k_force_totals(const double* __restrict__ partial, int groups, int d,
+                                                                double* __restrict__ out)
+{
+    __shared__ double part[3][FORCE_WAVES];
+    const size_t row = blockIdx.x;
+    force_partials_total(partial + row * 3 * (size_t)groups, groups, d, out + row * (size_t)d, part);
 }
 
 // The force of one step and the link-rule pre-pass of the next, in one launch of one
@@ -204,10 +147,7 @@ __global__ __launch_bounds__(SMALL_THREADS) void k_force_rules(
     __shared__ double part[3][FORCE_WAVES];
     const int tid = threadIdx.x;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int64_t r = tid; r < nf; r += FORCE_THREADS) {
-        const double v = (double)affine_row(F, fslot, fcoeff, fconstant, nf, fm, r);
-        for (int a = 0; a < d; ++a) acc[a] += weight[r * d + a] * v;
-    }
+    force_lane(F, ForceRows<T>{fslot, fcoeff, fconstant, weight, nf, fm, d}, 1, 0, tid, acc);
     force_wave_sums(acc, part);
     T v[SMALL_PER_THREAD];
     int64_t t[SMALL_PER_THREAD];
@@ -386,6 +326,27 @@ LBM_API int lbm_affine_force_rules(int dtype, void* f, const int64_t* force_slot
                 static_cast<const T*>(rule_coeff), static_cast<const T*>(rule_constant), rule_target, n_rules, rule_m);
             check_launch("fused force and pre-pass");
         });
+    });
+}
+
+LBM_API int lbm_force_totals(const double* partial, int64_t rows, int groups, int d, double* out, void* stream)
+{
+    return guarded("lbm_force_totals", [&] {
+        require(rows >= 0 && rows < (1ll << 31), "row count out of range");
+        require(groups >= 1 && groups <= FORCE_MAX_GROUPS, "1 <= groups <= 1024");
+        require(d >= 1 && d <= 3, "d must be 1, 2 or 3");
+        if (rows == 0) return;
+        require(partial && out, "null pointer");
+        k_force_totals<<<(unsigned)rows, FORCE_THREADS, 0, as_stream(stream)>>>(partial, groups, d, out);
+        check_launch("force totals");
+    });
+}
+
+LBM_API int lbm_affine_force_groups(int64_t n_rows, int* groups)
+{
+    return guarded("lbm_affine_force_groups", [&] {
+        require(n_rows >= 0 && groups != nullptr, "bad arguments");
+        *groups = force_groups(n_rows);
     });
 }
 

@@ -12,6 +12,7 @@
 // (152 B for D3Q19 float32) in every variant; the kind byte adds 1 B.
 
 #include "lbm_common.h"
+#include "lbm_force.h"
 #include "lbm_step.h"
 
 #include <cstring>
@@ -30,6 +31,152 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_pull(cons
     Coord c;
     if (!locate(c, g)) return;
     pull_update<L, T, FM, MASK>(src, dst, kind, wall_u, field, body, g, r, c);
+}
+
+// ------------------------------------------------------------------ pull step with link rules
+
+// The link rules of the two-lattice step, evaluated inside the step. A rule of link (x, j)
+// (x - c_j solid) gives f_j(x, t + 1) as an affine row of the stored state; the pre-pass form
+// writes it into the source slot (opp j, x) that the bounce-back of direction j reads. The
+// pull step never writes its source buffer, so rows evaluated during the step read exactly
+// the pre-rule populations the pre-pass reads, and the step can use the values without the
+// pre-pass launch. The nodes with rows (sorted) are described by node_info[k] = {node
+// index, directions i whose upstream node x - c_i is solid, directions j with a row, 0};
+// node k owns rows k per .. k per + per - 1, its rows in direction order, then padding
+// rows (never read), so a chunk's rows follow from its nodes without a lookup. The nodes
+// are updated in chunks of `chunk` nodes (rule_chunk_nodes).
+template <class T>
+struct RuleRows {
+    const uint4* node_info;
+    int count;
+    int per;
+    int chunk;
+    const int64_t* slot;
+    const T* coeff;
+    const T* constant;
+    int64_t n;
+    int m;
+};
+
+// Rule nodes per chunk: at most one wavefront of node lanes, and few enough that the chunks
+// cover the layer's workgroups once where they can (at least 16 nodes each), so the nodes'
+// scattered loads and stores spread over many compute units instead of queueing in a few
+// (880 x 164 Bouzidi channel, 492 rule nodes on 164 workgroups: 16-node chunks).
+constexpr int RULE_CHUNK_MIN = 16;
+constexpr int RULE_CHUNK_MAX = 64;
+
+inline int rule_chunk_nodes(int count, int hosts)
+{
+    int chunk = RULE_CHUNK_MIN;
+    while (chunk < RULE_CHUNK_MAX && (int64_t)chunk * hosts < count) chunk *= 2;
+    return chunk;
+}
+
+__host__ __device__ constexpr int rule_chunk_count(int count, int chunk) { return (count + chunk - 1) / chunk; }
+
+
+// One chunk of rule nodes by one workgroup of `lanes` threads (tid the linear index): the
+// workgroup evaluates the chunk's rows into the LDS array `vals` (row first + e by thread
+// e mod lanes; padding rows too, which cost a load of zero terms), threads below rr.chunk
+// update node c rr.chunk + tid as pull_update does,
+// their rule directions taking the rows' values after the barrier. With the blocked set in
+// the node's record, each population is loaded once from its selected slot (pulled or
+// bounce-back), in flight with the rows' loads, all before the barrier. Same operations in
+// the same order as pull_update after a pre-pass: the same bits.
+template <class L, class T, ForceMode FM>
+__device__ __forceinline__ void rule_chunk(const T* __restrict__ src, T* __restrict__ dst,
+                                           const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
+                                           const T* __restrict__ field, const T* __restrict__ body, const Grid& g,
+                                           const Relaxation<T>& r, const RuleRows<T>& rr, int c, int lanes, int tid,
+                                           T* __restrict__ vals)
+{
+    const int first_node = c * rr.chunk;
+    const int end_node = min(first_node + rr.chunk, rr.count);
+    const bool node_lane = first_node + tid < end_node;
+    uint4 info = make_uint4(0u, 0u, 0u, 0u);
+    if (node_lane) info = rr.node_info[first_node + tid];
+    const int64_t first_row = (int64_t)first_node * rr.per;
+    const int64_t end_row = (int64_t)end_node * rr.per;
+    Coord cc{0, 0, 0};
+    const uint32_t node = info.x;
+    const uint32_t blocked = info.y;
+    uint8_t k = 0;
+    T f[L::Q];
+    static_for<0, L::Q>([&](auto I) { f[I] = T(0); });
+    if (node_lane) {
+        cc.x = (int)(node % (uint32_t)g.nx);
+        const uint32_t rest = node / (uint32_t)g.nx;
+        cc.y = (int)(rest % (uint32_t)g.ny);
+        cc.z = (int)(rest / (uint32_t)g.ny);
+        k = kind[node];
+        static_for<0, L::Q>([&](auto I) {
+            constexpr int i = I;
+            const bool bounce = (blocked >> i) & 1u;
+            f[i] = src[bounce ? (size_t)L::opp[i] * g.ns + node
+                              : (size_t)i * g.ns + neighbour<L, i, -1>(cc.x, cc.y, cc.z, g)];
+        });
+    }
+    for (int64_t e = tid; first_row + e < end_row; e += lanes)
+        vals[e] = affine_row(src, rr.slot, rr.coeff, rr.constant, rr.n, rr.m, first_row + e);
+    __syncthreads();
+    if (node_lane) {
+        uint32_t has = info.z;
+        int local = tid * rr.per;
+        while (has) {
+            const int j = __ffs(has) - 1;
+            has &= has - 1u;
+            const T v = vals[local++];
+            // select into every slot (a store at a runtime index would move f to scratch)
+            static_for<1, L::Q>([&](auto I) { f[I] = (j == int(I)) ? v : f[I]; });
+        }
+        add_wall_terms<L, -1>(f, blocked, wall_u, cc, g);
+        T Fx = T(0), Fy = T(0), Fz = T(0);
+        if constexpr (FM != ForceMode::UNIFORM) node_force<L, FM>(r, k, field, body, node, g.n, Fx, Fy, Fz);
+        collide<L, FM>(f, r, Fx, Fy, Fz);
+        static_for<0, L::Q>([&](auto I) { dst[(size_t)I * g.ns + node] = f[I]; });
+    }
+    __syncthreads();  // vals is reused by the next chunk
+}
+
+// Two-lattice pull step with its link rules and, optionally, the momentum-exchange force of
+// the state it reads. The grid's z layers 0..nz-1 update every node except solid and rule
+// nodes (pull_update); the extra layer z = nz hosts, in turn, the rule chunks (rule_chunk)
+// and then the force probe's reduction groups (force_group_hosted, the groups and sums of
+// lbm_affine_force). Nothing in the launch writes src, so the force rows read the state
+// exactly as a separate force launch before the step would. The dynamic LDS holds the rule
+// values of a chunk (lanes values) or the force tree (3 x FORCE_WAVES doubles).
+template <class L, class T, ForceMode FM>
+__global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_pull_links(
+    const T* __restrict__ src, T* __restrict__ dst, const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
+    const T* __restrict__ field, const T* __restrict__ body, Grid g, Relaxation<T> r, RuleRows<T> rules,
+    ForceRows<T> fr, int force_groups, double* __restrict__ force_out, double* __restrict__ force_partial, int gx,
+    int gy)
+{
+    // A one-dimensional grid: the first `work` workgroups take the rule chunks and the force
+    // groups (so they start first), the rest the (gx, gy, nz) tiles of the lattice.
+    const int chunks = rules.count ? rule_chunk_count(rules.count, rules.chunk) : 0;
+    const int work = chunks + force_groups;
+    const int b = (int)blockIdx.x;
+    if (b < work) {
+        extern __shared__ double dynamic_lds[];
+        const int lanes = blockDim.x * blockDim.y;
+        const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+        if (b < chunks) {
+            rule_chunk<L, T, FM>(src, dst, kind, wall_u, field, body, g, r, rules, b, lanes, tid,
+                                 reinterpret_cast<T*>(dynamic_lds));
+        } else {
+            auto& part = *reinterpret_cast<double(*)[3][FORCE_WAVES]>(dynamic_lds);
+            force_group_hosted(src, fr, force_groups, b - chunks, lanes, tid, part, force_out, force_partial);
+        }
+        return;
+    }
+    const int tile = b - work;
+    Coord c;
+    c.x = (tile % gx) * (int)blockDim.x + (int)threadIdx.x;
+    c.y = ((tile / gx) % gy) * (int)blockDim.y + (int)threadIdx.y;
+    c.z = tile / (gx * gy);
+    if (c.x >= g.nx || c.y >= g.ny) return;
+    pull_update<L, T, FM, true, true>(src, dst, kind, wall_u, field, body, g, r, c);
 }
 
 // AA even half-step: node-local, slot reversal.
@@ -389,6 +536,82 @@ int run_step(const char* name, Variant v, int lattice, int dtype, const void* a,
     });
 }
 
+// The pull step with in-step link rules and the force layer (k_pull_links).
+int run_pull_links(int lattice, int dtype, const void* src, void* dst, const uint8_t* kind, const void* wall,
+                   const void* field, const void* body, int nx, int ny, int nz, double wp, double wm, double fx,
+                   double fy, double fz, int force_mode, const uint32_t* rule_node_info, int rule_nodes,
+                   int rules_per_node, const int64_t* rule_slot,
+                   const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m,
+                   const int64_t* force_slot, const void* force_coeff, const void* force_constant,
+                   const double* force_weight, int64_t n_force, int force_m, double* force_out,
+                   double* force_partial, int bx, int by, int groups_per_cu, void* stream)
+{
+    return guarded("lbm_pull_step_links", [&] {
+        const Grid g = make_grid(lattice, nx, ny, nz);
+        const dim3 block = block_of(bx, by, step_max_threads_of(lattice, dtype));
+        require(groups_per_cu >= 0, "groups_per_cu must be >= 0 (0: no occupancy cap)");
+        require(src && dst && kind, "null population buffer or kind array");
+        require(force_mode >= 0 && force_mode <= 2, "unknown force mode");
+        require(force_mode != 2 || body != nullptr, "dense force mode needs a body field");
+        require(force_mode != 1 || field != nullptr, "field force mode needs a force field");
+        require(n_rules >= 0 && n_rules < (1ll << 31), "rule count out of range");
+        require(rule_nodes >= 0 && (n_rules > 0) == (rule_nodes > 0), "rows need rule nodes and nodes rows");
+        require(n_rules == 0 || (rules_per_node >= 1 && rules_per_node <= 18 &&
+                                 n_rules == (int64_t)rule_nodes * rules_per_node),
+                "n_rules must be rule_nodes * rules_per_node, 1 <= rules_per_node <= 18");
+        require(n_rules == 0 || (rule_node_info && rule_slot && rule_coeff && rule_constant && rule_m >= 1 &&
+                                 rule_m <= 128),
+                "link rules need the node list, slot, coeff, constant and 1 <= m <= 128");
+        require(reinterpret_cast<uintptr_t>(rule_node_info) % 16 == 0, "rule_node_info must be 16-byte aligned");
+        require(n_force >= 0, "force row count out of range");
+        const int lanes = bx * by;
+        require(lanes >= RULE_CHUNK_MAX || rule_nodes == 0, "the rule layer needs at least 64 threads per workgroup");
+        const int groups = n_force > 0 ? force_groups(n_force) : 0;
+        if (groups) {
+            require(force_slot && force_coeff && force_constant && force_weight && force_m >= 1 && force_m <= 128,
+                    "force rows need slot, coeff, constant, weight and 1 <= m <= 128");
+            require(lanes % 64 == 0 && FORCE_THREADS % lanes == 0,
+                    "the force layer needs 64, 128, 256, 512 or 1024 threads per workgroup");
+            require(groups == 1 ? force_out != nullptr : force_partial != nullptr,
+                    "force rows need out (up to 2048 rows) or partial (above)");
+        }
+        const double force[3] = {fx, fy, fz};
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            const Relaxation<T> r = make_relaxation<L, T>(wp, wm, force);
+            const dim3 tiles = grid_of(g, block);
+            const int64_t n_tiles = (int64_t)tiles.x * tiles.y * tiles.z;
+            const int chunk = rule_chunk_nodes(rule_nodes, (int)std::min<int64_t>(n_tiles, 1 << 20));
+            const RuleRows<T> rr{reinterpret_cast<const uint4*>(rule_node_info), rule_nodes, rules_per_node, chunk,
+                                 rule_slot, static_cast<const T*>(rule_coeff), static_cast<const T*>(rule_constant),
+                                 n_rules, rule_m};
+            const ForceRows<T> fw{force_slot, static_cast<const T*>(force_coeff),
+                                  static_cast<const T*>(force_constant), force_weight, n_force, force_m, L::D};
+            const int64_t work = (rule_nodes ? rule_chunk_count(rule_nodes, chunk) : 0) + groups;
+            require(n_tiles + work < (1ll << 31), "grid too large for a one-dimensional launch");
+            const dim3 grid((unsigned)(n_tiles + work));
+            size_t needed = 0;
+            if (rule_nodes) needed = std::max(needed, sizeof(T) * (size_t)chunk * rules_per_node);
+            if (groups) needed = std::max(needed, sizeof(double) * 3 * FORCE_WAVES);
+            const size_t lds = occupancy_lds(groups_per_cu, needed);
+            const hipStream_t s = as_stream(stream);
+            auto go = [&](auto fm) {
+                constexpr ForceMode FM = decltype(fm)::value;
+                allow_lds(k_pull_links<L, T, FM>, lds);
+                k_pull_links<L, T, FM><<<grid, block, lds, s>>>(
+                    static_cast<const T*>(src), static_cast<T*>(dst), kind, static_cast<const T*>(wall),
+                    static_cast<const T*>(field), static_cast<const T*>(body), g, r, rr, fw, groups, force_out,
+                    force_partial, (int)tiles.x, (int)tiles.y);
+            };
+            if (force_mode == 0) go(std::integral_constant<ForceMode, ForceMode::UNIFORM>{});
+            else if (force_mode == 1) go(std::integral_constant<ForceMode, ForceMode::FIELD>{});
+            else go(std::integral_constant<ForceMode, ForceMode::DENSE>{});
+            check_launch("pull step with link rules");
+        });
+    });
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -442,6 +665,23 @@ LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst
     return run_step("lbm_pull_step", Variant::PULL, lattice, dtype, f_src, f_dst, kind, wall_u, force_field,
                     body_field, nx, ny, nz, omega_plus, omega_minus, fx, fy, fz, force_mode, mask_mode, bx, by,
                     groups_per_cu, stream);
+}
+
+LBM_API int lbm_pull_step_links(int lattice, int dtype, const void* f_src, void* f_dst, const uint8_t* kind,
+                                const void* wall_u, const void* force_field, const void* body_field, int nx, int ny,
+                                int nz, double omega_plus, double omega_minus, double fx, double fy, double fz,
+                                int force_mode, const uint32_t* rule_node_info, int rule_nodes,
+                                int rules_per_node, const int64_t* rule_slot,
+                                const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m,
+                                const int64_t* force_slot, const void* force_coeff, const void* force_constant,
+                                const double* force_weight, int64_t n_force, int force_m, double* force_out,
+                                double* force_partial, int bx, int by, int groups_per_cu, void* stream)
+{
+    return run_pull_links(lattice, dtype, f_src, f_dst, kind, wall_u, force_field, body_field, nx, ny, nz,
+                          omega_plus, omega_minus, fx, fy, fz, force_mode, rule_node_info, rule_nodes,
+                          rules_per_node, rule_slot, rule_coeff,
+                          rule_constant, n_rules, rule_m, force_slot, force_coeff, force_constant, force_weight,
+                          n_force, force_m, force_out, force_partial, bx, by, groups_per_cu, stream);
 }
 
 LBM_API int lbm_aa_even(int lattice, int dtype, void* f, const uint8_t* kind, const void* force_field,
@@ -563,6 +803,8 @@ LBM_API int lbm_kernel_resources(int lattice, int dtype, int* values, int max_ke
             };
             add("pull", reinterpret_cast<const void*>(&k_pull<L, T, ForceMode::UNIFORM, true>));
             add("pull_all_fluid", reinterpret_cast<const void*>(&k_pull<L, T, ForceMode::UNIFORM, false>));
+            add("pull_links", reinterpret_cast<const void*>(&k_pull_links<L, T, ForceMode::UNIFORM>));
+            add("pull_links_field", reinterpret_cast<const void*>(&k_pull_links<L, T, ForceMode::FIELD>));
             add("aa_even", reinterpret_cast<const void*>(&k_aa_even<L, T, ForceMode::UNIFORM, true>));
             add("aa_odd", reinterpret_cast<const void*>(&k_aa_odd<L, T, ForceMode::UNIFORM, true>));
             add("aa_even_all_fluid", reinterpret_cast<const void*>(&k_aa_even<L, T, ForceMode::UNIFORM, false>));

@@ -196,6 +196,44 @@ class _AffinePack:
         self.source = compiled
 
 
+KIND_RULES = 8
+
+
+def link_rule_index(lattice: VelocitySet, target: np.ndarray, solid: np.ndarray):
+    """The rule-node list of a pull-layout rule set for lbm_pull_step_links: the rows sorted
+    by (node, direction j), where row r writes slot target[r] = opp(j) n + node (n nodes of
+    the grid ``solid``). Returns (order, node_info, node_first): order sorts the rows;
+    node_info (count, 4) uint32 = {node, directions i with a solid upstream node x - c_i
+    (periodic), directions j with a row, first row} for the nodes with rows, ascending;
+    node_first = the first rows and the row count."""
+    shape = solid.shape
+    n_nodes = int(np.prod(shape))
+    target = np.asarray(target, dtype=np.int64)
+    if target.size >= 2**31:
+        raise ValueError("at most 2^31 - 1 link rules")
+    node = target % n_nodes
+    direction = np.asarray(lattice.opposite, dtype=np.int64)[target // n_nodes]
+    order = np.lexsort((direction, node))
+    node, direction = node[order], direction[order]
+    if node.size and np.any((node[1:] == node[:-1]) & (direction[1:] == direction[:-1])):
+        raise ValueError("two link rules write the same slot")
+    starts = np.flatnonzero(np.r_[True, node[1:] != node[:-1]]) if node.size else np.zeros(0, dtype=np.int64)
+    bits = np.left_shift(np.uint32(1), direction.astype(np.uint32))
+    masks = np.bitwise_or.reduceat(bits, starts) if node.size else np.zeros(0, dtype=np.uint32)
+    nodes = node[starts]
+    flat_solid = np.asarray(solid).reshape(-1)
+    blocked = np.zeros(nodes.size, dtype=np.uint32)
+    for i, c in enumerate(lattice.c):
+        if any(c):
+            upstream = neighbour_index(nodes, [-a for a in c], shape)
+            blocked |= flat_solid[upstream].astype(np.uint32) << np.uint32(i)
+    if np.any(masks & ~blocked):
+        raise ValueError("a link rule sits on a direction whose upstream node is not solid")
+    node_info = np.stack([nodes.astype(np.uint32), blocked, masks.astype(np.uint32), starts.astype(np.uint32)], axis=1)
+    node_first = np.r_[starts, node.size].astype(np.int32)
+    return order, np.ascontiguousarray(node_info), node_first
+
+
 class HipSolver(Solver):
     """Fused stream-bounce-collide on MI355X.
 
@@ -273,6 +311,32 @@ class HipSolver(Solver):
                 pack.scratch = torch.empty(max(1, pack.n), dtype=self._torch_dtype, device=self.device)
                 pack.direct = not reads_other_targets(compiled.target, compiled.values.slots, compiled.values.coeff)
                 self._rule_passes[layout] = pack
+        # The pull step evaluates its link rules in the consuming lanes (lbm_pull_step_links):
+        # the rows sorted by (node, direction) and a per-node index, built once.
+        self._links = None
+        if variant == "pull" and "pull" in self._rule_passes:
+            compiled = compile_rules(p.lattice, p.link_rules, "pull", p.solid, p.wall_velocity)
+            order, node_info, node_first = link_rule_index(p.lattice, compiled.target, p.solid)
+            # node k owns rows k * per ... k * per + per - 1: its rows, then zero padding
+            counts = np.diff(node_first)
+            per = int(counts.max())
+            at = np.repeat(np.arange(counts.size) * per, counts) + (np.arange(order.size) - np.repeat(node_first[:-1],
+                                                                                                       counts))
+            v = compiled.values
+            slots = np.zeros((counts.size * per, np.asarray(v.slots).shape[1]), dtype=np.int64)
+            coeff = np.zeros((counts.size * per, np.asarray(v.coeff).shape[1]))
+            constant = np.zeros(counts.size * per)
+            slots[at], coeff[at], constant[at] = (np.asarray(v.slots)[order], np.asarray(v.coeff)[order],
+                                                  np.asarray(v.constant)[order])
+            links = _AffinePack(torch, CompiledAffine(slots=slots, coeff=coeff, constant=constant), self._np_dtype,
+                                self.device)
+            links.node_info = torch.as_tensor(node_info.view(np.int32), device=self.device)
+            links.nodes = counts.size
+            links.per = per
+            self._links = links
+            flat = kinds.reshape(-1)
+            flat[node_info[:, 0].astype(np.int64)] |= KIND_RULES
+            self._kind = torch.as_tensor(flat, device=self.device)
         self._affine_cache: dict[int, _AffinePack] = {}
         self._f = torch.empty((p.lattice.q, self._n), dtype=self._torch_dtype, device=self.device)
         self._f_next = None if variant == "aa" else torch.empty_like(self._f)
@@ -443,10 +507,38 @@ class HipSolver(Solver):
                     + [("lbm_aa_odd", (L, D, f, kind, wall, field, body, *args))]
                     + self._pre_pass_calls("aa_even", f, stream)
                     + [("lbm_aa_even", (L, D, f, kind, field, body, *args))]]
-        name = "lbm_lds_step" if self.variant == "lds" else "lbm_pull_step"
         a, b = pointer(self._f), pointer(self._f_next)
+        if self._links is not None and self.block[0] * self.block[1] >= 64:
+            return [self._ibm_calls("pull", src, stream) + [self._pull_links_call(src, dst, stream)]
+                    for src, dst in ((a, b), (b, a))]
+        name = "lbm_lds_step" if self.variant == "lds" else "lbm_pull_step"
         return [self._pre_pass_calls("pull", src, stream) + [(name, (L, D, src, dst, kind, wall, field, body, *args))]
                 for src, dst in ((a, b), (b, a))]
+
+    def _pull_links_call(self, src: int, dst: int, stream, force=None, out=None, partial=None) -> tuple:
+        """The pull step from ``src`` to ``dst`` with the link rules evaluated in the step and,
+        when ``force`` (a force pack) is given, the force of the state at ``src`` into
+        ``out`` (one reduction group) or ``partial`` (several)."""
+        p = self.params
+        wp, wm = self._rates
+        fx, fy, fz = self._force3
+        bx, by = self.block
+        lk = self._links
+        if lk is not None:
+            rules = (pointer(lk.node_info), lk.nodes, lk.per, pointer(lk.slot), pointer(lk.coeff),
+                     pointer(lk.constant), lk.n, lk.m)
+        else:
+            rules = (None, 0, 1, None, None, None, 0, 1)
+        if force is None:
+            rows = (None, None, None, None, 0, 1, None, None)
+        else:
+            rows = (pointer(force.slot), pointer(force.coeff), pointer(force.constant), pointer(force.weight),
+                    force.n, force.m, out, partial)
+        return ("lbm_pull_step_links", (
+            self._lattice_code, self._dtype_code, src, dst, pointer(self._kind), self._wall_ptr, pointer(self._field),
+            pointer(self._body), p.nx, p.ny, self._nz, wp, wm, fx, fy, fz, self._force_mode, *rules, *rows, bx, by,
+            self._groups_per_cu, stream,
+        ))
 
     def _pre_pass(self, layout: str) -> None:
         """The IBM force field, then the link rules, for the layout the next kernel reads."""
@@ -457,6 +549,14 @@ class HipSolver(Solver):
         the link rules. The IBM moments read the state before any rule overwrote a target
         slot, as the NumPy step does (it evaluates the IBM on the stored state,
         src/lbm/solver.py:263-271)."""
+        calls = self._ibm_calls(layout, f, stream)
+        pack = self._rule_passes.get(layout)
+        if pack is not None:
+            calls.append(self._rules_call(pack, f, stream))
+        return calls
+
+    def _ibm_calls(self, layout: str, f: int, stream) -> list[tuple]:
+        """The IBM force-field launch on the state at address ``f`` (none without a body)."""
         calls = []
         if self._ibm is not None:
             ibm = self._ibm
@@ -469,9 +569,6 @@ class HipSolver(Solver):
                 ibm.iterations, fx, fy, fz, self._n, pointer(self._field), pointer(self._body_force),
                 pointer(self._ibm_scratch), None, stream,
             )))
-        pack = self._rule_passes.get(layout)
-        if pack is not None:
-            calls.append(self._rules_call(pack, f, stream))
         return calls
 
     def _rules_call(self, pack: _AffinePack, f: int, stream) -> tuple:
@@ -594,6 +691,9 @@ class HipSolver(Solver):
         out = torch.zeros((steps // m, d), dtype=torch.float64, device=self.device)
         pack = self._force_pack(probe)
         stream = self._stream
+        lanes = self.block[0] * self.block[1]
+        if self.variant == "pull" and self._mask_mode and lanes % 64 == 0 and 1024 % lanes == 0:
+            return self._record_forces_in_step(pack, steps, out)
         sequences = self._step_calls(stream)
         # the state after a step: the AA buffer, or the destination of that parity's step
         states = [pointer(self._f)] if self.variant == "aa" else [pointer(self._f_next), pointer(self._f)]
@@ -622,6 +722,45 @@ class HipSolver(Solver):
             lib.call_all([self._force_call(pack, states[(count - 1) % len(sequences)], base + (count - 1) * row,
                                            stream)])
         if self.variant != "aa" and steps % 2:
+            self._f, self._f_next = self._f_next, self._f
+        self.time_step += steps
+        self._check_health()
+        return out.cpu().numpy()
+
+    def _record_forces_in_step(self, pack: _AffinePack, steps: int, out) -> np.ndarray:
+        """``record_forces`` for the pull variant: the force of the state a step reads is
+        reduced by the extra workgroup layer of that step's own launch
+        (lbm_pull_step_links), so a step with its force is one launch (plus the IBM
+        pre-pass when there is a body). Step k records the force after step k - 1; the
+        force after the last step is one lbm_affine_force launch. Above 2048 rows the
+        layer writes each step's partial sums, added up per row by one lbm_force_totals
+        launch at the end. Every row is summed in lbm_affine_force's order, bit for bit."""
+        import ctypes
+
+        torch, lib, stream = self._torch, self._lib, self._stream
+        d = self.params.d
+        groups = ctypes.c_int()
+        self._call("lbm_affine_force_groups", pack.n, ctypes.byref(groups))
+        groups = groups.value
+        partial = (torch.empty((steps - 1, groups, 3), dtype=torch.float64, device=self.device)
+                   if groups > 1 and steps > 1 else None)
+        a, b = pointer(self._f), pointer(self._f_next)
+        base, row = out.data_ptr(), 8 * d
+        prow = 8 * 3 * groups
+        for k in range(steps):
+            src, dst = (a, b) if k % 2 == 0 else (b, a)
+            if k == 0:
+                step = self._pull_links_call(src, dst, stream)
+            elif partial is None:
+                step = self._pull_links_call(src, dst, stream, pack, base + (k - 1) * row, None)
+            else:
+                step = self._pull_links_call(src, dst, stream, pack, None, partial.data_ptr() + (k - 1) * prow)
+            lib.call_all(self._ibm_calls("pull", src, stream) + [step])
+        final = b if steps % 2 else a
+        lib.call_all([self._force_call(pack, final, base + (steps - 1) * row, stream)])
+        if partial is not None:
+            self._call("lbm_force_totals", partial.data_ptr(), steps - 1, groups, d, base, stream)
+        if steps % 2:
             self._f, self._f_next = self._f_next, self._f
         self.time_step += steps
         self._check_health()
@@ -668,5 +807,5 @@ class HipSolver(Solver):
         return out
 
 
-__all__ = ["HipSolver", "DeviceArrays", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block",
+__all__ = ["HipSolver", "DeviceArrays", "KIND_RULES", "link_rule_index", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block",
            "default_groups_per_cu"]

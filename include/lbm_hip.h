@@ -77,6 +77,39 @@ LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst
                           double omega_minus, double fx, double fy, double fz, int force_mode,
                           int mask_mode, int bx, int by, int groups_per_cu, void* stream);
 
+/* lbm_pull_step (mask_mode 1) with the link rules evaluated inside the step and, optionally,
+ * the momentum-exchange force of the state it reads, in one launch.
+ * Link rules: rule rows in the pull layout (as for lbm_rules_apply: term-major slot/coeff,
+ * row stride n_rules, m = rule_m terms, constant), where the row of link (node, j) gives
+ * f_j(node, t + 1). The rule_nodes nodes with rows, sorted, are described by
+ * rule_node_info[4 k .. 4 k + 3] = {node, bit set of the directions i whose upstream node
+ * node - c_i is solid, bit set of the directions j with a row, 0} (16-byte aligned); node k
+ * owns rows k * rules_per_node ... + rules_per_node - 1: its rows in direction order, then
+ * padding rows (zero coefficients; evaluated, never used), so n_rules = rule_nodes *
+ * rules_per_node. kind must carry KIND_RULES (8) on exactly the listed nodes: those are
+ * updated, 16 to 64 nodes per workgroup, by an extra layer of workgroups that evaluate the
+ * nodes' rows (one row per thread, the values shared through LDS; bx * by >= 64), every other
+ * non-solid node by the ordinary grid. f_src is never written during the step, so the rows
+ * read the populations a pre-pass reads and the result equals lbm_rules_apply +
+ * lbm_pull_step bit for bit, without the pre-pass launch. n_rules = 0 (rule_nodes = 0): no
+ * rules (the rule arrays may be NULL).
+ * Force: when n_force > 0, the extra layer also reduces the force rows on f_src exactly as
+ * lbm_affine_force does (same groups, same sums; bx * by must be 64, 128, 256, 512 or 1024)
+ * into force_out[d] (up to 2048 rows) or force_partial (3 doubles per group of
+ * lbm_affine_force_groups; lbm_force_totals adds them).
+ * Replaces _RulePass.__call__ followed by stream_collide_naive
+ * (src/lbm/backends/cuda_solver.py:157-176, 279-284, stream_collide_naive.cu:15-49) and,
+ * for per-step drag, ForceProbe.evaluate (src/lbm/forces.py:98-103). */
+LBM_API int lbm_pull_step_links(int lattice, int dtype, const void* f_src, void* f_dst, const uint8_t* kind,
+                                const void* wall_u, const void* force_field, const void* body_field, int nx, int ny,
+                                int nz, double omega_plus, double omega_minus, double fx, double fy, double fz,
+                                int force_mode, const uint32_t* rule_node_info, int rule_nodes,
+                                int rules_per_node, const int64_t* rule_slot,
+                                const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m,
+                                const int64_t* force_slot, const void* force_coeff, const void* force_constant,
+                                const double* force_weight, int64_t n_force, int force_m, double* force_out,
+                                double* force_partial, int bx, int by, int groups_per_cu, void* stream);
+
 /* AA pattern, even half-step: read F[i](x), collide, write F[opp i](x).
  * Replaces aa_even (src/lbm/backends/kernels/stream_collide_aa.cu:33-57). */
 LBM_API int lbm_aa_even(int lattice, int dtype, void* f, const uint8_t* kind,
@@ -194,6 +227,16 @@ LBM_API int lbm_affine_force_rules(int dtype, void* f, const int64_t* force_slot
                                    int force_m, int d, double* out, const int64_t* rule_slot,
                                    const void* rule_coeff, const void* rule_constant,
                                    const int64_t* rule_target, int64_t n_rules, int rule_m, void* stream);
+
+/* Number of reduction groups of lbm_affine_force for n_rows rows (1 up to 2048 rows, then
+ * one per 1024 rows up to 1024). No counterpart in the reference. Host only. */
+LBM_API int lbm_affine_force_groups(int64_t n_rows, int* groups);
+
+/* Totals of a history of force partial sums: out[s][a] (a < d) from partial[s][groups][3],
+ * s < rows, added in lbm_affine_force's order (its second pass), so each row equals what
+ * lbm_affine_force returns for the same state. No counterpart in the reference (per-step
+ * drag histories on the device). */
+LBM_API int lbm_force_totals(const double* partial, int64_t rows, int groups, int d, double* out, void* stream);
 
 /* Bytes of device scratch lbm_affine_force needs for n_rows rows (0 up to 2048). No
  * counterpart in the reference (its force sum is a host matmul). Host only. */

@@ -243,3 +243,40 @@ def test_inconsistent_velocity_sets_are_rejected():
         VelocitySet("X", 2, ((0, 0), (1,)), (0.5, 0.5))
     with pytest.raises(ValueError, match="dimension"):
         default_lattice(4)
+
+
+def test_link_rule_index_orders_rules_by_node_and_direction():
+    """The rule-node list of the in-step rule evaluation: rows sorted by (node, direction),
+    one entry per node with rows, node | direction bits << 32 and the first row, for a rule
+    set given in scrambled order."""
+    from lbm.backends.hip_solver import link_rule_index
+
+    solid = np.ones((5, 10), dtype=bool)  # n = 50 nodes; every upstream node is solid
+    n = solid.size
+    opp = np.asarray(D2Q9.opposite)
+    links = [(7, 1), (7, 5), (7, 8), (3, 2), (40, 6), (3, 4)]  # (node, direction j)
+    target = np.array([opp[j] * n + x for x, j in links], dtype=np.int64)
+    order, info, first = link_rule_index(D2Q9, target[::-1], solid)
+    assert [links[::-1][r] for r in order] == sorted(links)
+    every = sum(1 << i for i in range(1, 9))
+    assert info.tolist() == [[3, every, (1 << 2) | (1 << 4), 0], [7, every, (1 << 1) | (1 << 5) | (1 << 8), 2],
+                             [40, every, 1 << 6, 5]]
+    assert first.tolist() == [0, 2, 5, 6]
+    with pytest.raises(ValueError, match="same slot"):
+        link_rule_index(D2Q9, np.r_[target, target[:1]], solid)
+
+
+def test_link_rule_index_records_the_solid_upstream_directions():
+    """The blocked set of a rule node: bit i when x - c_i is solid, with periodic wrap."""
+    from lbm.backends.hip_solver import link_rule_index
+
+    solid = np.zeros((6, 8), dtype=bool)
+    solid[:, 0] = True  # the first column: upstream of direction 1 (c = (1, 0)) at x = 1
+    node = 2 * 8 + 1  # (y, x) = (2, 1)
+    opp = np.asarray(D2Q9.opposite)
+    _, info, _ = link_rule_index(D2Q9, np.array([opp[1] * solid.size + node]), solid)
+    blocked = [i for i, c in enumerate(D2Q9.c) if any(c) and solid[(2 - c[1]) % 6, (1 - c[0]) % 8]]
+    assert blocked == [1, 5, 8]
+    assert info[0].tolist() == [node, sum(1 << i for i in blocked), 1 << 1, 0]
+    with pytest.raises(ValueError, match="not solid"):
+        link_rule_index(D2Q9, np.array([opp[2] * solid.size + node]), solid)

@@ -246,3 +246,66 @@ def test_fused_force_and_pre_pass_equal_the_separate_launches(nf, fm, nr, rm, d)
     assert torch.equal(out_fused[:d], out_separate[:d])
     if nr:
         assert not torch.equal(fused, f0)
+
+
+def old_pull_steps(solver, steps):
+    """The pre-pass form of the two-lattice step: lbm_rules_apply(_direct) on the source,
+    then lbm_pull_step (the launches before the rules moved into the step)."""
+    from lbm.backends._native import pointer
+
+    L, D = solver._lattice_code, solver._dtype_code
+    for _ in range(steps):
+        src, dst = pointer(solver._f), pointer(solver._f_next)
+        calls = solver._pre_pass_calls("pull", src, solver._stream)
+        calls.append(("lbm_pull_step", (L, D, src, dst, pointer(solver._kind), solver._wall_ptr,
+                                        pointer(solver._field), pointer(solver._body), *solver._step_args())))
+        solver._lib.call_all(calls)
+        solver._f, solver._f_next = solver._f_next, solver._f
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("d", [2, 3])
+def test_rules_evaluated_in_the_step_equal_the_pre_pass_form(d, dtype):
+    """lbm_pull_step_links (each rule row evaluated by the lane that consumes it) against
+    the rule pre-pass followed by lbm_pull_step: the same bits after 25 steps, for Bouzidi
+    bodies with moving walls and for Zou-He inlet/outlet channels."""
+    from dataclasses import replace
+
+    for params in (bouzidi_case(d, dtype)[0], replace(open_case(d)[0], dtype=dtype)):
+        a, b = make_solver(params, backend="hip-pull"), make_solver(params, backend="hip-pull")
+        assert a._links is not None and a._links.n == a._links.nodes * a._links.per >= len(params.link_rules)
+        for s in (a, b):
+            s.initialize(*random_state(params, 4))
+        a.run(25)
+        old_pull_steps(b, 25)
+        np.testing.assert_array_equal(a.populations(), b.populations())
+
+
+@pytest.mark.parametrize("steps", [6, 7])
+def test_step_layer_force_history_of_a_large_probe_equals_the_separate_launches(steps):
+    """More than 2048 probe rows: the step's force layer writes partial sums per group,
+    lbm_force_totals adds them; every row equals lbm_affine_force on the same state."""
+    import torch
+
+    from lbm.geometry import sphere, voxelize, wall_links
+    from lbm.lattice import D3Q19
+    from lbm.link_rules import bouzidi_rules
+
+    shape = (28, 28, 36)
+    body = sphere(17.3, 13.6, 14.2, 7.4)
+    solid = voxelize(body, shape)
+    links = wall_links(D3Q19, body, solid)
+    params = SimulationParams(nx=36, ny=28, nz=28, tau=0.7, collision="trt", solid=solid, dtype="float32",
+                              link_rules=bouzidi_rules(D3Q19, links, solid), force=(1e-5, 0.0, 0.0))
+    probe = ForceProbe.build(params, links)
+    assert len(links) > 2048
+    fast, slow = make_solver(params, backend="hip-pull"), make_solver(params, backend="hip-pull")
+    for s in (fast, slow):
+        s.initialize(*random_state(params, 1))
+    recorded = fast.record_forces(probe, steps)
+    expected = torch.zeros((steps, 3), dtype=torch.float64, device="cuda")
+    for k in range(steps):
+        slow.run(1, check_every=1)
+        slow.force_on_device(probe, expected[k])
+    assert np.array_equal(recorded, expected.cpu().numpy())
+    assert np.array_equal(fast.populations(), slow.populations())

@@ -98,6 +98,66 @@ __device__ __forceinline__ T affine_row(const T* __restrict__ F, const int64_t* 
     return c + affine_sum(F, slot, coeff, stride, r, m);
 }
 
+// affine_row for rows of at most M terms (M a multiple of 8): all the row's coefficients,
+// slots and its constant are fetched at once, then all its populations, so a row costs two
+// memory latencies whatever m is. The sum is affine_row's: a running sum from 0 for m < 8,
+// else 8 interleaved partial sums over the whole 8-term chunks, their fixed tree, then the
+// remaining terms one by one; the same bits.
+template <class T, int M>
+__device__ __forceinline__ T affine_row_wide(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                             const T* __restrict__ coeff, const T* __restrict__ constant,
+                                             int64_t stride, int m, int64_t r)
+{
+    static_assert(M % 8 == 0 && M >= 8, "M is a multiple of 8");
+    T a[M];
+    int64_t s[M];
+    const T c = constant[r];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        if (k < m) {
+            const int64_t e = (int64_t)k * stride + r;
+            a[k] = coeff[e];
+            s[k] = slot[e];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (k < m) hold_value(s[k]);
+    T v[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (k < m) v[k] = F[a[k] == T(0) ? 0 : s[k]];
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+        if (k < m) hold_value(v[k]);
+    T t[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) t[k] = (k < m && a[k] != T(0)) ? a[k] * v[k] : T(0);
+    T sum = T(0);
+    if (m < 8) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < m) sum = sum + t[k];
+        return c + sum;
+    }
+    const int lim = m - (m % 8);
+    T acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = t[k];
+#pragma unroll
+    for (int p = 8; p < M; p += 8) {
+        if (p < lim) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) acc[k] = acc[k] + t[p + k];
+        }
+    }
+    sum = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+#pragma unroll
+    for (int p = 8; p < M; ++p)
+        if (p >= lim && p < m) sum = sum + t[p];
+    return c + sum;
+}
+
 // One row's coefficients, slots and constant fetched ahead (rows of at most M terms, 8 <= M
 // <= 16), so a kernel can issue those loads before other independent work and gather
 // from F afterwards; affine_finish then sums in affine_row's order (8 interleaved partial

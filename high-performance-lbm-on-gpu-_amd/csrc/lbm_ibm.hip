@@ -110,6 +110,237 @@ __global__ __launch_bounds__(ibm_max_threads(D)) void k_ibm(const T* __restrict_
     }
 }
 
+// ------------------------------------------------------------------ many workgroups
+//
+// The same forcing in 2 + 2 x iterations launches of many workgroups, for bodies whose
+// intermediates do not fit one workgroup's LDS (3D spheres): the launch boundaries take
+// the place of the one-workgroup kernel's barriers. Every value is computed by the same
+// expression in the same order as k_ibm computes it (see lbm_ibm_core.h), so the two give
+// the same bits; only which lane computes it changes. Intermediates live in `scratch` in
+// the non-recording layout of ibm_forward_layout (one iteration slot, force updated in
+// place), followed by the d running marker totals; u holds the velocity of the next
+// iteration.
+
+// One wavefront per workgroup: the work items' gathers are scattered, so spreading them
+// over as many compute units as possible shortens each launch (3D sphere, 491 markers,
+// about 2,500 stencil nodes: 256-lane groups left the spread on 10 compute units).
+constexpr int IBM_MULTI_THREADS = 64;
+constexpr int IBM_NODES_PER_GROUP = 16;   // k_ibm_nodes: (D + 1) x 16 moment rows per group
+constexpr int IBM_TOTAL_CHUNK = 1024;     // markers staged in LDS per pass of the total (16 per lane)
+constexpr int IBM_TOTAL_BATCH = 32;       // staged values read ahead of their sequential adds
+
+// P0 and P1: the D + 1 moment rows of 16 stencil nodes, one row per lane (rows of one
+// moment for consecutive nodes are consecutive, so the coefficient loads coalesce), then,
+// after the group barrier, per node rho, u*, force = 0 and the velocity of the first
+// iteration u = u* + force / (2 rho).
+template <class T, int D>
+__global__ __launch_bounds__(IBM_MULTI_THREADS) void k_ibm_nodes(const T* __restrict__ F,
+                                                                 const int64_t* __restrict__ slot,
+                                                                 const T* __restrict__ coeff,
+                                                                 const T* __restrict__ constant, int m, int nn, T hfx,
+                                                                 T hfy, T hfz, IbmForward<T> s)
+{
+    static_assert((D + 1) * IBM_NODES_PER_GROUP <= IBM_MULTI_THREADS, "one row per lane");
+    __shared__ T mom[D + 1][IBM_NODES_PER_GROUP];
+    const int tid = threadIdx.x;
+    const int n0 = blockIdx.x * IBM_NODES_PER_GROUP;
+    const int moment = tid / IBM_NODES_PER_GROUP;
+    const int n = n0 + tid % IBM_NODES_PER_GROUP;
+    const int rows = (D + 1) * nn;
+    if (moment <= D && n < nn) {
+        const int64_t r = (int64_t)moment * nn + n;
+        const T v = m <= 24 ? affine_row_wide<T, 24>(F, slot, coeff, constant, rows, m, r)
+                            : affine_row(F, slot, coeff, constant, rows, m, r);
+        s.moments[r] = v;
+        mom[moment][tid % IBM_NODES_PER_GROUP] = v;
+    }
+    __syncthreads();
+    const int node = n0 + tid;
+    if (tid < IBM_NODES_PER_GROUP && node < nn) {
+        const T half_force[3] = {hfx, hfy, hfz};
+        const T rho = T(1) + mom[0][tid];
+        s.rho[node] = rho;
+        const T two_rho = T(2) * rho;
+        for (int a = 0; a < D; ++a) {
+            const T ustar = (mom[1 + a][tid] + half_force[a]) / rho;
+            s.ustar[(size_t)a * nn + node] = ustar;
+            s.force[(size_t)a * nn + node] = T(0);
+            s.u[(size_t)a * nn + node] = ustar + T(0) / two_rho;
+        }
+    }
+}
+
+// P2 (first iteration only) and P4 for one marker per lane: rho_l = pairwise sum of
+// w rho over the stencil, U_l = sequential sum of w u, F_l = 2 rho_l (0 - U_l) dA_l. The
+// stencil's gathers of all velocity components go out together.
+template <class T, int D>
+__global__ __launch_bounds__(IBM_MULTI_THREADS) void k_ibm_markers(IbmForward<T> s, const int* __restrict__ inverse,
+                                                                   const T* __restrict__ weights,
+                                                                   const T* __restrict__ areas, int nn, int nm,
+                                                                   int first)
+{
+    constexpr int S = D == 2 ? 9 : 27;
+    const int l = blockIdx.x * IBM_MULTI_THREADS + threadIdx.x;
+    if (l >= nm) return;
+    int inv[S];
+    T w[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        inv[p] = inverse[(size_t)l * S + p];
+        w[p] = weights[(size_t)l * S + p];
+    }
+    hold_loaded(inv);
+    T rho_m;
+    if (first) {
+        T rv[S];
+#pragma unroll
+        for (int p = 0; p < S; ++p) rv[p] = s.rho[inv[p]];
+        hold_loaded(rv);
+        rho_m = pairwise_block<T>(S, [&](int p) { return w[p] * rv[p]; });
+        s.rho_m[l] = rho_m;
+    } else {
+        rho_m = s.rho_m[l];
+    }
+    const T two_rho_m = T(2) * rho_m;
+    const T area = areas[l];
+    T v[D][S];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int p = 0; p < S; ++p) v[a][p] = s.u[(size_t)a * nn + inv[p]];
+#pragma unroll
+    for (int a = 0; a < D; ++a) hold_loaded(v[a]);
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        T um = w[0] * v[a][0];
+#pragma unroll
+        for (int p = 1; p < S; ++p) um = um + w[p] * v[a][p];
+        s.U[(size_t)a * nm + l] = um;
+        s.Fm[(size_t)a * nm + l] = (two_rho_m * (T(0) - um)) * area;
+    }
+}
+
+// P5 for one stencil node per lane: force += the float64 sum of w_e F_l(e) over the node's
+// entries in increasing flat (marker, stencil) order (fetched eight entries at a time), then
+// the velocity of the next iteration, or after the last iteration the field scatter. The
+// extra last workgroup adds this iteration's marker forces to the totals, sequentially over
+// the markers staged through LDS, and after the last iteration writes body_force = -total.
+template <class T, int D>
+__global__ __launch_bounds__(IBM_MULTI_THREADS) void k_ibm_spread(IbmForward<T> s, IbmTables<T> tb,
+                                                                  const int64_t* __restrict__ nodes, int nn, int nm,
+                                                                  int first, int last, uint32_t n_total,
+                                                                  T* __restrict__ field, T* __restrict__ total,
+                                                                  T* __restrict__ body_force)
+{
+    constexpr int S = D == 2 ? 9 : 27;
+    const int tid = threadIdx.x;
+    const int node_groups = (nn + IBM_MULTI_THREADS - 1) / IBM_MULTI_THREADS;
+    if ((int)blockIdx.x == node_groups) {
+        // Staged one component row at a time, so the staging indices need no division; the
+        // sequential sum reads its terms from LDS 32 at a time (whole batches carry no bounds
+        // tests, so their reads go out together) and adds them one by one.
+        __shared__ T stage[D][IBM_TOTAL_CHUNK];
+        T sum = T(0);
+        for (int c0 = 0; c0 < nm; c0 += IBM_TOTAL_CHUNK) {
+            const int count = min(IBM_TOTAL_CHUNK, nm - c0);
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                constexpr int AHEAD = IBM_TOTAL_CHUNK / IBM_MULTI_THREADS;
+                T v[AHEAD];
+#pragma unroll
+                for (int k = 0; k < AHEAD; ++k) {
+                    const int i = k * IBM_MULTI_THREADS + tid;
+                    if (i < count) v[k] = s.Fm[(size_t)a * nm + c0 + i];
+                }
+#pragma unroll
+                for (int k = 0; k < AHEAD; ++k) {
+                    const int i = k * IBM_MULTI_THREADS + tid;
+                    if (i < count) stage[a][i] = v[k];
+                }
+            }
+            __syncthreads();
+            if (tid < D) {
+                const T* row = stage[tid];
+                int p = 0;
+                if (c0 == 0) {  // the sum starts from the first marker's force
+                    sum = row[0];
+                    p = 1;
+                }
+                for (; p + IBM_TOTAL_BATCH <= count; p += IBM_TOTAL_BATCH) {
+                    T v[IBM_TOTAL_BATCH];
+#pragma unroll
+                    for (int k = 0; k < IBM_TOTAL_BATCH; ++k) v[k] = row[p + k];
+#pragma unroll
+                    for (int k = 0; k < IBM_TOTAL_BATCH; ++k) sum = sum + v[k];
+                }
+                for (; p < count; ++p) sum = sum + row[p];
+            }
+            __syncthreads();
+        }
+        if (tid < D) {
+            const T t = (first ? T(0) : total[tid]) + sum;
+            total[tid] = t;
+            if (last) body_force[tid] = -t;
+        }
+        return;
+    }
+    const int n = blockIdx.x * IBM_MULTI_THREADS + tid;
+    if (n >= nn) return;
+    const int begin = tb.spread_start[n], end = tb.spread_start[n + 1];
+    double acc[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) acc[a] = 0.0;
+    for (int e0 = begin; e0 < end; e0 += 8) {
+        int k[8];
+        T w[8], f[8][D];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (e0 + j < end) k[j] = tb.spread_entry[e0 + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            if (e0 + j < end) {
+                w[j] = tb.weights[k[j]];
+#pragma unroll
+                for (int a = 0; a < D; ++a) f[j][a] = s.Fm[(size_t)a * nm + k[j] / S];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (e0 + j < end)
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc[a] += (double)(w[j] * f[j][a]);
+    }
+    const T two_rho = T(2) * s.rho[n];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const T fn = T((double)s.force[(size_t)a * nn + n] + acc[a]);
+        s.force[(size_t)a * nn + n] = fn;
+        if (last) field[(size_t)a * n_total + nodes[n]] = fn;
+        else s.u[(size_t)a * nn + n] = s.ustar[(size_t)a * nn + n] + fn / two_rho;
+    }
+}
+
+// The launches of the many-workgroup forcing (forward only, no record).
+template <class T, int D>
+void ibm_force_multi(const T* f, const int64_t* slot, const T* coeff, const T* constant, int m, const int64_t* nodes,
+                     int nn, const int* inverse, const T* weights, const T* areas, int nm, const int* start,
+                     const int* entry, int iterations, const T (&hf)[3], uint32_t n_total, T* field, T* body_force,
+                     T* scratch, hipStream_t st)
+{
+    IbmForward<T> s;
+    T* total = scratch + ibm_forward_layout(scratch, D, nn, nm, 1, &s);
+    const IbmTables<T> tb{inverse, weights, areas, start, entry};
+    const unsigned node_groups = (unsigned)((nn + IBM_MULTI_THREADS - 1) / IBM_MULTI_THREADS);
+    k_ibm_nodes<T, D><<<(nn + IBM_NODES_PER_GROUP - 1) / IBM_NODES_PER_GROUP, IBM_MULTI_THREADS, 0, st>>>(
+        f, slot, coeff, constant, m, nn, hf[0], hf[1], hf[2], s);
+    for (int it = 0; it < iterations; ++it) {
+        k_ibm_markers<T, D><<<(nm + IBM_MULTI_THREADS - 1) / IBM_MULTI_THREADS, IBM_MULTI_THREADS, 0, st>>>(
+            s, inverse, weights, areas, nn, nm, it == 0);
+        k_ibm_spread<T, D><<<node_groups + 1, IBM_MULTI_THREADS, 0, st>>>(
+            s, tb, nodes, nn, nm, it == 0, it == iterations - 1, n_total, field, total, body_force);
+    }
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -172,11 +403,50 @@ LBM_API int lbm_ibm_force(int lattice, int dtype, const void* f, const int64_t* 
                     static_cast<T*>(record));
             };
             const bool staged = lds <= lds_limit();
+            if (!staged && !record) {
+                ibm_force_multi<T, L::D>(static_cast<const T*>(f), slot, static_cast<const T*>(coeff),
+                                         static_cast<const T*>(constant), m, nodes, n_nodes, inverse,
+                                         static_cast<const T*>(weights), static_cast<const T*>(areas), n_markers,
+                                         spread_start, spread_entry, iterations, hf, (uint32_t)n_total,
+                                         static_cast<T*>(field), static_cast<T*>(body_force),
+                                         static_cast<T*>(scratch), as_stream(stream));
+                check_launch("IBM kernels");
+                return;
+            }
             if (staged && record) go(std::true_type{}, std::true_type{});
             else if (staged) go(std::true_type{}, std::false_type{});
             else if (record) go(std::false_type{}, std::true_type{});
             else go(std::false_type{}, std::false_type{});
             check_launch("IBM kernel");
+        });
+    });
+}
+
+LBM_API int lbm_ibm_force_multi(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
+                                const void* constant, int m, const int64_t* nodes, int n_nodes, const int* inverse,
+                                const void* weights, const void* areas, int n_markers, const int* spread_start,
+                                const int* spread_entry, int iterations, double fx, double fy, double fz,
+                                int64_t n_total, void* field, void* body_force, void* scratch, void* stream)
+{
+    return guarded("lbm_ibm_force_multi", [&] {
+        require(n_nodes > 0 && n_markers > 0, "the immersed body needs markers and stencil nodes");
+        require(iterations >= 1, "iterations must be >= 1");
+        require(m >= 1 && m <= 128, "moment rows need 1 <= m <= 128 terms");
+        require(n_total > 0 && n_total < (1ll << 31), "node count out of range");
+        require(f && slot && coeff && constant && nodes && inverse && weights && areas && spread_start &&
+                    spread_entry && field && body_force && scratch,
+                "null pointer");
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            const T hf[3] = {T(0.5 * fx), T(0.5 * fy), T(0.5 * fz)};
+            ibm_force_multi<T, L::D>(static_cast<const T*>(f), slot, static_cast<const T*>(coeff),
+                                     static_cast<const T*>(constant), m, nodes, n_nodes, inverse,
+                                     static_cast<const T*>(weights), static_cast<const T*>(areas), n_markers,
+                                     spread_start, spread_entry, iterations, hf, (uint32_t)n_total,
+                                     static_cast<T*>(field), static_cast<T*>(body_force), static_cast<T*>(scratch),
+                                     as_stream(stream));
+            check_launch("IBM kernels");
         });
     });
 }

@@ -62,6 +62,8 @@ SIGNATURES: dict[str, list] = {
     "lbm_ibm_force": [_i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _i, _p, _p, _i, _d, _d, _d, _i64, _p, _p, _p, _p,
                       _p],
     "lbm_ibm_record_size": [_i, _i, _i, _i, _i, POINTER(c_int64)],
+    "lbm_ibm_force_multi": [_i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _i, _p, _p, _i, _d, _d, _d, _i64, _p, _p,
+                            _p, _p],
 }
 
 

@@ -248,7 +248,8 @@ LBM_API int lbm_affine_force_scratch_size(int64_t n_rows, int64_t* bytes);
 LBM_API int lbm_ibm_scratch_size(int lattice, int dtype, int n_nodes, int n_markers, int64_t* bytes);
 
 /* Immersed-boundary direct forcing, all multi-direct iterations in one launch
- * (one workgroup). Moment rows ((d+1) * n_nodes affine rows, term-major m-padded,
+ * (one workgroup; a body whose intermediates do not fit its LDS takes the many-workgroup
+ * launches of lbm_ibm_force_multi when no record is asked for). Moment rows ((d+1) * n_nodes affine rows, term-major m-padded,
  * compiled for the current layout) give drho and rho u at the unique stencil nodes
  * nodes[n_nodes]; inverse[n_markers][3^d] and weights[n_markers][3^d] describe the
  * Roma stencils, areas[n_markers] the surface elements; spread_start[n_nodes + 1] /
@@ -265,6 +266,18 @@ LBM_API int lbm_ibm_force(int lattice, int dtype, const void* f, const int64_t* 
                           const int* spread_entry, int iterations, double fx, double fy, double fz,
                           int64_t n_total, void* field, void* body_force, void* scratch, void* record,
                           void* stream);
+
+/* lbm_ibm_force (without record) in 2 + 2 * iterations launches of many workgroups: per
+ * stencil node the moment rows and u*, per marker the interpolation and marker force, per
+ * node the spread, each value by the same expression in the same order, so the outputs equal
+ * lbm_ibm_force's bit for bit. lbm_ibm_force takes this path itself when the body's
+ * intermediates do not fit one workgroup's LDS (3D spheres); scratch as for lbm_ibm_force.
+ * No counterpart in the reference (see lbm_ibm_force). */
+LBM_API int lbm_ibm_force_multi(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
+                                const void* constant, int m, const int64_t* nodes, int n_nodes, const int* inverse,
+                                const void* weights, const void* areas, int n_markers, const int* spread_start,
+                                const int* spread_entry, int iterations, double fx, double fy, double fz,
+                                int64_t n_total, void* field, void* body_force, void* scratch, void* stream);
 
 /* Bytes of one step's record of the forcing intermediates (optional `record` argument of
  * lbm_ibm_force, read back by lbm_ibm_adjoint), a multiple of 16; records at 16-byte aligned

@@ -309,3 +309,68 @@ def test_step_layer_force_history_of_a_large_probe_equals_the_separate_launches(
         slow.force_on_device(probe, expected[k])
     assert np.array_equal(recorded, expected.cpu().numpy())
     assert np.array_equal(fast.populations(), slow.populations())
+
+
+def large_sphere_case(dtype):
+    """A sphere of radius 5.3 (about 400 markers, about 2,500 stencil nodes): its forcing
+    intermediates do not fit one workgroup's LDS, so lbm_ibm_force takes its
+    many-workgroup launches."""
+    from lbm import SimulationParams
+    from lbm.boundaries import channel_walls
+    from lbm.geometry import sphere
+    from lbm.ibm import ImmersedBoundary
+
+    solid = channel_walls(30, 24, 22)
+    body = ImmersedBoundary.from_ball(sphere(14.3, 11.6, 10.8, 5.3))
+    return SimulationParams(nx=30, ny=24, nz=22, tau=0.7, collision="trt", force=(1e-5, -2e-6, 3e-6), solid=solid,
+                            dtype=dtype, immersed=body)
+
+
+def test_many_workgroup_ibm_of_a_large_sphere_is_bit_identical_to_the_oracle():
+    """Pull layout, float64: 10 steps of the oracle and of hip-pull, whose forcing runs as
+    the many-workgroup launches (the intermediates need more LDS than a workgroup has)."""
+    import ctypes
+
+    from lbm.backends._native import library
+
+    params = large_sphere_case("float64")
+    ibm_nodes = make_solver(params, backend="hip-pull")._ibm
+    size = ctypes.c_int64()
+    library().raw("lbm_ibm_scratch_size")(1, 1, ibm_nodes.n_nodes, ibm_nodes.n_markers, ctypes.byref(size))
+    assert size.value > 160 * 1024  # larger than one workgroup's LDS
+    ref, gpu = both(params, "hip-pull", steps=10)
+    np.testing.assert_array_equal(gpu.populations()[:, params.fluid], ref.populations()[:, params.fluid])
+    np.testing.assert_array_equal(gpu.immersed_force, ref.immersed_force)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("d", [2, 3])
+def test_many_workgroup_ibm_equals_the_one_workgroup_kernel(d, dtype):
+    """lbm_ibm_force_multi against lbm_ibm_force (one workgroup, tables staged in LDS) on the
+    same state: the same field and body force, bit for bit."""
+    import torch
+
+    from lbm.backends._native import pointer
+
+    params = ibm_case(d, dtype)
+    s = make_solver(params, backend="hip-pull")
+    s.initialize(*random_state(params, 5))
+    s.run(3)
+    calls = s._ibm_calls("pull", pointer(s._f), s._stream)
+    assert len(calls) == 1 and calls[0][0] == "lbm_ibm_force"
+    name, args = calls[0]
+    outs = []
+    for entry in ("lbm_ibm_force", "lbm_ibm_force_multi"):
+        field = torch.zeros_like(s._field)
+        body = torch.zeros_like(s._body_force)
+        scratch = torch.full_like(s._ibm_scratch, float("nan"))
+        a = list(args)
+        a[20], a[21], a[22] = pointer(field), pointer(body), pointer(scratch)
+        if entry == "lbm_ibm_force_multi":
+            del a[23]  # no record argument
+        s._call(entry, *a)
+        torch.cuda.synchronize()
+        outs.append((field.cpu().numpy(), body.cpu().numpy()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    assert np.abs(outs[0][0]).max() > 0

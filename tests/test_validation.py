@@ -97,3 +97,63 @@ def test_abc_converges_on_the_oracle():
     results = [run_beltrami(n=n, solver_factory=ReferenceSolver) for n in (16, 24)]
     assert results[1].l2_error < results[0].l2_error
     assert 1.8 < convergence_order(results)[0] < 2.2
+
+
+def periodic_disk_params(rules: bool):
+    """A disk in a fully periodic box pushed by a uniform body force (the reference's
+    tests/test_geometry_link_rules.py:195-209)."""
+    from lbm import SimulationParams
+    from lbm.geometry import circle, voxelize, wall_links
+    from lbm.lattice import D2Q9
+    from lbm.link_rules import bouzidi_rules
+
+    size, body = 24, circle(11.5, 11.5, 3.3)
+    solid = voxelize(body, (size, size))
+    links = wall_links(D2Q9, body, solid)
+    params = SimulationParams(nx=size, ny=size, tau=1.0, collision="trt", force=(1e-6, 0.0), solid=solid,
+                              dtype="float64", link_rules=bouzidi_rules(D2Q9, links, solid) if rules else None)
+    return params, links
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend", ["hip-pull", "hip-aa"])
+@pytest.mark.parametrize("rules", [False, True], ids=["plain", "bouzidi"])
+def test_steady_momentum_exchange_balances_the_body_force(backend, rules):
+    """At steady state the momentum exchange on the disk equals the body force on all the
+    fluid, to 1e-9 relative (tests/test_geometry_link_rules.py:213-221), read on the device
+    (force_on_device) and on the host (ForceProbe.evaluate)."""
+    import torch
+
+    from lbm import make_solver
+    from lbm.forces import ForceProbe
+
+    params, links = periodic_disk_params(rules)
+    solver = make_solver(params, backend=backend)
+    solver.run(6000)
+    probe = ForceProbe.build(params, links)
+    total = params.force[0] * params.fluid.sum()
+    out = torch.zeros(2, dtype=torch.float64, device="cuda")
+    solver.force_on_device(probe, out)
+    for force in (probe.evaluate(solver), out.cpu().numpy()):
+        assert force[0] == pytest.approx(total, rel=1e-9)
+        assert abs(force[1]) < 1e-12 * total
+
+
+@pytest.mark.gpu
+def test_cavity_re1000_matches_ghia():
+    """Ghia, Ghia & Shin Re = 1000 at 128^2 (tests/validation/test_cavity.py:26-29): largest
+    centreline deviation below 0.025 lid velocities (the reference measured 0.0148)."""
+    from lbm.cases.lid_driven_cavity import run_cavity
+
+    assert run_cavity(reynolds=1000, n=128, backend="hip-aa", dtype="float32").max_error < 0.025
+
+
+@pytest.mark.gpu
+def test_ibm_sphere_is_too_draggy_at_coarse_resolution():
+    """The immersed-boundary sphere at N = 10, Re = 20 (tests/validation/test_sphere_drag.py:
+    23-25): C_D 10-25 % above Clift-Grace-Weber, as the reference measured (3.112 against
+    2.735), the diffuse interface widening the body."""
+    from lbm.cases.sphere_drag import run_sphere_drag
+
+    ibm = run_sphere_drag(10, 20.0, "ibm")
+    assert 0.10 < ibm.cd / ibm.reference_cgw - 1 < 0.25

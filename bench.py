@@ -81,7 +81,7 @@ def kernel_times_ms(torch, solver, pairs: int = 20) -> dict[str, float]:
     return {"aa_odd": float(np.mean(odd)), "aa_even": float(np.mean(even))}
 
 
-def cpu_baseline(n: int = 256, steps: int = 2, n2: int = 2048, steps2: int = 4) -> dict:
+def cpu_baseline(n: int = 256, steps: int = 4, n2: int = 2048, steps2: int = 8) -> dict:
     """The NumPy oracle on a bounded sample of the headline workload: D3Q19 fp32 TRT + Guo ABC
     box at n^3, ``steps`` steps after one warm-up step (the first step also pays the page
     faults of NumPy's temporaries), plus the D2Q9 fp32 TRT periodic n2^2 lattice the

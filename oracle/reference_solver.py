@@ -33,8 +33,6 @@ generated from the reference in tests/golden/.
 
 from __future__ import annotations
 
-import math
-
 import numpy as np
 
 # ---------------------------------------------------------------- lattices
@@ -416,15 +414,3 @@ class ReferenceSolver:
         p = self.params
         rho, u = macroscopic_from_post_collision(self.lat, self._f, self._collision_force(), p.solid)
         return Fields(rho, u, np.array(p.solid))
-
-
-def exact_abc_fields(n: int, velocity: float, viscosity: float, t: float):
-    """Decaying ABC (Beltrami) flow with A = B = C = U on an n^3 box (cases/beltrami.py:45-54)."""
-    k = 2.0 * math.pi / n
-    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64),) * 3, indexing="ij")
-    amplitude = velocity * math.exp(-viscosity * k * k * t)
-    ux = amplitude * (np.sin(k * z) + np.cos(k * y))
-    uy = amplitude * (np.sin(k * x) + np.cos(k * z))
-    uz = amplitude * (np.sin(k * y) + np.cos(k * x))
-    p = -0.5 * (ux**2 + uy**2 + uz**2)
-    return 1.0 + p / CS2, ux, uy, uz

@@ -1,8 +1,10 @@
 """Validated, immutable description of one simulation (2D or 3D).
 
-Mirrors ``SimulationParams`` of the reference (``src/lbm/params.py:86-205``):
-same fields, same defaults, same validation messages, so a caller's
-parameter objects and error handling carry over unchanged.
+The interface is the reference's ``SimulationParams`` (``src/lbm/params.py:86-205``):
+the same fields, defaults and validation messages, so a caller's parameter objects
+and error handling carry over. The checks themselves are organised here as one
+normaliser per field (``_normalise_*``), each returning the stored value or
+raising; ``SimulationParams.__post_init__`` only chains them.
 
 Relaxation: nu = cs^2 (tau - 1/2). TRT rates are omega_plus = 1/tau and
 omega_minus = 1 / (1/2 + Lambda / (tau - 1/2)); BGK has omega_minus = omega_plus.
@@ -30,7 +32,7 @@ class StabilityWarning(UserWarning):
 
 
 def check_tau(tau: float) -> None:
-    """Reject tau <= 1/2 and warn below ``TAU_WARNING`` (``params.py:50-60``)."""
+    """Reject tau <= 1/2 (no positive viscosity); warn below ``TAU_WARNING``."""
     if not tau > 0.5:
         raise ValueError(f"tau must be > 0.5 (viscosity nu = (tau - 0.5)/3 > 0); got tau={tau}")
     if tau < TAU_WARNING:
@@ -42,34 +44,73 @@ def check_tau(tau: float) -> None:
         )
 
 
-def _frozen(array: np.ndarray) -> np.ndarray:
-    out = np.array(array, copy=True)
-    out.setflags(write=False)
-    return out
+def _grid_shape(nx: int, ny: int, nz: int | None) -> tuple[int, ...]:
+    """(ny, nx) or (nz, ny, nx); every given extent must be positive."""
+    named = [("nx", nx), ("ny", ny)] + ([] if nz is None else [("nz", nz)])
+    if min(n for _, n in named) <= 0:
+        raise ValueError(
+            f"{' and '.join(k for k, _ in named)} must be positive; got {', '.join(f'{k}={n}' for k, n in named)}"
+        )
+    return (ny, nx) if nz is None else (nz, ny, nx)
 
 
-def _lattice_for(spec, d: int) -> VelocitySet:
+def _check_choice(value, allowed: tuple[str, ...], what: str) -> None:
+    if value not in allowed:
+        raise ValueError(f"{what} must be one of {allowed}; got {value!r}")
+
+
+def _normalise_lattice(spec, d: int) -> VelocitySet:
+    """The velocity set named by ``spec`` (default D2Q9 / D3Q19), of dimension ``d``."""
     if spec is None:
         return default_lattice(d)
-    if isinstance(spec, VelocitySet):
-        lattice = spec
-    elif spec in LATTICES:
-        lattice = LATTICES[spec]
-    else:
+    lattice = spec if isinstance(spec, VelocitySet) else LATTICES.get(spec)
+    if lattice is None:
         raise ValueError(f"unknown lattice {spec!r}; choose from {tuple(LATTICES)}")
-    if lattice.d != d:
-        raise ValueError(
-            f"lattice {lattice.name} is {lattice.d}D but the grid is {d}D "
-            "(set nz for 3D, leave it unset for 2D)"
-        )
-    return lattice
+    if lattice.d == d:
+        return lattice
+    raise ValueError(
+        f"lattice {lattice.name} is {lattice.d}D but the grid is {d}D (set nz for 3D, leave it unset for 2D)"
+    )
+
+
+def _normalise_solid(mask, shape: tuple[int, ...]) -> np.ndarray:
+    """A boolean copy of the mask (all fluid when None)."""
+    if mask is None:
+        return np.zeros(shape, dtype=bool)
+    mask = np.array(mask, dtype=bool, copy=True)
+    if mask.shape != shape:
+        raise ValueError(f"solid mask must have shape {shape}; got {mask.shape}")
+    return mask
+
+
+def _normalise_walls(velocity, solid: np.ndarray) -> np.ndarray:
+    """A float64 copy, shape (d, *shape), kept only at solid nodes (zero when None)."""
+    expected = (solid.ndim, *solid.shape)
+    if velocity is None:
+        return np.zeros(expected)
+    velocity = np.asarray(velocity, dtype=np.float64)
+    if velocity.shape != expected:
+        raise ValueError(f"wall_velocity must have shape {expected}; got {velocity.shape}")
+    return np.where(solid, velocity, 0.0)
+
+
+def _normalise_force(force, d: int) -> tuple[float, ...]:
+    components = (0.0,) * d if force is None else tuple(map(float, force))
+    if len(components) != d:
+        raise ValueError(f"force must have {d} components for a {d}D grid; got {components}")
+    return components
+
+
+def _read_only(array: np.ndarray) -> np.ndarray:
+    array.setflags(write=False)
+    return array
 
 
 @dataclass(frozen=True, eq=False)
 class SimulationParams:
     """Everything a solver needs to advance one simulation.
 
-    Attributes (as in the reference): ``nx``, ``ny``, ``tau``, ``collision``
+    Fields (the reference's): ``nx``, ``ny``, ``tau``, ``collision``
     (``"bgk"``/``"trt"``), ``magic`` (TRT Lambda), ``force`` (uniform body
     force, d components), ``solid`` (bool mask), ``wall_velocity`` (shape
     (d, *shape), read at solid nodes only), ``dtype``, and keyword-only
@@ -91,52 +132,25 @@ class SimulationParams:
     immersed: ImmersedBoundary | None = field(default=None, kw_only=True, repr=False)
 
     def __post_init__(self) -> None:
-        sizes = {"nx": self.nx, "ny": self.ny}
-        if self.nz is not None:
-            sizes["nz"] = self.nz
-        if any(n <= 0 for n in sizes.values()):
-            got = ", ".join(f"{k}={v}" for k, v in sizes.items())
-            raise ValueError(f"{' and '.join(sizes)} must be positive; got {got}")
-        if self.collision not in COLLISION_MODELS:
-            raise ValueError(f"collision must be one of {COLLISION_MODELS}; got {self.collision!r}")
+        shape = _grid_shape(self.nx, self.ny, self.nz)
+        _check_choice(self.collision, COLLISION_MODELS, "collision")
         if not self.magic > 0:
             raise ValueError(f"TRT magic parameter must be > 0; got magic={self.magic}")
-        if self.dtype not in SUPPORTED_DTYPES:
-            raise ValueError(f"dtype must be one of {SUPPORTED_DTYPES}; got {self.dtype!r}")
+        _check_choice(self.dtype, SUPPORTED_DTYPES, "dtype")
         check_tau(self.tau)
-
-        d = 2 if self.nz is None else 3
-        lattice = _lattice_for(self.lattice, d)
-        shape = (self.ny, self.nx) if d == 2 else (self.nz, self.ny, self.nx)
-
-        solid = np.zeros(shape, dtype=bool) if self.solid is None else np.asarray(self.solid)
-        if solid.shape != shape:
-            raise ValueError(f"solid mask must have shape {shape}; got {solid.shape}")
-        solid = solid.astype(bool)
-
-        if self.wall_velocity is None:
-            wall = np.zeros((d, *shape))
-        else:
-            wall = np.asarray(self.wall_velocity, dtype=np.float64)
-        if wall.shape != (d, *shape):
-            raise ValueError(f"wall_velocity must have shape {(d, *shape)}; got {wall.shape}")
-        wall = np.where(solid, wall, 0.0)
-
-        force = (0.0,) * d if self.force is None else tuple(float(v) for v in self.force)
-        if len(force) != d:
-            raise ValueError(f"force must have {d} components for a {d}D grid; got {force}")
-
+        d = len(shape)
+        lattice = _normalise_lattice(self.lattice, d)
+        solid = _normalise_solid(self.solid, shape)
+        wall = _normalise_walls(self.wall_velocity, solid)
+        force = _normalise_force(self.force, d)
         if self.immersed is not None and self.immersed.d != d:
             raise ValueError(f"the immersed body is {self.immersed.d}D but the grid is {d}D")
         if self.link_rules is not None:
             self.link_rules.validate(lattice, solid, wall)
-
-        object.__setattr__(self, "lattice", lattice)
-        object.__setattr__(self, "solid", _frozen(solid))
-        object.__setattr__(self, "wall_velocity", _frozen(wall))
-        object.__setattr__(self, "force", force)
-        object.__setattr__(self, "tau", float(self.tau))
-        object.__setattr__(self, "magic", float(self.magic))
+        normalised = {"lattice": lattice, "solid": _read_only(solid), "wall_velocity": _read_only(wall),
+                      "force": force, "tau": float(self.tau), "magic": float(self.magic)}
+        for name, value in normalised.items():
+            object.__setattr__(self, name, value)
 
     @property
     def d(self) -> int:
@@ -144,11 +158,11 @@ class SimulationParams:
 
     @property
     def shape(self) -> tuple[int, ...]:
-        return (self.ny, self.nx) if self.d == 2 else (self.nz, self.ny, self.nx)
+        return self.solid.shape
 
     @property
     def n_nodes(self) -> int:
-        return int(np.prod(self.shape))
+        return int(self.solid.size)
 
     @property
     def fluid(self) -> np.ndarray:
@@ -160,7 +174,7 @@ class SimulationParams:
 
     @property
     def relaxation_rates(self) -> tuple[float, float]:
-        """(omega_plus, omega_minus), computed in float64 (``params.py:198-205``)."""
+        """(omega_plus, omega_minus), in float64 (the reference's params.py:198-205)."""
         omega_plus = 1.0 / self.tau
         if self.collision == "bgk":
             return omega_plus, omega_plus

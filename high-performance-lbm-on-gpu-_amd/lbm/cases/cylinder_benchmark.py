@@ -23,13 +23,14 @@ from __future__ import annotations
 import argparse
 import math
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
+from typing import NamedTuple
 
 import numpy as np
 
 from lbm.backends import make_solver
 from lbm.boundaries import channel_walls
-from lbm.cases.common import add_backend_arguments, run_until_steady
+from lbm.cases.common import add_backend_arguments, even, run_until_steady
 from lbm.forces import ForceProbe
 from lbm.geometry import circle, voxelize, wall_links
 from lbm.ibm import ImmersedBoundary
@@ -181,58 +182,62 @@ def run_drag_channel(reynolds: float = 20.0, steps: int = 50_000, backend: str =
                       float(coeff[:, 1].mean()), history)
 
 
-@dataclass
+def _pressure_points(g: Geometry) -> AffineSet:
+    """Two affine rows: the mean of 1 + rho - 1 = sum_k f~*_k over the two lattice rows around
+    y = 0.2, at x = 0.15 and x = 0.25 (weight 1/2 per node and direction)."""
+    dx = DIAMETER / g.n
+    centre_row = 0.5 + CENTRE[1] / dx
+    ys = np.array([math.floor(centre_row), math.ceil(centre_row)])
+    xs = np.array([round(1 + 0.15 / dx), round(1 + 0.25 / dx)])
+    q = g.params.lattice.q
+    nodes = np.ravel_multi_index((ys[None, :], xs[:, None]), g.params.shape)  # (point, row)
+    per_point = np.repeat(nodes, q, axis=1)  # node of each term: rows outer, directions inner
+    directions = np.tile(np.arange(q), (2, len(ys)))
+    return AffineSet(directions, per_point.astype(np.int64), np.full(per_point.shape, 0.5), np.zeros(2))
+
+
 class Observer:
-    """Records C_D, C_L and dp of a running benchmark (``cylinder_benchmark.py:150-196``)."""
+    """Callable recording (C_D, C_L, dp) of a running benchmark; ``history`` holds
+    (time step, C_D, C_L, dp) per call. The drag and lift come from the momentum-exchange
+    probe, reduced on the device into a float64 tensor (or from the immersed body's force);
+    dp from the two pressure points, one sparse evaluation per call."""
 
-    geometry: Geometry
-    probe: ForceProbe | None = field(init=False)
-    history: list = field(default_factory=list)
-
-    def __post_init__(self) -> None:
-        g = self.geometry
-        self.probe = None if g.cylinder_links is None else ForceProbe.build(g.params, g.cylinder_links)
-        # Pressure points: density = 1 + sum_k f~*_k averaged over the two rows around y = 0.2
-        # at x = 0.15 and x = 0.25, one affine row each (weight 1/2 per row node).
-        dx = DIAMETER / g.n
-        row = 0.5 + CENTRE[1] / dx
-        rows = (math.floor(row), math.ceil(row))
-        shape = g.params.shape
-        q = g.params.lattice.q
-        points = []
-        for x in (round(1 + 0.15 / dx), round(1 + 0.25 / dx)):
-            nodes = [int(np.ravel_multi_index((y, x), shape)) for y in rows]
-            points.append([(k, node, 0.5) for node in nodes for k in range(q)])
-        self._points = AffineSet.from_lists(points)
-        self._compiled = {}
+    def __init__(self, geometry: Geometry):
+        self.geometry = geometry
+        links = geometry.cylinder_links
+        self.probe = None if links is None else ForceProbe.build(geometry.params, links)
+        self.history: list = []
+        self._points = _pressure_points(geometry)
+        self._points_by_layout: dict = {}
         self._force = None
 
-    def _density(self, solver) -> np.ndarray:
-        layout = solver.state_layout
-        if layout not in self._compiled:
-            p = self.geometry.params
-            self._compiled[layout] = compile_affine(p.lattice, self._points, layout, p.solid, p.wall_velocity)
-        return 1.0 + solver.evaluate_affine(self._compiled[layout])
+    def _drag_lift(self, solver) -> np.ndarray:
+        if self.probe is None:
+            return np.asarray(solver.immersed_force)
+        if self._force is None:
+            torch = solver._torch
+            self._force = torch.zeros(self.geometry.params.d, dtype=torch.float64, device=solver.device)
+        solver.force_on_device(self.probe, self._force)
+        return self._force.cpu().numpy()
+
+    def _pressure_drop(self, solver) -> float:
+        p = self.geometry.params
+        rows = self._points_by_layout.get(solver.state_layout)
+        if rows is None:
+            rows = compile_affine(p.lattice, self._points, solver.state_layout, p.solid, p.wall_velocity)
+            self._points_by_layout[solver.state_layout] = rows
+        front, back = 1.0 + solver.evaluate_affine(rows)
+        return CS2 * (front - back) * self.geometry.velocity_scale**2
 
     def __call__(self, solver) -> np.ndarray:
         g = self.geometry
-        if self.probe is None:
-            fx, fy = solver.immersed_force
-        else:
-            if self._force is None:
-                torch = solver._torch
-                self._force = torch.zeros(g.params.d, dtype=torch.float64, device=solver.device)
-            solver.force_on_device(self.probe, self._force)
-            fx, fy = self._force.cpu().numpy()
-        front, back = self._density(solver)
-        scale = 0.5 * g.u_mean**2 * g.n
-        values = np.array([fx / scale, fy / scale, CS2 * (front - back) * g.velocity_scale**2])
+        coefficients = self._drag_lift(solver) / (0.5 * g.u_mean**2 * g.n)
+        values = np.append(coefficients, self._pressure_drop(solver))
         self.history.append((solver.time_step, *values))
         return values
 
 
-@dataclass(frozen=True)
-class SteadyResult:
+class SteadyResult(NamedTuple):
     n: int
     cd: float
     cl: float
@@ -240,21 +245,7 @@ class SteadyResult:
     steps: int
 
 
-def run_steady(n: int = 20, backend: str = "hip-aa", dtype: str = "float64", **options) -> SteadyResult:
-    """Test 2D-1 to a steady state (checks once per convective time D / U_mean)."""
-    geometry = build("2D-1", n, dtype=dtype, **options)
-    solver = make_solver(geometry.params, backend=backend)
-    geometry.start(solver)
-    observe = Observer(geometry)
-    interval = 2 * round(n / geometry.u_mean / 2)
-    tol = 1e-7 if dtype == "float64" else 1e-5
-    steps = run_until_steady(solver, observe, interval=interval, tol=tol, max_steps=5_000_000)
-    cd, cl, dp = observe(solver)
-    return SteadyResult(n, float(cd), float(cl), float(dp), steps)
-
-
-@dataclass(frozen=True)
-class PeriodicResult:
+class PeriodicResult(NamedTuple):
     n: int
     cd_max: float
     cl_max: float
@@ -263,30 +254,46 @@ class PeriodicResult:
     history: np.ndarray
 
 
+def _observed_run(test: str, n: int, backend: str, dtype: str, options: dict):
+    """Geometry, a solver started from the inflow profile, and an observer."""
+    geometry = build(test, n, dtype=dtype, **options)
+    solver = make_solver(geometry.params, backend=backend)
+    geometry.start(solver)
+    return geometry, solver, Observer(geometry)
+
+
+def run_steady(n: int = 20, backend: str = "hip-aa", dtype: str = "float64", **options) -> SteadyResult:
+    """Test 2D-1 to a steady state, checked once per convective time D / U_mean (to 1e-7 in
+    float64, 1e-5 in float32)."""
+    geometry, solver, observe = _observed_run("2D-1", n, backend, dtype, options)
+    convective = even(n / geometry.u_mean)
+    steps = run_until_steady(solver, observe, interval=convective, tol={"float64": 1e-7, "float32": 1e-5}[dtype],
+                             max_steps=5_000_000)
+    return SteadyResult(n, *map(float, observe(solver)), steps)
+
+
 def lift_period(t: np.ndarray, cl: np.ndarray) -> float:
-    """Mean period of C_L(t) in steps from its upward zero crossings, each located by linear
-    interpolation between the samples around it (``cylinder_benchmark.py:248-258``)."""
+    """Mean period of C_L(t) in steps between its upward zero crossings, each placed by
+    linear interpolation between the two samples around it."""
     up = np.flatnonzero((cl[:-1] < 0) & (cl[1:] >= 0))
-    if len(up) < 2:
+    if up.size < 2:
         raise ValueError("need at least two upward zero crossings of C_L to measure a period")
-    crossings = t[up] - cl[up] * (t[up + 1] - t[up]) / (cl[up + 1] - cl[up])
-    return float(np.mean(np.diff(crossings)))
+    fraction = cl[up] / (cl[up] - cl[up + 1])
+    crossings = t[up] + fraction * (t[up + 1] - t[up])
+    return float(np.diff(crossings).mean())
 
 
 def run_periodic(n: int = 20, backend: str = "hip-aa", dtype: str = "float32", convective_times: float = 150.0,
                  samples_per_time: int = 40, **options) -> PeriodicResult:
-    """Test 2D-2: run ``convective_times`` D / U_mean, measure on the last quarter."""
-    geometry = build("2D-2", n, dtype=dtype, **options)
-    solver = make_solver(geometry.params, backend=backend)
-    geometry.start(solver)
-    observe = Observer(geometry)
-    t_conv = n / geometry.u_mean
-    every = max(2, 2 * round(t_conv / samples_per_time / 2))
-    total = every * round(convective_times * t_conv / every)
+    """Test 2D-2: ``convective_times`` D / U_mean with ``samples_per_time`` samples per
+    convective time; C_D, C_L maxima and the Strouhal number from the last quarter."""
+    geometry, solver, observe = _observed_run("2D-2", n, backend, dtype, options)
+    convective = n / geometry.u_mean
+    every = even(convective / samples_per_time)
+    total = every * round(convective_times * convective / every)
     solver.run(total, every=every, callback=observe)
     history = np.asarray(observe.history)
-    tail = history[len(history) * 3 // 4 :]
-    t, cd, cl = tail[:, 0], tail[:, 1], tail[:, 2]
+    t, cd, cl = history[len(history) * 3 // 4 :, :3].T
     strouhal = n / (lift_period(t, cl) * geometry.u_mean)
     return PeriodicResult(n, float(cd.max()), float(cl.max()), strouhal, total, history)
 

@@ -40,16 +40,16 @@ __global__ __launch_bounds__(step_max_threads(L::D, sizeof(T))) void k_pull(cons
 // writes it into the source slot (opp j, x) that the bounce-back of direction j reads. The
 // pull step never writes its source buffer, so rows evaluated during the step read exactly
 // the pre-rule populations the pre-pass reads, and the step can use the values without the
-// pre-pass launch. The nodes with rows (sorted) are described by node_info[k] = {node
-// index, directions i whose upstream node x - c_i is solid, directions j with a row, 0};
-// node k owns rows k per .. k per + per - 1, its rows in direction order, then padding
-// rows (never read), so a chunk's rows follow from its nodes without a lookup. The nodes
-// are updated in chunks of `chunk` nodes (rule_chunk_nodes).
+// pre-pass launch. Rows are sorted by (node, direction); the nodes with rows (sorted) are
+// described by node_info[k] = {node index, directions i whose upstream node x - c_i is
+// solid, directions j with a row, the node's first row}, and node_first[k] = that first
+// row again (node_first[count] = n) gives a chunk's row range. The nodes are updated in
+// chunks of `chunk` nodes (rule_chunk_nodes).
 template <class T>
 struct RuleRows {
     const uint4* node_info;
+    const int32_t* node_first;
     int count;
-    int per;
     int chunk;
     const int64_t* slot;
     const T* coeff;
@@ -77,8 +77,7 @@ __host__ __device__ constexpr int rule_chunk_count(int count, int chunk) { retur
 
 // One chunk of rule nodes by one workgroup of `lanes` threads (tid the linear index): the
 // workgroup evaluates the chunk's rows into the LDS array `vals` (row first + e by thread
-// e mod lanes; padding rows too, which cost a load of zero terms), threads below rr.chunk
-// update node c rr.chunk + tid as pull_update does,
+// e mod lanes), threads below rr.chunk update node c rr.chunk + tid as pull_update does,
 // their rule directions taking the rows' values after the barrier. With the blocked set in
 // the node's record, each population is loaded once from its selected slot (pulled or
 // bounce-back), in flight with the rows' loads, all before the barrier. Same operations in
@@ -95,8 +94,8 @@ __device__ __forceinline__ void rule_chunk(const T* __restrict__ src, T* __restr
     const bool node_lane = first_node + tid < end_node;
     uint4 info = make_uint4(0u, 0u, 0u, 0u);
     if (node_lane) info = rr.node_info[first_node + tid];
-    const int64_t first_row = (int64_t)first_node * rr.per;
-    const int64_t end_row = (int64_t)end_node * rr.per;
+    const int64_t first_row = rr.node_first[first_node];
+    const int64_t end_row = rr.node_first[end_node];
     Coord cc{0, 0, 0};
     const uint32_t node = info.x;
     const uint32_t blocked = info.y;
@@ -121,7 +120,7 @@ __device__ __forceinline__ void rule_chunk(const T* __restrict__ src, T* __restr
     __syncthreads();
     if (node_lane) {
         uint32_t has = info.z;
-        int local = tid * rr.per;
+        int local = (int)((int64_t)info.w - first_row);
         while (has) {
             const int j = __ffs(has) - 1;
             has &= has - 1u;
@@ -539,8 +538,8 @@ int run_step(const char* name, Variant v, int lattice, int dtype, const void* a,
 // The pull step with in-step link rules and the force layer (k_pull_links).
 int run_pull_links(int lattice, int dtype, const void* src, void* dst, const uint8_t* kind, const void* wall,
                    const void* field, const void* body, int nx, int ny, int nz, double wp, double wm, double fx,
-                   double fy, double fz, int force_mode, const uint32_t* rule_node_info, int rule_nodes,
-                   int rules_per_node, const int64_t* rule_slot,
+                   double fy, double fz, int force_mode, const uint32_t* rule_node_info,
+                   const int32_t* rule_node_first, int rule_nodes, const int64_t* rule_slot,
                    const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m,
                    const int64_t* force_slot, const void* force_coeff, const void* force_constant,
                    const double* force_weight, int64_t n_force, int force_m, double* force_out,
@@ -556,11 +555,8 @@ int run_pull_links(int lattice, int dtype, const void* src, void* dst, const uin
         require(force_mode != 1 || field != nullptr, "field force mode needs a force field");
         require(n_rules >= 0 && n_rules < (1ll << 31), "rule count out of range");
         require(rule_nodes >= 0 && (n_rules > 0) == (rule_nodes > 0), "rows need rule nodes and nodes rows");
-        require(n_rules == 0 || (rules_per_node >= 1 && rules_per_node <= 18 &&
-                                 n_rules == (int64_t)rule_nodes * rules_per_node),
-                "n_rules must be rule_nodes * rules_per_node, 1 <= rules_per_node <= 18");
-        require(n_rules == 0 || (rule_node_info && rule_slot && rule_coeff && rule_constant && rule_m >= 1 &&
-                                 rule_m <= 128),
+        require(n_rules == 0 || (rule_node_info && rule_node_first && rule_slot && rule_coeff && rule_constant &&
+                                 rule_m >= 1 && rule_m <= 128),
                 "link rules need the node list, slot, coeff, constant and 1 <= m <= 128");
         require(reinterpret_cast<uintptr_t>(rule_node_info) % 16 == 0, "rule_node_info must be 16-byte aligned");
         require(n_force >= 0, "force row count out of range");
@@ -583,7 +579,7 @@ int run_pull_links(int lattice, int dtype, const void* src, void* dst, const uin
             const dim3 tiles = grid_of(g, block);
             const int64_t n_tiles = (int64_t)tiles.x * tiles.y * tiles.z;
             const int chunk = rule_chunk_nodes(rule_nodes, (int)std::min<int64_t>(n_tiles, 1 << 20));
-            const RuleRows<T> rr{reinterpret_cast<const uint4*>(rule_node_info), rule_nodes, rules_per_node, chunk,
+            const RuleRows<T> rr{reinterpret_cast<const uint4*>(rule_node_info), rule_node_first, rule_nodes, chunk,
                                  rule_slot, static_cast<const T*>(rule_coeff), static_cast<const T*>(rule_constant),
                                  n_rules, rule_m};
             const ForceRows<T> fw{force_slot, static_cast<const T*>(force_coeff),
@@ -592,7 +588,7 @@ int run_pull_links(int lattice, int dtype, const void* src, void* dst, const uin
             require(n_tiles + work < (1ll << 31), "grid too large for a one-dimensional launch");
             const dim3 grid((unsigned)(n_tiles + work));
             size_t needed = 0;
-            if (rule_nodes) needed = std::max(needed, sizeof(T) * (size_t)chunk * rules_per_node);
+            if (rule_nodes) needed = std::max(needed, sizeof(T) * (size_t)chunk * (L::Q - 1));
             if (groups) needed = std::max(needed, sizeof(double) * 3 * FORCE_WAVES);
             const size_t lds = occupancy_lds(groups_per_cu, needed);
             const hipStream_t s = as_stream(stream);
@@ -670,16 +666,16 @@ LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst
 LBM_API int lbm_pull_step_links(int lattice, int dtype, const void* f_src, void* f_dst, const uint8_t* kind,
                                 const void* wall_u, const void* force_field, const void* body_field, int nx, int ny,
                                 int nz, double omega_plus, double omega_minus, double fx, double fy, double fz,
-                                int force_mode, const uint32_t* rule_node_info, int rule_nodes,
-                                int rules_per_node, const int64_t* rule_slot,
+                                int force_mode, const uint32_t* rule_node_info, const int32_t* rule_node_first,
+                                int rule_nodes, const int64_t* rule_slot,
                                 const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m,
                                 const int64_t* force_slot, const void* force_coeff, const void* force_constant,
                                 const double* force_weight, int64_t n_force, int force_m, double* force_out,
                                 double* force_partial, int bx, int by, int groups_per_cu, void* stream)
 {
     return run_pull_links(lattice, dtype, f_src, f_dst, kind, wall_u, force_field, body_field, nx, ny, nz,
-                          omega_plus, omega_minus, fx, fy, fz, force_mode, rule_node_info, rule_nodes,
-                          rules_per_node, rule_slot, rule_coeff,
+                          omega_plus, omega_minus, fx, fy, fz, force_mode, rule_node_info, rule_node_first,
+                          rule_nodes, rule_slot, rule_coeff,
                           rule_constant, n_rules, rule_m, force_slot, force_coeff, force_constant, force_weight,
                           n_force, force_m, force_out, force_partial, bx, by, groups_per_cu, stream);
 }

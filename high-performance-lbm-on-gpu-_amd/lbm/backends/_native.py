@@ -26,7 +26,7 @@ SIGNATURES: dict[str, list] = {
     "lbm_lattice_tables_device": [_i, _i, _p, _p, _p, _p],
     "lbm_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _i, _p],
     "lbm_pull_step_links": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i,
-                            _p, _i, _i, _p, _p, _p, _i64, _i, _p, _p, _p, _p, _i64, _i, _p, _p, _i, _i, _i, _p],
+                            _p, _p, _i, _p, _p, _p, _i64, _i, _p, _p, _p, _p, _i64, _i, _p, _p, _i, _i, _i, _p],
     "lbm_force_totals": [_p, _i64, _i, _i, _p, _p],
     "lbm_affine_force_groups": [_i64, POINTER(c_int)],
     "lbm_aa_even": [_i, _i, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _i, _i, _i, _i, _p],

@@ -199,6 +199,19 @@ class _AffinePack:
 KIND_RULES = 8
 
 
+# The pull step's link rules go inside the step (one launch per step) up to this many nodes;
+# above, the saved pre-pass launch is worth less than the rule layer's scattered accesses
+# to the end planes of open ducts. Bouzidi body with Zou-He ends, fp32, MLUPS pre-pass ->
+# in-step: channels 1024^2 51.9 -> 59.8 K, 2048^2 62.6 -> 65.4 K, 4096^2 75.6 -> 75.8 K;
+# ducts 256 x 64 x 64 25.5 -> 29.2 K, 384 x 96 x 96 24.3 -> 24.1 K, 512 x 128 x 128
+# 28.9 -> 25.7 K; the 880 x 164 channel 15.5 -> 6.9 us per step.
+RULES_IN_STEP_MAX_NODES = 2**22
+
+
+def default_rules_in_step(params) -> bool:
+    return params.n_nodes <= RULES_IN_STEP_MAX_NODES
+
+
 def link_rule_index(lattice: VelocitySet, target: np.ndarray, solid: np.ndarray):
     """The rule-node list of a pull-layout rule set for lbm_pull_step_links: the rows sorted
     by (node, direction j), where row r writes slot target[r] = opp(j) n + node (n nodes of
@@ -245,10 +258,14 @@ class HipSolver(Solver):
             ``default_groups_per_cu``.
         autotune: Time a few block shapes and caps on this grid and keep the fastest.
         graphs: Capture two steps (pre-passes and kernels) into a HIP graph and replay it.
+        rules_in_step: Pull variant with link rules: evaluate them inside the step
+            (``lbm_pull_step_links``) rather than in a pre-pass launch (same bits); default
+            from ``default_rules_in_step``. The per-step force of ``record_forces`` rides in
+            the step whenever the block allows.
     """
 
     def __init__(self, params, variant: str = "pull", block=None, autotune: bool = False, graphs: bool = False,
-                 groups_per_cu=None):
+                 groups_per_cu=None, rules_in_step=None):
         check_variant(variant, params.d)
         block = default_block(params, variant) if block is None else check_block(block)
         if groups_per_cu is None:
@@ -314,25 +331,18 @@ class HipSolver(Solver):
         # The pull step evaluates its link rules in the consuming lanes (lbm_pull_step_links):
         # the rows sorted by (node, direction) and a per-node index, built once.
         self._links = None
-        if variant == "pull" and "pull" in self._rule_passes:
+        self.rules_in_step = default_rules_in_step(p) if rules_in_step is None else bool(rules_in_step)
+        if variant == "pull" and "pull" in self._rule_passes and self.rules_in_step:
             compiled = compile_rules(p.lattice, p.link_rules, "pull", p.solid, p.wall_velocity)
             order, node_info, node_first = link_rule_index(p.lattice, compiled.target, p.solid)
-            # node k owns rows k * per ... k * per + per - 1: its rows, then zero padding
-            counts = np.diff(node_first)
-            per = int(counts.max())
-            at = np.repeat(np.arange(counts.size) * per, counts) + (np.arange(order.size) - np.repeat(node_first[:-1],
-                                                                                                       counts))
             v = compiled.values
-            slots = np.zeros((counts.size * per, np.asarray(v.slots).shape[1]), dtype=np.int64)
-            coeff = np.zeros((counts.size * per, np.asarray(v.coeff).shape[1]))
-            constant = np.zeros(counts.size * per)
-            slots[at], coeff[at], constant[at] = (np.asarray(v.slots)[order], np.asarray(v.coeff)[order],
-                                                  np.asarray(v.constant)[order])
-            links = _AffinePack(torch, CompiledAffine(slots=slots, coeff=coeff, constant=constant), self._np_dtype,
-                                self.device)
+            links = _AffinePack(torch, CompiledAffine(slots=np.asarray(v.slots)[order],
+                                                      coeff=np.asarray(v.coeff)[order],
+                                                      constant=np.asarray(v.constant)[order]),
+                                self._np_dtype, self.device)
             links.node_info = torch.as_tensor(node_info.view(np.int32), device=self.device)
-            links.nodes = counts.size
-            links.per = per
+            links.node_first = torch.as_tensor(node_first, device=self.device)
+            links.nodes = len(node_first) - 1
             self._links = links
             flat = kinds.reshape(-1)
             flat[node_info[:, 0].astype(np.int64)] |= KIND_RULES
@@ -525,10 +535,10 @@ class HipSolver(Solver):
         bx, by = self.block
         lk = self._links
         if lk is not None:
-            rules = (pointer(lk.node_info), lk.nodes, lk.per, pointer(lk.slot), pointer(lk.coeff),
+            rules = (pointer(lk.node_info), pointer(lk.node_first), lk.nodes, pointer(lk.slot), pointer(lk.coeff),
                      pointer(lk.constant), lk.n, lk.m)
         else:
-            rules = (None, 0, 1, None, None, None, 0, 1)
+            rules = (None, None, 0, None, None, None, 0, 1)
         if force is None:
             rows = (None, None, None, None, 0, 1, None, None)
         else:
@@ -692,7 +702,10 @@ class HipSolver(Solver):
         pack = self._force_pack(probe)
         stream = self._stream
         lanes = self.block[0] * self.block[1]
-        if self.variant == "pull" and self._mask_mode and lanes % 64 == 0 and 1024 % lanes == 0:
+        rules_ride = self._links is not None or "pull" not in self._rule_passes
+        if self.variant == "pull" and self._mask_mode and rules_ride and lanes % 64 == 0 and 1024 % lanes == 0:
+            # (with a rule pre-pass the force must be read before the pre-pass writes the
+            # slots it reads, so that case keeps the launches below)
             return self._record_forces_in_step(pack, steps, out)
         sequences = self._step_calls(stream)
         # the state after a step: the AA buffer, or the destination of that parity's step
@@ -807,5 +820,5 @@ class HipSolver(Solver):
         return out
 
 
-__all__ = ["HipSolver", "DeviceArrays", "KIND_RULES", "link_rule_index", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block",
+__all__ = ["HipSolver", "DeviceArrays", "KIND_RULES", "link_rule_index", "default_rules_in_step", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block",
            "default_groups_per_cu"]

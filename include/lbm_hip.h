@@ -80,19 +80,18 @@ LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst
 /* lbm_pull_step (mask_mode 1) with the link rules evaluated inside the step and, optionally,
  * the momentum-exchange force of the state it reads, in one launch.
  * Link rules: rule rows in the pull layout (as for lbm_rules_apply: term-major slot/coeff,
- * row stride n_rules, m = rule_m terms, constant), where the row of link (node, j) gives
- * f_j(node, t + 1). The rule_nodes nodes with rows, sorted, are described by
- * rule_node_info[4 k .. 4 k + 3] = {node, bit set of the directions i whose upstream node
- * node - c_i is solid, bit set of the directions j with a row, 0} (16-byte aligned); node k
- * owns rows k * rules_per_node ... + rules_per_node - 1: its rows in direction order, then
- * padding rows (zero coefficients; evaluated, never used), so n_rules = rule_nodes *
- * rules_per_node. kind must carry KIND_RULES (8) on exactly the listed nodes: those are
- * updated, 16 to 64 nodes per workgroup, by an extra layer of workgroups that evaluate the
- * nodes' rows (one row per thread, the values shared through LDS; bx * by >= 64), every other
- * non-solid node by the ordinary grid. f_src is never written during the step, so the rows
- * read the populations a pre-pass reads and the result equals lbm_rules_apply +
- * lbm_pull_step bit for bit, without the pre-pass launch. n_rules = 0 (rule_nodes = 0): no
- * rules (the rule arrays may be NULL).
+ * row stride n_rules, m = rule_m terms, constant) sorted by (node, direction j), where the
+ * row of link (node, j) gives f_j(node, t + 1). The rule_nodes nodes with rows, sorted, are
+ * described by rule_node_info[4 k .. 4 k + 3] = {node, bit set of the directions i whose
+ * upstream node node - c_i is solid, bit set of the directions j with a row, index of the
+ * node's first row} (16-byte aligned), with rule_node_first[k] = that first row again
+ * (rule_node_first[rule_nodes] = n_rules). kind must carry KIND_RULES (8) on exactly the
+ * listed nodes: those are updated, 16 to 64 nodes per workgroup, by an extra layer of
+ * workgroups that evaluate the nodes' rows (one row per thread, the values shared through
+ * LDS; bx * by >= 64), every other non-solid node by the ordinary grid. f_src is never
+ * written during the step, so the rows read the populations a pre-pass reads and the
+ * result equals lbm_rules_apply + lbm_pull_step bit for bit, without the pre-pass launch.
+ * n_rules = 0 (rule_nodes = 0): no rules (the rule arrays may be NULL).
  * Force: when n_force > 0, the extra layer also reduces the force rows on f_src exactly as
  * lbm_affine_force does (same groups, same sums; bx * by must be 64, 128, 256, 512 or 1024)
  * into force_out[d] (up to 2048 rows) or force_partial (3 doubles per group of
@@ -103,8 +102,8 @@ LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst
 LBM_API int lbm_pull_step_links(int lattice, int dtype, const void* f_src, void* f_dst, const uint8_t* kind,
                                 const void* wall_u, const void* force_field, const void* body_field, int nx, int ny,
                                 int nz, double omega_plus, double omega_minus, double fx, double fy, double fz,
-                                int force_mode, const uint32_t* rule_node_info, int rule_nodes,
-                                int rules_per_node, const int64_t* rule_slot,
+                                int force_mode, const uint32_t* rule_node_info, const int32_t* rule_node_first,
+                                int rule_nodes, const int64_t* rule_slot,
                                 const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m,
                                 const int64_t* force_slot, const void* force_coeff, const void* force_constant,
                                 const double* force_weight, int64_t n_force, int force_m, double* force_out,

@@ -316,8 +316,9 @@ def test_channel_and_duct_runs_report_throughput_and_drag():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize(("backend", "steps"), [("hip-pull", 7), ("hip-pull", 8), ("hip-aa", 8)])
-def test_prepared_force_recording_equals_the_step_by_step_loop(backend, steps):
+@pytest.mark.parametrize(("backend", "steps", "options"), [("hip-pull", 7, {}), ("hip-pull", 8, {}), ("hip-aa", 8, {}),
+                                                          ("hip-pull", 7, {"rules_in_step": False})])
+def test_prepared_force_recording_equals_the_step_by_step_loop(backend, steps, options):
     """``record_forces`` (launch lists prepared once) against ``_advance`` + ``force_on_device``
     per step: the same forces and the same final state, bit for bit, odd step counts too."""
     import torch
@@ -328,7 +329,7 @@ def test_prepared_force_recording_equals_the_step_by_step_loop(backend, steps):
     params, links, profile = drag_channel(reynolds=20.0, nx=220, ny=42, radius=5.0, centre=(50.0, 20.0))
     probe = ForceProbe.build(params, links)
     ux = np.where(params.solid, 0.0, profile[:, None])
-    fast, slow = make_solver(params, backend=backend), make_solver(params, backend=backend)
+    fast, slow = make_solver(params, backend=backend, **options), make_solver(params, backend=backend)
     for s in (fast, slow):
         s.initialize(1.0, ux, 0.0)
     recorded = fast.record_forces(probe, steps)

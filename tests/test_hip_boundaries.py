@@ -272,8 +272,8 @@ def test_rules_evaluated_in_the_step_equal_the_pre_pass_form(d, dtype):
     from dataclasses import replace
 
     for params in (bouzidi_case(d, dtype)[0], replace(open_case(d)[0], dtype=dtype)):
-        a, b = make_solver(params, backend="hip-pull"), make_solver(params, backend="hip-pull")
-        assert a._links is not None and a._links.n == a._links.nodes * a._links.per >= len(params.link_rules)
+        a, b = make_solver(params, backend="hip-pull"), make_solver(params, backend="hip-pull", rules_in_step=False)
+        assert a._links is not None and a._links.n == len(params.link_rules) and b._links is None
         for s in (a, b):
             s.initialize(*random_state(params, 4))
         a.run(25)
@@ -374,3 +374,21 @@ def test_many_workgroup_ibm_equals_the_one_workgroup_kernel(d, dtype):
     np.testing.assert_array_equal(outs[0][0], outs[1][0])
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
     assert np.abs(outs[0][0]).max() > 0
+
+
+def test_large_lattices_keep_the_rule_pre_pass_and_both_forms_agree():
+    """Above RULES_IN_STEP_MAX_NODES the pull solver applies its rules in a pre-pass by
+    default; either form, asked for explicitly, gives the same bits."""
+    from lbm.backends.hip_solver import RULES_IN_STEP_MAX_NODES
+    from lbm.cases.sphere_drag import duct_sphere
+
+    params, _, inflow = duct_sphere(nx=256, ny=128, nz=136, radius=8.0, centre=(64.0, 64.0, 68.0))
+    assert params.n_nodes > RULES_IN_STEP_MAX_NODES
+    default = make_solver(params, backend="hip-pull")
+    assert not default.rules_in_step and default._links is None
+    in_step = make_solver(params, backend="hip-pull", rules_in_step=True)
+    ux = np.where(params.solid, 0.0, inflow[0][:, :, None])
+    for s in (default, in_step):
+        s.initialize(1.0, ux, 0.0, 0.0)
+        s.run(6)
+    np.testing.assert_array_equal(default.populations(), in_step.populations())

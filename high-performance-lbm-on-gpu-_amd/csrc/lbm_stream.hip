@@ -273,25 +273,43 @@ __device__ __forceinline__ int tile_wrap(int v, int n)
     return v < n ? v : v % n;
 }
 
-// Stage one tile cell (ex, ey) in [0, hx) x [0, hy): all Q planes and the kind byte.
+// Stage one tile cell (ex, ey) in [0, hx) x [0, hy) and its kind byte: Q loads and Q LDS
+// stores, unconditional. Of a halo cell (halo) the tile pulls only the planes i whose
+// downstream cell (ex + c_ix, ey + c_iy) lies in the tile (3 of 9 on an edge, 1 at a
+// corner); an unpulled plane is read at the same row in the nearest column of the tile
+// (ix) and lands in an LDS slot that nothing reads. For the halo columns and the corners,
+// which belong to the tiles next door in x, that keeps their cache lines out of this
+// workgroup's traffic: the round-robin dispatch over the XCDs (each with its own L2) gives
+// those tiles to other XCDs, so their lines were fetched from HBM a second time. Cells of
+// the halo rows read themselves (ix = ex), as before.
 template <class L, class T, bool MASK>
-__device__ __forceinline__ void stage_cell(T* tile, uint8_t* kind_tile, int cells, int hx, int ex, int ey,
-                                           int x0, int y0, const T* __restrict__ src,
+__device__ __forceinline__ void stage_cell(T* tile, uint8_t* kind_tile, int cells, int hx, int hy, int ex, int ey,
+                                           bool halo, int x0, int y0, const T* __restrict__ src,
                                            const uint8_t* __restrict__ kind, const Grid& g)
 {
     const int gx = tile_wrap(x0 + ex - 1, g.nx), gy = tile_wrap(y0 + ey - 1, g.ny);
     const uint32_t gnode = (uint32_t)gy * (uint32_t)g.nx + (uint32_t)gx;
+    const int ix = min(max(ex, 1), hx - 2);
+    // the offset to gnode as a value: a select between the two variables themselves is
+    // compiled as a select of their addresses, which puts both on the stack
+    const uint32_t inner = (uint32_t)gy * (uint32_t)g.nx + (uint32_t)tile_wrap(x0 + ix - 1, g.nx);
+    const uint32_t step_out = gnode - inner;
     const int e = ey * hx + ex;
     T v[L::Q];
-    static_for<0, L::Q>([&](auto I) { v[I] = src[(size_t)I * g.n + gnode]; });
+    static_for<0, L::Q>([&](auto I) {
+        using Di = Dir<L, I>;
+        const int dx = ex + Di::x, dy = ey + Di::y;
+        const bool pulled = !halo || (dx >= 1 && dx < hx - 1 && dy >= 1 && dy < hy - 1);
+        v[I] = src[(size_t)I * g.n + (inner + (pulled ? step_out : 0u))];
+    });
     kind_tile[e] = MASK ? kind[gnode] : 0;
     static_for<0, L::Q>([&](auto I) { tile[I * cells + e] = v[I]; });
 }
 
-// LDS-tiled two-lattice step (2D): the block's tile plus a one-node halo of all
-// Q planes and of the kind bytes is staged in LDS, then each thread pulls its
-// neighbours from LDS. Tile = block; out-of-domain threads help load and reach
-// the barrier (src/lbm/backends/kernels/stream_collide_tiled.cu:24-79). Each
+// LDS-tiled two-lattice step (2D): the block's tile (all Q planes) plus a one-node halo
+// (the planes pulled into the tile) and the kind bytes of both are staged in LDS, then each
+// thread pulls its neighbours from LDS. Tile = block; out-of-domain threads help load and
+// reach the barrier (src/lbm/backends/kernels/stream_collide_tiled.cu:24-79). Each
 // thread stages its own cell (coalesced rows, no index division); the halo ring
 // of 2 (bx + 2) + 2 by cells is staged by the first threads in a second pass.
 template <class L, class T, ForceMode FM, bool MASK>
@@ -309,7 +327,7 @@ __global__ __launch_bounds__(1024) void k_lds(const T* __restrict__ src, T* __re
     uint8_t* kind_tile = lds_raw + sizeof(T) * (size_t)L::Q * cells;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = blockIdx.x * bx, y0 = blockIdx.y * by;
-    stage_cell<L, T, MASK>(tile, kind_tile, cells, hx, tx + 1, ty + 1, x0, y0, src, kind, g);
+    stage_cell<L, T, MASK>(tile, kind_tile, cells, hx, hy, tx + 1, ty + 1, false, x0, y0, src, kind, g);
     const int ring = 2 * hx + 2 * by;
     for (int h = ty * bx + tx; h < ring; h += bx * by) {
         int ex, ey;
@@ -321,7 +339,7 @@ __global__ __launch_bounds__(1024) void k_lds(const T* __restrict__ src, T* __re
             ex = c < by ? 0 : hx - 1;
             ey = 1 + (c < by ? c : c - by);
         }
-        stage_cell<L, T, MASK>(tile, kind_tile, cells, hx, ex, ey, x0, y0, src, kind, g);
+        stage_cell<L, T, MASK>(tile, kind_tile, cells, hx, hy, ex, ey, true, x0, y0, src, kind, g);
     }
     __syncthreads();
     const int x = x0 + tx, y = y0 + ty;

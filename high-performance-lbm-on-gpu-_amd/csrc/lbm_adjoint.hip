@@ -369,23 +369,147 @@ struct IbmAdjScratch : IbmForward<T> {
 };
 
 template <class T>
-__host__ __device__ inline size_t ibm_adj_layout(T* base, int d, int nn, int nm, int iters, IbmAdjScratch<T>* s)
+__host__ __device__ __forceinline__ size_t ibm_adj_layout(T* base, int d, int nn, int nm, int iters, IbmAdjScratch<T>* s)
 {
     size_t o = ibm_forward_layout<T>(base, d, nn, nm, iters, s);
-    IbmAdjScratch<T> dummy;
-    IbmAdjScratch<T>& v = s ? *s : dummy;
-    auto take = [&](T** p, size_t count) {
-        *p = base + o;
+    auto take = [&](size_t count) {
+        T* p = base + o;
         o += count;
+        return p;
     };
-    take(&v.force_bar, (size_t)d * nn);
-    take(&v.ustar_bar, (size_t)d * nn);
-    take(&v.rho_bar, nn);
-    take(&v.rho_m_bar, nm);
-    take(&v.Fm_bar, (size_t)d * nm);
-    take(&v.U_bar, (size_t)d * nm);
-    take(&v.m_bar, (size_t)(d + 1) * nn);
+    T* force_bar = take((size_t)d * nn);
+    T* ustar_bar = take((size_t)d * nn);
+    T* rho_bar = take(nn);
+    T* rho_m_bar = take(nm);
+    T* Fm_bar = take((size_t)d * nm);
+    T* U_bar = take((size_t)d * nm);
+    T* m_bar = take((size_t)(d + 1) * nn);
+    if (s) {
+        s->force_bar = force_bar;
+        s->ustar_bar = ustar_bar;
+        s->rho_bar = rho_bar;
+        s->rho_m_bar = rho_m_bar;
+        s->Fm_bar = Fm_bar;
+        s->U_bar = U_bar;
+        s->m_bar = m_bar;
+    }
     return o;
+}
+
+// The reverse sweep of the forcing iterations and the moment adjoints m_bar, once the
+// seeds (force_bar = nu, ustar_bar = rho_bar = rho_m_bar = 0) and the forward intermediates
+// are in place behind a barrier. Touches only the workgroup's intermediates and grad_w (LDS
+// when staged), never lam; ends with a barrier.
+//
+// Per iteration, two intervals between barriers:
+//   1. marker lanes (the last lanes of the group, l = nthreads - 1 - tid): Fm_bar = total_bar
+//      + sum_s w_ls force_bar(x_ls) in stencil order, then U_bar and rho_m_bar; entry lanes
+//      (k = l S + s, the first lanes): grad_w[k] += sum_a Fm_a,l force_bar_a(x_k), which
+//      reads force_bar before interval 2 updates it;
+//   2. node lanes: u_bar from the spread lists into ustar_bar, force_bar and rho_bar; entry
+//      lanes: grad_w[k] += sum_a U_bar_a,l u_a(x_k).
+// The float64 weight gradients, the bulk of the arithmetic, thus run one entry per lane over
+// nm 3^d lanes instead of 3^d entries in turn on nm lanes; every sum keeps its order.
+template <class T, int D>
+__device__ __forceinline__ void ibm_adj_reverse(const IbmAdjScratch<T> s, const IbmTables<T> tb, int nn, int nm,
+                                                int iters, const double* __restrict__ body_weight,
+                                                double* __restrict__ grad_w, int tid, int nthreads)
+{
+    constexpr int S = D == 2 ? 9 : 27;
+    const int entries = nm * S;
+    const int back = nthreads - 1 - tid;
+    const T total_bar[3] = {T(-body_weight[0]), T(-body_weight[1]), T(D == 3 ? -body_weight[2] : 0.0)};
+    for (int it = iters - 1; it >= 0; --it) {
+        const T* fo = s.force + (size_t)it * D * nn;
+        const T* u = s.u + (size_t)it * D * nn;
+        const T* U = s.U + (size_t)it * D * nm;
+        const T* Fm = s.Fm + (size_t)it * D * nm;
+        // Interval 1: Fm_bar, U_bar, rho_m_bar per marker.
+        for (int l = back; l < nm; l += nthreads) {
+            int inv[S];
+            T w[S];
+#pragma unroll
+            for (int p = 0; p < S; ++p) {
+                inv[p] = tb.inverse[(size_t)l * S + p];
+                w[p] = tb.weights[(size_t)l * S + p];
+            }
+            T fb[3] = {total_bar[0], total_bar[1], total_bar[2]};
+#pragma unroll
+            for (int p = 0; p < S; ++p) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) fb[a] += w[p] * s.force_bar[a * nn + inv[p]];
+            }
+            const T area = tb.areas[l];
+            T rmb = T(0);
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                s.Fm_bar[a * nm + l] = fb[a];
+                rmb += fb[a] * T(2) * (T(0) - U[a * nm + l]) * area;
+                s.U_bar[a * nm + l] = -fb[a] * T(2) * s.rho_m[l] * area;
+            }
+            s.rho_m_bar[l] += rmb;
+        }
+        // Interval 1: weight gradients of the spread, one stencil entry per lane.
+        for (int k = tid; k < entries; k += nthreads) {
+            const int l = k / S, x = tb.inverse[k];
+            double gw = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) gw += (double)(Fm[a * nm + l] * s.force_bar[a * nn + x]);
+            grad_w[k] += gw;
+        }
+        __syncthreads();
+        // Interval 2: u_bar (one walk of the spread list for all components), then into
+        // ustar_bar, force_bar and rho_bar.
+        for (int n = tid; n < nn; n += nthreads) {
+            T acc[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) acc[a] = T(0);
+            for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
+                const int k = tb.spread_entry[e];
+                const T w = tb.weights[k];
+#pragma unroll
+                for (int a = 0; a < D; ++a) acc[a] += w * s.U_bar[a * nm + k / S];
+            }
+            const T rho = s.rho[n];
+            const T two_rho = T(2) * rho;
+            T rb = s.rho_bar[n];
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                s.ustar_bar[a * nn + n] += acc[a];
+                s.force_bar[a * nn + n] += acc[a] / two_rho;
+                rb -= acc[a] * fo[a * nn + n] / (two_rho * rho);
+            }
+            s.rho_bar[n] = rb;
+        }
+        // Interval 2: weight gradients of the interpolation, one stencil entry per lane.
+        for (int k = back; k < entries; k += nthreads) {
+            const int l = k / S, x = tb.inverse[k];
+            double gw = 0.0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) gw += (double)(s.U_bar[a * nm + l] * u[a * nn + x]);
+            grad_w[k] += gw;
+        }
+        __syncthreads();
+    }
+    // rho_m = sum_s w rho(x_ls): weight gradient (per entry) and, per node, rho_bar and the
+    // moment adjoints m_bar.
+    for (int k = back; k < entries; k += nthreads) grad_w[k] += (double)(s.rho_m_bar[k / S] * s.rho[tb.inverse[k]]);
+    for (int n = tid; n < nn; n += nthreads) {
+        T acc = T(0);
+        for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
+            const int k = tb.spread_entry[e];
+            acc += tb.weights[k] * s.rho_m_bar[k / S];
+        }
+        T rb = s.rho_bar[n] + acc;
+        const T rho = s.rho[n];
+        for (int a = 0; a < D; ++a) {
+            const T usb = s.ustar_bar[a * nn + n];
+            s.m_bar[(1 + a) * nn + n] = usb / rho;
+            rb -= usb * s.ustar[a * nn + n] / rho;
+        }
+        s.m_bar[n] = rb;
+    }
+    __syncthreads();
 }
 
 // One workgroup: the forward intermediates of the step (read from `record` when given,
@@ -398,10 +522,15 @@ __host__ __device__ inline size_t ibm_adj_layout(T* base, int d, int nn, int nm,
 // Byte offsets of the staged variant's LDS (16-byte aligned parts): float64 weight
 // gradients, intermediates, stencil tables and, when they fit, the rule adjoints a_r.
 struct AdjLdsPlan {
-    size_t grad_w, base, weights, areas, inverse, entry, start, rule_a, total;
+    size_t grad_w, base, weights, areas, inverse, entry, start, rule_a, tstart, trow, tcoeff, total;
 };
 
-__host__ __device__ inline AdjLdsPlan adj_lds_plan(int d, int nn, int nm, int iters, size_t real, int lds_rules)
+// Entries of the transposed moment rows per slot: slot (k, y) is read only by the d + 1
+// moment rows of the node y + c_k.
+__host__ __device__ constexpr int adj_row_entries(int d) { return d + 1; }
+
+__host__ __device__ __forceinline__ AdjLdsPlan adj_lds_plan(int d, int nn, int nm, int iters, size_t real, int lds_rules,
+                                                   int lds_trows = 0)
 {
     auto up = [](size_t x) { return (x + 15) & ~(size_t)15; };
     const size_t S = d == 2 ? 9 : 27;
@@ -424,6 +553,14 @@ __host__ __device__ inline AdjLdsPlan adj_lds_plan(int d, int nn, int nm, int it
     o = up(o + (nn + 1) * sizeof(int));
     p.rule_a = o;
     o = up(o + rules * real);
+    const size_t trows = lds_trows;
+    const size_t entries = trows * adj_row_entries(d);
+    p.tstart = o;
+    o = up(o + (trows ? trows + 1 : 0) * sizeof(int));
+    p.trow = o;
+    o = up(o + entries * sizeof(int));
+    p.tcoeff = o;
+    o = up(o + entries * real);
     p.total = o;
     return p;
 }
@@ -435,19 +572,226 @@ __host__ __device__ inline AdjLdsPlan adj_lds_plan(int d, int nn, int nm, int it
 // transposed moment rows (distinct slots with their (row, coeff) lists), and
 // grad_w[l][s] += dL/dw_ls in float64. With n_rules > 0 the transposed link rules are
 // applied first (they act on slots the IBM rows never read).
-template <class T, int D, bool STAGED>
-__device__ __forceinline__ void ibm_adj_group(
-    const T* __restrict__ F, const int64_t* __restrict__ slot, const T* __restrict__ coeff,
-    const T* __restrict__ constant, int m, const int64_t* __restrict__ nodes, int nn, const int* __restrict__ inverse_g,
-    const T* __restrict__ weights_g, const T* __restrict__ areas_g, int nm, const int* __restrict__ start_g,
-    const int* __restrict__ entry_g, int iters, T hfx, T hfy, T hfz, uint32_t n_total,
-    const T* __restrict__ nu, const double* __restrict__ body_weight, T* __restrict__ lam,
-    const int64_t* __restrict__ tslot, const int* __restrict__ tstart, const int* __restrict__ trow,
-    const T* __restrict__ tcoeff, int n_tslots, double* __restrict__ grad_w_g, T* __restrict__ scratch,
-    const int64_t* __restrict__ rtarget, int n_rules, const int64_t* __restrict__ ruslot,
-    const int* __restrict__ rustart, const int* __restrict__ rurule, const T* __restrict__ rucoeff, int n_runique,
-    T* __restrict__ rule_a, int lds_rules, const T* __restrict__ record, int tid, int nthreads)
+#define LBM_IBM_ADJ_PARAMS                                                                                              \
+    const T *__restrict__ F, const int64_t *__restrict__ slot, const T *__restrict__ coeff,                             \
+        const T *__restrict__ constant, int m, const int64_t *__restrict__ nodes, int nn,                               \
+        const int *__restrict__ inverse_g, const T *__restrict__ weights_g, const T *__restrict__ areas_g, int nm,      \
+        const int *__restrict__ start_g, const int *__restrict__ entry_g, int iters, T hfx, T hfy, T hfz,               \
+        uint32_t n_total, const T *__restrict__ nu, const double *__restrict__ body_weight, T *__restrict__ lam,         \
+        const int64_t *__restrict__ tslot, const int *__restrict__ tstart, const int *__restrict__ trow,                \
+        const T *__restrict__ tcoeff, int n_tslots, double *__restrict__ grad_w_g, T *__restrict__ scratch,             \
+        const int64_t *__restrict__ rtarget, int n_rules, const int64_t *__restrict__ ruslot,                           \
+        const int *__restrict__ rustart, const int *__restrict__ rurule, const T *__restrict__ rucoeff, int n_runique,  \
+        T *__restrict__ rule_a, int lds_rules, const T *__restrict__ record, int lds_trows
+#define LBM_IBM_ADJ_ARGS                                                                                                \
+    F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g, nm, start_g, entry_g, iters, hfx, hfy, hfz,  \
+        n_total, nu, body_weight, lam, tslot, tstart, trow, tcoeff, n_tslots, grad_w_g, scratch, rtarget, n_rules,      \
+        ruslot, rustart, rurule, rucoeff, n_runique, rule_a, lds_rules, record, lds_trows
+
+
+// Transposed slots and rule term slots whose values a lane of the prefetching variant reads
+// ahead and keeps in registers across the sweep (the rest go through the generic loops).
+constexpr int ADJ_AHEAD_TSLOTS = 3;
+constexpr int ADJ_AHEAD_RULE_SLOTS = 2;
+constexpr int ADJ_AHEAD_ENTRIES = 3;
+
+// acc = sum over the entries [b, e) of coefficient * x[row], in list order from 0 (the
+// order of RowBatch::apply).
+template <class T>
+__device__ __forceinline__ T row_list_sum(const int* __restrict__ row, const T* __restrict__ coeff, int b, int e,
+                                          const T* __restrict__ x)
 {
+    T acc = T(0);
+    for (int i = b; i < e; ++i) acc += coeff[i] * x[row[i]];
+    return acc;
+}
+
+// The staged IBM adjoint with every global read issued before the first barrier, in two
+// dependent levels: (1) target and node indices, the lane's rule term slots and transposed
+// slots, the entry count, the float64 weight-gradient totals, the stencil tables and the
+// record; (2) lam at the rule targets, the rule term slots and the transposed slots, nu at
+// the nodes, the rule rows' entries, and the transposed rows (staged in LDS). After the
+// barrier the rule adjoint, the sweep and the transposed scatter run on registers and LDS
+// only, and their lam stores go out at the end. Valid when no rule term slot is a rule target
+// or a transposed slot (the host's slots_disjoint flag): every value read ahead is then
+// final before the kernel starts. Float32 only, with the rule adjoints a_r in LDS (float64
+// values would not fit the lanes' registers). Same arithmetic in the same order as
+// ibm_adj_group, so the same bits.
+template <class T, int D>
+__device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int tid, int nthreads)
+{
+    constexpr int S = D == 2 ? 9 : 27;
+    constexpr int KT = ADJ_AHEAD_TSLOTS, KR = ADJ_AHEAD_RULE_SLOTS, E = ADJ_AHEAD_ENTRIES;
+    extern __shared__ __align__(16) unsigned char adj_lds[];
+    const AdjLdsPlan p = adj_lds_plan(D, nn, nm, iters, sizeof(T), lds_rules, lds_trows);
+    double* gw = reinterpret_cast<double*>(adj_lds + p.grad_w);
+    T* base = reinterpret_cast<T*>(adj_lds + p.base);
+    IbmAdjScratch<T> s;
+    ibm_adj_layout<T>(base, D, nn, nm, iters, &s);
+    T* w = reinterpret_cast<T*>(adj_lds + p.weights);
+    T* a = reinterpret_cast<T*>(adj_lds + p.areas);
+    int* inv = reinterpret_cast<int*>(adj_lds + p.inverse);
+    int* ent = reinterpret_cast<int*>(adj_lds + p.entry);
+    int* st = reinterpret_cast<int*>(adj_lds + p.start);
+    int* tst = reinterpret_cast<int*>(adj_lds + p.tstart);
+    int* trw = reinterpret_cast<int*>(adj_lds + p.trow);
+    T* tco = reinterpret_cast<T*>(adj_lds + p.tcoeff);
+    // The rule adjoints a_r are staged too (the host picks this variant only then).
+    rule_a = reinterpret_cast<T*>(adj_lds + p.rule_a);
+    const IbmTables<T> tb{inv, w, a, st, ent};
+    const int gw_count = nm * S;
+
+    // ---- level 1
+    const bool has_rule = tid < n_rules, has_node = tid < nn;
+    const int64_t target0 = has_rule ? rtarget[tid] : 0;
+    const int64_t node0 = has_node ? nodes[tid] : 0;
+    int64_t usl[KR];
+    int ub[KR], ue[KR];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int t = tid + k * nthreads;
+        const bool in = t < n_runique;
+        usl[k] = n_runique > 0 ? ruslot[in ? t : 0] : 0;
+        ub[k] = in ? rustart[t] : 0;
+        ue[k] = in ? rustart[t + 1] : 0;
+    }
+    int64_t tsl[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+        const int t = tid + k * nthreads;
+        tsl[k] = n_tslots > 0 ? tslot[t < n_tslots ? t : 0] : 0;
+    }
+    const int te = tstart[n_tslots];
+    const double gw_old = tid < gw_count ? grad_w_g[tid] : 0.0;
+    for (int e = tid; e < gw_count; e += nthreads) gw[e] = 0.0;
+    {
+        const WordCopy tables[5] = {
+            word_copy(inv, inverse_g, (size_t)nm * S),
+            word_copy(w, weights_g, (size_t)nm * S),
+            word_copy(ent, entry_g, (size_t)nm * S),
+            word_copy(a, areas_g, (size_t)nm),
+            word_copy(st, start_g, (size_t)nn + 1),
+        };
+        // Three 16-byte units per lane: the record of a 2D body goes in one round.
+        const WordCopy rec[1] = {word_copy(base, record, record ? ibm_record_elems(D, nn, nm, iters) : 0)};
+        copy_staged<5, 1, 3>(tables, rec, tid, nthreads);
+    }
+
+    // ---- level 2
+    int rr[KR][E];
+    T rc[KR][E], rold[KR];
+    if (n_runique > 0) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int nv = ue[k] - ub[k] < E ? ue[k] - ub[k] : E;
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const int e = j < nv ? ub[k] + j : 0;
+                rr[k][j] = rurule[e];
+                rc[k][j] = rucoeff[e];
+            }
+            rold[k] = lam[usl[k]];
+        }
+    }
+    const T lam0 = has_rule ? lam[target0] : T(0);
+    T nu0[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) nu0[c] = has_node ? nu[(size_t)c * n_total + node0] : T(0);
+    T told[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) told[k] = n_tslots > 0 ? lam[tsl[k]] : T(0);
+    // The transposed rows go to LDS when their entries fit the plan (always, for moment rows).
+    const bool rows_staged = te <= lds_trows * adj_row_entries(D);
+    {
+        const WordCopy none[1] = {WordCopy{nullptr, nullptr, 0}};
+        const WordCopy rows[3] = {
+            word_copy(tst, tstart, (size_t)n_tslots + 1),
+            word_copy(trw, trow, rows_staged ? (size_t)te : 0),
+            word_copy(tco, tcoeff, rows_staged ? (size_t)te : 0),
+        };
+        copy_staged<1, 3>(none, rows, tid, nthreads);
+    }
+    if (has_rule) {
+        rule_a[tid] = lam0;
+        lam[target0] = T(0);
+    }
+    if (has_node) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            s.force_bar[c * nn + tid] = nu0[c];
+            s.ustar_bar[c * nn + tid] = T(0);
+        }
+        s.rho_bar[tid] = T(0);
+    }
+    for (int r = tid + nthreads; r < n_rules; r += nthreads) {
+        const int64_t t = rtarget[r];
+        rule_a[r] = lam[t];
+        lam[t] = T(0);
+    }
+    for (int n = tid + nthreads; n < nn; n += nthreads) {
+        for (int c = 0; c < D; ++c) {
+            s.force_bar[c * nn + n] = nu[(size_t)c * n_total + nodes[n]];
+            s.ustar_bar[c * nn + n] = T(0);
+        }
+        s.rho_bar[n] = T(0);
+    }
+    for (int l = tid; l < nm; l += nthreads) s.rho_m_bar[l] = T(0);
+    __syncthreads();
+
+    // ---- rule adjoint of the slots read ahead (stored at the end); later slots as usual
+    T rnew[KR] = {};
+    if (n_runique > 0) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            T acc = T(0);
+#pragma unroll
+            for (int j = 0; j < E; ++j)
+                if (j < ue[k] - ub[k]) acc += rc[k][j] * rule_a[rr[k][j]];
+            for (int e = ub[k] + E; e < ue[k]; ++e) acc += rucoeff[e] * rule_a[rurule[e]];
+            rnew[k] = rold[k] + acc;
+        }
+        scatter_rows(lam, ruslot, rustart, rurule, rucoeff, n_runique, rule_a, tid + KR * nthreads, nthreads);
+    }
+    if (!record) {
+        const int rows = (D + 1) * nn;
+        const T hf[3] = {hfx, hfy, hfz};
+        for (int r = tid; r < rows; r += nthreads) s.moments[r] = affine_row(F, slot, coeff, constant, rows, m, r);
+        __syncthreads();
+        ibm_iterate<T, D>(s, tb, nn, nm, iters, true, hf, nullptr, tid, nthreads);
+    }
+    ibm_adj_reverse<T, D>(s, tb, nn, nm, iters, body_weight, gw, tid, nthreads);
+
+    // ---- transposed moment rows (from LDS when staged), then every deferred store
+    const T* m_bar = s.m_bar;
+    auto transposed = [&](const int* __restrict__ rw, const T* __restrict__ cf) {
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            const int t = tid + k * nthreads;
+            if (t < n_tslots) lam[tsl[k]] = told[k] + row_list_sum(rw, cf, tst[t], tst[t + 1], m_bar);
+        }
+        for (int t = tid + KT * nthreads; t < n_tslots; t += nthreads)
+            lam[tslot[t]] = lam[tslot[t]] + row_list_sum(rw, cf, tst[t], tst[t + 1], m_bar);
+    };
+    if (rows_staged) transposed(trw, tco);
+    else transposed(trow, tcoeff);
+    if (n_runique > 0) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k)
+            if (tid + k * nthreads < n_runique) lam[usl[k]] = rnew[k];
+    }
+    if (tid < gw_count) grad_w_g[tid] = gw_old + gw[tid];
+    for (int e = tid + nthreads; e < gw_count; e += nthreads) grad_w_g[e] += gw[e];
+}
+
+template <class T, int D, bool STAGED>
+__device__ __forceinline__ void ibm_adj_group(LBM_IBM_ADJ_PARAMS, int tid, int nthreads)
+{
+    if constexpr (STAGED && sizeof(T) == 4) {
+        if (lds_trows > 0) {
+            ibm_adj_group_prefetched<T, D>(LBM_IBM_ADJ_ARGS, tid, nthreads);
+            return;
+        }
+    }
     constexpr int S = D == 2 ? 9 : 27;
     extern __shared__ __align__(16) unsigned char adj_lds[];
     // Index loads of the first interval go out before the table copies, so their second
@@ -537,127 +881,12 @@ __device__ __forceinline__ void ibm_adj_group(
         __syncthreads();
         ibm_iterate<T, D>(s, tb, nn, nm, iters, true, hf, nullptr, tid, nthreads);
     }
-    // ---- reverse sweep (the rule scatter above touches only lam, which the sweep leaves
-    // alone until the final scatter, several barriers later)
-    const T total_bar[3] = {T(-body_weight[0]), T(-body_weight[1]), T(D == 3 ? -body_weight[2] : 0.0)};
-    for (int it = iters - 1; it >= 0; --it) {
-        const T* fo = s.force + (size_t)it * D * nn;
-        const T* u = s.u + (size_t)it * D * nn;
-        const T* U = s.U + (size_t)it * D * nm;
-        const T* Fm = s.Fm + (size_t)it * D * nm;
-        // Fm_bar from the spread and the total; weight gradients of the spread and of U.
-        for (int l = tid; l < nm; l += nthreads) {
-            int inv[S];
-            T w[S];
-#pragma unroll
-            for (int p = 0; p < S; ++p) {
-                inv[p] = tb.inverse[(size_t)l * S + p];
-                w[p] = tb.weights[(size_t)l * S + p];
-            }
-            T fm[D];
-#pragma unroll
-            for (int a = 0; a < D; ++a) fm[a] = Fm[a * nm + l];
-            T fb[3] = {total_bar[0], total_bar[1], total_bar[2]};
-#pragma unroll
-            for (int p = 0; p < S; ++p) {
-                double gw = 0.0;
-#pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    const T fbar = s.force_bar[a * nn + inv[p]];
-                    fb[a] += w[p] * fbar;
-                    gw += (double)(fm[a] * fbar);
-                }
-                grad_w[l * S + p] += gw;
-            }
-            const T area = tb.areas[l];
-            T rmb = T(0);
-            T ub[D];
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                s.Fm_bar[a * nm + l] = fb[a];
-                rmb += fb[a] * T(2) * (T(0) - U[a * nm + l]) * area;
-                ub[a] = -fb[a] * T(2) * s.rho_m[l] * area;
-                s.U_bar[a * nm + l] = ub[a];
-            }
-            s.rho_m_bar[l] += rmb;
-#pragma unroll
-            for (int p = 0; p < S; ++p) {
-                double gw = 0.0;
-#pragma unroll
-                for (int a = 0; a < D; ++a) gw += (double)(ub[a] * u[a * nn + inv[p]]);
-                grad_w[l * S + p] += gw;
-            }
-        }
-        __syncthreads();
-        // u_bar (one walk of the spread list for all components), then into ustar_bar,
-        // force_bar and rho_bar.
-        for (int n = tid; n < nn; n += nthreads) {
-            T acc[D];
-#pragma unroll
-            for (int a = 0; a < D; ++a) acc[a] = T(0);
-            for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
-                const int k = tb.spread_entry[e];
-                const T w = tb.weights[k];
-#pragma unroll
-                for (int a = 0; a < D; ++a) acc[a] += w * s.U_bar[a * nm + k / S];
-            }
-            const T rho = s.rho[n];
-            const T two_rho = T(2) * rho;
-            T rb = s.rho_bar[n];
-#pragma unroll
-            for (int a = 0; a < D; ++a) {
-                s.ustar_bar[a * nn + n] += acc[a];
-                s.force_bar[a * nn + n] += acc[a] / two_rho;
-                rb -= acc[a] * fo[a * nn + n] / (two_rho * rho);
-            }
-            s.rho_bar[n] = rb;
-        }
-        __syncthreads();
-    }
-    // rho_m = sum_s w rho(x_ls): weight gradient (per marker) and, per node, rho_bar and the
-    // moment adjoints m_bar.
-    for (int l = tid; l < nm; l += nthreads)
-#pragma unroll
-        for (int p = 0; p < S; ++p) {
-            grad_w[l * S + p] += (double)(s.rho_m_bar[l] * s.rho[tb.inverse[l * S + p]]);
-        }
-    for (int n = tid; n < nn; n += nthreads) {
-        T acc = T(0);
-        for (int e = tb.spread_start[n]; e < tb.spread_start[n + 1]; ++e) {
-            const int k = tb.spread_entry[e];
-            acc += tb.weights[k] * s.rho_m_bar[k / S];
-        }
-        T rb = s.rho_bar[n] + acc;
-        const T rho = s.rho[n];
-        for (int a = 0; a < D; ++a) {
-            const T usb = s.ustar_bar[a * nn + n];
-            s.m_bar[(1 + a) * nn + n] = usb / rho;
-            rb -= usb * s.ustar[a * nn + n] / rho;
-        }
-        s.m_bar[n] = rb;
-    }
-    __syncthreads();
+    ibm_adj_reverse<T, D>(s, tb, nn, nm, iters, body_weight, grad_w, tid, nthreads);
     scatter_rows(lam, tslot, tstart, trow, tcoeff, n_tslots, s.m_bar, tid, nthreads);
     if constexpr (STAGED) {
         for (int e = tid; e < nm * S; e += nthreads) grad_w_g[e] += grad_w[e];
     }
 }
-
-#define LBM_IBM_ADJ_PARAMS                                                                                              \
-    const T *__restrict__ F, const int64_t *__restrict__ slot, const T *__restrict__ coeff,                             \
-        const T *__restrict__ constant, int m, const int64_t *__restrict__ nodes, int nn,                               \
-        const int *__restrict__ inverse_g, const T *__restrict__ weights_g, const T *__restrict__ areas_g, int nm,      \
-        const int *__restrict__ start_g, const int *__restrict__ entry_g, int iters, T hfx, T hfy, T hfz,               \
-        uint32_t n_total, const T *__restrict__ nu, const double *__restrict__ body_weight, T *__restrict__ lam,         \
-        const int64_t *__restrict__ tslot, const int *__restrict__ tstart, const int *__restrict__ trow,                \
-        const T *__restrict__ tcoeff, int n_tslots, double *__restrict__ grad_w_g, T *__restrict__ scratch,             \
-        const int64_t *__restrict__ rtarget, int n_rules, const int64_t *__restrict__ ruslot,                           \
-        const int *__restrict__ rustart, const int *__restrict__ rurule, const T *__restrict__ rucoeff, int n_runique,  \
-        T *__restrict__ rule_a, int lds_rules, const T *__restrict__ record
-#define LBM_IBM_ADJ_ARGS                                                                                                \
-    F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g, nm, start_g, entry_g, iters, hfx, hfy, hfz,  \
-        n_total, nu, body_weight, lam, tslot, tstart, trow, tcoeff, n_tslots, grad_w_g, scratch, rtarget, n_rules,      \
-        ruslot, rustart, rurule, rucoeff, n_runique, rule_a, lds_rules, record
 
 template <class T, int D, bool STAGED>
 __global__ __launch_bounds__(ibm_adj_max_threads(D)) void k_ibm_adj(LBM_IBM_ADJ_PARAMS)
@@ -720,16 +949,18 @@ void check_step_args(int force_mode, int mask_mode, const void* kind, const void
     require(force_mode != 2 || body_field, "dense force mode needs a body field");
 }
 
-// Launch shape of the IBM adjoint: lanes, LDS bytes, staging and rule-adjoint placement.
+// Launch shape of the IBM adjoint: lanes, LDS bytes, staging, rule-adjoint placement and
+// (the prefetching variant) the transposed rows staged in LDS.
 struct IbmAdjShape {
     int threads;
     size_t lds;
     bool staged;
     int rules_in_lds;
+    int rows_in_lds;
 };
 
 template <class L, class T>
-IbmAdjShape ibm_adj_shape(int n_nodes, int n_markers, int iterations, int n_rules)
+IbmAdjShape ibm_adj_shape(int n_nodes, int n_markers, int iterations, int n_rules, int n_tslots, bool disjoint)
 {
     const size_t limit = lds_limit();
     const size_t lds_tables = adj_lds_plan(L::D, n_nodes, n_markers, iterations, sizeof(T), 0).total;
@@ -738,6 +969,15 @@ IbmAdjShape ibm_adj_shape(int n_nodes, int n_markers, int iterations, int n_rule
     sh.staged = lds_tables <= limit;
     sh.rules_in_lds = sh.staged && n_rules > 0 && lds_rules <= limit ? n_rules : 0;
     sh.lds = sh.staged ? (sh.rules_in_lds ? lds_rules : lds_tables) : 0;
+    // The prefetching variant (float32): staged with the rule adjoints in LDS, slots
+    // disjoint, and room for the transposed rows.
+    const size_t lds_rows =
+        adj_lds_plan(L::D, n_nodes, n_markers, iterations, sizeof(T), sh.rules_in_lds, n_tslots).total;
+    sh.rows_in_lds = sizeof(T) == 4 && sh.staged && disjoint && n_tslots > 0 && (n_rules == 0 || sh.rules_in_lds) &&
+                             lds_rows <= limit
+                         ? n_tslots
+                         : 0;
+    if (sh.rows_in_lds) sh.lds = lds_rows;
     const int rows = (L::D + 1) * n_nodes;
     sh.threads = std::max(256, std::min(ibm_adj_max_threads(L::D), 64 * ((rows + 63) / 64)));
     return sh;
@@ -869,7 +1109,7 @@ LBM_API int lbm_ibm_adjoint_scratch_size(int lattice, int dtype, int n_nodes, in
         static_cast<T*>(lam), tslot, tstart, trow, static_cast<const T*>(tcoeff), n_tslots, grad_w,                     \
         static_cast<T*>(scratch), rule_target, n_rules, rule_uslot, rule_ustart, rule_urule,                           \
         static_cast<const T*>(rule_ucoeff), n_rule_unique, static_cast<T*>(rule_scratch), sh.rules_in_lds,             \
-        static_cast<const T*>(record)
+        static_cast<const T*>(record), sh.rows_in_lds
 
 #define LBM_IBM_ADJ_CHECK                                                                                              \
     check_ibm_adj_args(m, n_nodes, n_markers, iterations, n_total, n_rules,                                            \
@@ -885,14 +1125,15 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
                             const int* tstart, const int* trow, const void* tcoeff, int n_tslots, double* grad_w,
                             void* scratch, const int64_t* rule_target, int n_rules, const int64_t* rule_uslot,
                             const int* rule_ustart, const int* rule_urule, const void* rule_ucoeff, int n_rule_unique,
-                            void* rule_scratch, const void* record, void* stream)
+                            void* rule_scratch, int slots_disjoint, const void* record, void* stream)
 {
     return guarded("lbm_ibm_adjoint", [&] {
         LBM_IBM_ADJ_CHECK;
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
             using L = decltype(Lt);
             using T = decltype(Tt);
-            const IbmAdjShape sh = ibm_adj_shape<L, T>(n_nodes, n_markers, iterations, n_rules);
+            const IbmAdjShape sh =
+                ibm_adj_shape<L, T>(n_nodes, n_markers, iterations, n_rules, n_tslots, slots_disjoint != 0);
             auto go = [&](auto staged) {
                 constexpr bool ST = decltype(staged)::value;
                 if (ST) allow_lds(k_ibm_adj<T, L::D, ST>, sh.lds);
@@ -913,8 +1154,8 @@ LBM_API int lbm_ibm_adjoint_pull(int lattice, int dtype, const void* f, const in
                                  const int64_t* tslot, const int* tstart, const int* trow, const void* tcoeff,
                                  int n_tslots, double* grad_w, void* scratch, const int64_t* rule_target, int n_rules,
                                  const int64_t* rule_uslot, const int* rule_ustart, const int* rule_urule,
-                                 const void* rule_ucoeff, int n_rule_unique, void* rule_scratch, const void* record,
-                                 const void* pull_src, void* pull_dst, const uint8_t* kind, const void* wall_u,
+                                 const void* rule_ucoeff, int n_rule_unique, void* rule_scratch, int slots_disjoint,
+                                 const void* record, const void* pull_src, void* pull_dst, const uint8_t* kind, const void* wall_u,
                                  const void* force_field, const void* body_field, int nx, int ny, int nz,
                                  double omega_plus, double omega_minus, int force_mode, int mask_mode, void* stream)
 {
@@ -929,7 +1170,8 @@ LBM_API int lbm_ibm_adjoint_pull(int lattice, int dtype, const void* f, const in
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
             using L = decltype(Lt);
             using T = decltype(Tt);
-            const IbmAdjShape sh = ibm_adj_shape<L, T>(n_nodes, n_markers, iterations, n_rules);
+            const IbmAdjShape sh =
+                ibm_adj_shape<L, T>(n_nodes, n_markers, iterations, n_rules, n_tslots, slots_disjoint != 0);
             const Relaxation<T> r = make_relaxation<L, T>(omega_plus, omega_minus, force);
             const int pbx = std::min(256, 64 * ((nx + 63) / 64));
             const int pby = std::max(1, std::min(ny, sh.threads / pbx));

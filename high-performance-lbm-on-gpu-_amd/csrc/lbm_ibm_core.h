@@ -31,6 +31,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "lbm_hold.h"
+
 #include <cstddef>
 #include <cstdint>
 
@@ -102,48 +104,76 @@ __device__ __forceinline__ WordCopy word_copy(T* dst, const U* src, size_t count
                     count * (sizeof(T) / 4)};
 }
 
-template <int N>
-__device__ __forceinline__ void copy_tables(const WordCopy (&c)[N], WordCopy vec, int tid, int nthreads)
+// Copies of N word tables and NV further arrays moved in 16-byte units (KV units of each per
+// lane and round; words that do not fill a unit, or a whole array whose ends are not both
+// 16-byte aligned, go word by word). Every lane loads from an index valid either way (the
+// last element past an array's end) and only the stores are guarded, so all loads of a round
+// go out together and are waited for once: guarded loads compile to one branch and one full
+// wait each. (An empty asm holding the 16-byte values would keep them issued together as
+// well, but it pushed the callers' pointer structs into private memory.)
+template <int N, int NV, int KV = 2>
+__device__ __forceinline__ void copy_staged(const WordCopy (&c)[N], const WordCopy (&vec)[NV], int tid, int nthreads)
 {
     using V4 = uint32_t __attribute__((ext_vector_type(4)));
-    constexpr int KW = 2, KV = 2;
-    const bool aligned = ((reinterpret_cast<uintptr_t>(vec.dst) | reinterpret_cast<uintptr_t>(vec.src)) & 15) == 0;
-    const size_t units = aligned ? vec.words / 4 : 0;
-    const WordCopy rest{vec.dst + units * 4, vec.src + units * 4, vec.words - units * 4};
-    const V4* vsrc = reinterpret_cast<const V4*>(vec.src);
-    V4* vdst = reinterpret_cast<V4*>(vec.dst);
+    constexpr int KW = 2;
     const size_t step = (size_t)nthreads;
-    size_t rounds = (units + KV * step - 1) / (KV * step);
-    size_t most = rest.words;
+    size_t units[NV];
+    WordCopy rest[NV];
+    size_t rounds = 0, most = 0;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        const bool aligned =
+            ((reinterpret_cast<uintptr_t>(vec[j].dst) | reinterpret_cast<uintptr_t>(vec[j].src)) & 15) == 0;
+        units[j] = aligned ? vec[j].words / 4 : 0;
+        rest[j] = WordCopy{vec[j].dst + units[j] * 4, vec[j].src + units[j] * 4, vec[j].words - units[j] * 4};
+        const size_t r = (units[j] + KV * step - 1) / (KV * step);
+        rounds = r > rounds ? r : rounds;
+        most = rest[j].words > most ? rest[j].words : most;
+    }
 #pragma unroll
     for (int j = 0; j < N; ++j) most = c[j].words > most ? c[j].words : most;
     const size_t word_rounds = (most + KW * step - 1) / (KW * step);
     if (word_rounds > rounds) rounds = word_rounds;
+    auto clamp = [](size_t i, size_t n) { return i < n ? i : n - 1; };
     for (size_t r = 0; r < rounds; ++r) {
         const size_t iw = tid + r * KW * step, iv = tid + r * KV * step;
-        uint32_t w[N + 1][KW];
-        V4 v[KV];
+        V4 v[NV][KV];
+        uint32_t w[N + NV][KW];
 #pragma unroll
-        for (int k = 0; k < KV; ++k)
-            if (iv + k * step < units) v[k] = vsrc[iv + k * step];
+        for (int j = 0; j < NV; ++j) {
+            const V4* src = reinterpret_cast<const V4*>(vec[j].src);
 #pragma unroll
-        for (int j = 0; j <= N; ++j) {
-            const WordCopy& cj = j < N ? c[j] : rest;
-#pragma unroll
-            for (int k = 0; k < KW; ++k)
-                if (iw + k * step < cj.words) w[j][k] = cj.src[iw + k * step];
+            for (int k = 0; k < KV; ++k) v[j][k] = units[j] ? src[clamp(iv + k * step, units[j])] : V4{0, 0, 0, 0};
         }
 #pragma unroll
-        for (int k = 0; k < KV; ++k)
-            if (iv + k * step < units) vdst[iv + k * step] = v[k];
+        for (int j = 0; j < N + NV; ++j) {
+            const WordCopy& cj = j < N ? c[j] : rest[j - N];
 #pragma unroll
-        for (int j = 0; j <= N; ++j) {
-            const WordCopy& cj = j < N ? c[j] : rest;
+            for (int k = 0; k < KW; ++k) w[j][k] = cj.words ? cj.src[clamp(iw + k * step, cj.words)] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            V4* dst = reinterpret_cast<V4*>(vec[j].dst);
+#pragma unroll
+            for (int k = 0; k < KV; ++k)
+                if (iv + k * step < units[j]) dst[iv + k * step] = v[j][k];
+        }
+#pragma unroll
+        for (int j = 0; j < N + NV; ++j) {
+            const WordCopy& cj = j < N ? c[j] : rest[j - N];
 #pragma unroll
             for (int k = 0; k < KW; ++k)
                 if (iw + k * step < cj.words) cj.dst[iw + k * step] = w[j][k];
         }
     }
+}
+
+// N word tables and one array in 16-byte units (copy_staged with one unit array).
+template <int N>
+__device__ __forceinline__ void copy_tables(const WordCopy (&c)[N], WordCopy vec, int tid, int nthreads)
+{
+    const WordCopy vecs[1] = {vec};
+    copy_staged<N, 1>(c, vecs, tid, nthreads);
 }
 
 template <class T>
@@ -152,24 +182,28 @@ struct IbmForward {
     T *force, *u, *U, *Fm;
 };
 
+// Parts are carved from `base` in order and stored into *s when s is given. The stores sit
+// behind a test of s, not behind a reference that selects between *s and a dummy: the
+// select would keep the caller's pointer struct in private memory, and every access through
+// it would become a generic (flat) access instead of an LDS one.
 template <class T>
-__host__ __device__ inline size_t ibm_forward_layout(T* base, int d, int nn, int nm, int slots, IbmForward<T>* s)
+__host__ __device__ __forceinline__ size_t ibm_forward_layout(T* base, int d, int nn, int nm, int slots, IbmForward<T>* s)
 {
     size_t o = 0;
-    IbmForward<T> dummy;
-    IbmForward<T>& v = s ? *s : dummy;
-    auto take = [&](T** p, size_t count) {
-        *p = base + o;
+    auto take = [&](size_t count) {
+        T* p = base + o;
         o += count;
+        return p;
     };
-    take(&v.moments, (size_t)(d + 1) * nn);
-    take(&v.rho, nn);
-    take(&v.ustar, (size_t)d * nn);
-    take(&v.rho_m, nm);
-    take(&v.force, (size_t)(slots + 1) * d * nn);
-    take(&v.u, (size_t)slots * d * nn);
-    take(&v.U, (size_t)slots * d * nm);
-    take(&v.Fm, (size_t)slots * d * nm);
+    T* moments = take((size_t)(d + 1) * nn);
+    T* rho = take(nn);
+    T* ustar = take((size_t)d * nn);
+    T* rho_m = take(nm);
+    T* force = take((size_t)(slots + 1) * d * nn);
+    T* u = take((size_t)slots * d * nn);
+    T* U = take((size_t)slots * d * nm);
+    T* Fm = take((size_t)slots * d * nm);
+    if (s) *s = IbmForward<T>{moments, rho, ustar, rho_m, force, u, U, Fm};
     return o;
 }
 

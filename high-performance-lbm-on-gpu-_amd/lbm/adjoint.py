@@ -162,10 +162,16 @@ class DragAdjoint:
         s, torch, dev = self.solver, self._torch, self.solver.device
         pack = s._rule_passes.get("pull")
         self._rules = None
+        self._slots_disjoint = 1
         if pack is None:
             return
         compiled = pack.source
         uslot, ustart, urule, ucoeff = transpose_rows(np.asarray(compiled.slots), np.asarray(compiled.coeff))
+        # Rule term slots that are neither rule targets nor transposed IBM slots let the IBM
+        # adjoint read every value it updates ahead of its first barrier (slots_disjoint).
+        targets = pack.target.cpu().numpy()
+        self._slots_disjoint = int(not (np.intersect1d(uslot, targets).size
+                                        or np.intersect1d(uslot, self._tslot.cpu().numpy()).size))
         self._rules = {
             "target": pack.target,
             "n": pack.n,
@@ -308,7 +314,7 @@ class DragAdjoint:
                         pointer(lam_out), pointer(self._tslot), pointer(self._tstart), pointer(self._trow),
                         pointer(self._tcoeff), len(self._tslot), pointer(self._grad_w), pointer(self._ibm_adj_scratch),
                         pointer(r["target"]), r["n"], pointer(r["uslot"]), pointer(r["ustart"]), pointer(r["urule"]),
-                        pointer(r["ucoeff"]), r["nu"], pointer(r["scratch"]),
+                        pointer(r["ucoeff"]), r["nu"], pointer(r["scratch"]), self._slots_disjoint,
                         None if records is None else pointer(records[j]))
             if with_partner:
                 s._call("lbm_ibm_adjoint_pull", *ibm_args, pointer(buf_p[jp]), pointer(buf_p[jp + 1]),

@@ -53,11 +53,11 @@ SIGNATURES: dict[str, list] = {
                                 _i, _i, _p, _p, _p, _p, _p, _i64, _i, _i, _p],
     "lbm_adj_rules": [_i, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p],
     "lbm_ibm_adjoint_scratch_size": [_i, _i, _i, _i, _i, POINTER(c_int64)],
-    "lbm_ibm_adjoint": [_i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _i, _p, _p, _i, _d, _d, _d, _i64, _p, _p, _p,
-                        _p, _p, _p, _p, _i, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p, _p],
-    "lbm_ibm_adjoint_pull": [_i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _i, _p, _p, _i, _d, _d, _d, _i64, _p, _p,
-                             _p, _p, _p, _p, _p, _i, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p,
-                             _p, _p, _i, _i, _i, _d, _d, _i, _i, _p],
+    "lbm_ibm_adjoint": [_i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _i, _p, _p, _i, _d, _d, _d, _i64, _p, _p,
+                        _p, _p, _p, _p, _p, _i, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p],
+    "lbm_ibm_adjoint_pull": [_i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _i, _p, _p, _i, _d, _d, _d, _i64, _p,
+                             _p, _p, _p, _p, _p, _p, _i, _p, _p, _p, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _p,
+                             _p, _p, _p, _i, _i, _i, _d, _d, _i, _i, _p],
     "lbm_ibm_scratch_size": [_i, _i, _i, _i, POINTER(c_int64)],
     "lbm_ibm_force": [_i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _i, _p, _p, _i, _d, _d, _d, _i64, _p, _p, _p, _p,
                       _p],

@@ -337,7 +337,10 @@ LBM_API int lbm_ibm_adjoint_scratch_size(int lattice, int dtype, int n_nodes, in
  * (distinct slots tslot[n_tslots] with (trow, tcoeff) lists in tstart order) and accumulates
  * dL/dw into grad_w[n_markers][3^d] (double). When n_rules > 0 it first applies the
  * transposed link rules exactly as lbm_adj_rules does (same argument meaning), so one
- * launch covers both sparse adjoints of a step. One workgroup. */
+ * launch covers both sparse adjoints of a step. slots_disjoint = 1 states that no rule term
+ * slot (rule_uslot) is a rule target or a transposed moment slot (tslot): the kernel may then
+ * read every lam value it updates before its first barrier; 0 is always correct. One
+ * workgroup. */
 LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff,
                             const void* constant, int m, const int64_t* nodes, int n_nodes, const int* inverse,
                             const void* weights, const void* areas, int n_markers, const int* spread_start,
@@ -347,7 +350,7 @@ LBM_API int lbm_ibm_adjoint(int lattice, int dtype, const void* f, const int64_t
                             int n_tslots, double* grad_w, void* scratch, const int64_t* rule_target,
                             int n_rules, const int64_t* rule_uslot, const int* rule_ustart,
                             const int* rule_urule, const void* rule_ucoeff, int n_rule_unique,
-                            void* rule_scratch, const void* record, void* stream);
+                            void* rule_scratch, int slots_disjoint, const void* record, void* stream);
 
 /* lbm_ibm_adjoint plus, in the same launch, an independent pull step (as lbm_pull_step, with
  * the uniform force fx, fy, fz of the IBM arguments and an n_total-node grid) from pull_src to
@@ -363,7 +366,8 @@ LBM_API int lbm_ibm_adjoint_pull(int lattice, int dtype, const void* f, const in
                                  const void* tcoeff, int n_tslots, double* grad_w, void* scratch,
                                  const int64_t* rule_target, int n_rules, const int64_t* rule_uslot,
                                  const int* rule_ustart, const int* rule_urule, const void* rule_ucoeff,
-                                 int n_rule_unique, void* rule_scratch, const void* record, const void* pull_src,
+                                 int n_rule_unique, void* rule_scratch, int slots_disjoint, const void* record,
+                                 const void* pull_src,
                                  void* pull_dst, const uint8_t* kind, const void* wall_u, const void* force_field,
                                  const void* body_field, int nx, int ny, int nz, double omega_plus,
                                  double omega_minus, int force_mode, int mask_mode, void* stream);

@@ -322,6 +322,31 @@ def test_fused_adjoint_step_gives_the_same_bits_with_either_pre_pass():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_prefetching_ibm_adjoint_gives_the_same_bits_as_the_generic_kernel(dtype):
+    """The IBM adjoint that reads every value it updates ahead of its first barrier (chosen
+    when no rule term slot is a rule target or a transposed moment slot) and the generic
+    kernel (slots_disjoint = 0) give the same gradients bit for bit, with the link-rule
+    adjoint in the same launch, over three checkpoint segments (so both the plain and the
+    pull-carrying launch run)."""
+    from lbm.adjoint import DragAdjoint
+
+    params = case(dtype=dtype, rules=True)
+    g = np.random.default_rng(3).standard_normal((2, *params.shape)) * 1e-6 * params.fluid
+    grads = []
+    for disjoint in (1, 0):
+        problem = DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params))
+        assert problem._slots_disjoint == 1  # Zou-He rows read neither targets nor IBM slots
+        problem._slots_disjoint = disjoint
+        problem.set_body_force(g)
+        problem.forward()
+        grads.append(problem.backward())
+    for a, b in zip(*grads, strict=True):
+        assert np.array_equal(a, b)
+        assert np.any(a != 0)
+
+
+@pytest.mark.gpu
 def test_full_size_ibm_channel_gradients_match_central_finite_differences():
     """The differentiable channel at its full 512 x 128 size (Zou-He inlet and outlet, walls,
     IBM cylinder R = 12 with 3 multi-direct passes, Re 100) in float64, over 400 steps in

@@ -105,8 +105,8 @@ __global__ __launch_bounds__(ibm_max_threads(D)) void k_ibm(const T* __restrict_
         for (int a = 0; a < D; ++a) field[(size_t)a * n_total + nodes[n]] = last[(size_t)a * nn + n];
     if (tid < D) body_force[tid] = -total[tid];
     if constexpr (STAGED && RECORD) {
-        const WordCopy none[1] = {WordCopy{nullptr, nullptr, 0}};
-        copy_tables(none, word_copy(record, base, ibm_record_elems(D, nn, nm, iterations)), tid, nthreads);
+        const WordCopy out = word_copy(record, base, ibm_record_elems(D, nn, nm, iterations));
+        copy_from_lds(out.dst, out.src, out.words, tid, nthreads);
     }
 }
 

@@ -104,13 +104,19 @@ __device__ __forceinline__ WordCopy word_copy(T* dst, const U* src, size_t count
                     count * (sizeof(T) / 4)};
 }
 
+// Source of the copies of empty arrays: the loads below are issued unconditionally. Never
+// written; not const, so it sits in the global address space like the arrays it stands in for
+// (a constant-space stand-in made every source pointer generic).
+static __device__ __align__(16) uint32_t copy_zero[4] = {0, 0, 0, 0};
+
 // Copies of N word tables and NV further arrays moved in 16-byte units (KV units of each per
 // lane and round; words that do not fill a unit, or a whole array whose ends are not both
 // 16-byte aligned, go word by word). Every lane loads from an index valid either way (the
-// last element past an array's end) and only the stores are guarded, so all loads of a round
-// go out together and are waited for once: guarded loads compile to one branch and one full
-// wait each. (An empty asm holding the 16-byte values would keep them issued together as
-// well, but it pushed the callers' pointer structs into private memory.)
+// last element past an array's end, a zero word for an empty array) and only the stores are
+// guarded, and the first round is not inside the loop, so all loads of a round go out
+// together, also with earlier loads of the caller still in flight: guarded loads compiled to
+// one branch and one full wait each, and a load target written before its load (the value of
+// an empty array) or a loop header made the compiler wait for every earlier load.
 template <int N, int NV, int KV = 2>
 __device__ __forceinline__ void copy_staged(const WordCopy (&c)[N], const WordCopy (&vec)[NV], int tid, int nthreads)
 {
@@ -134,23 +140,39 @@ __device__ __forceinline__ void copy_staged(const WordCopy (&c)[N], const WordCo
     for (int j = 0; j < N; ++j) most = c[j].words > most ? c[j].words : most;
     const size_t word_rounds = (most + KW * step - 1) / (KW * step);
     if (word_rounds > rounds) rounds = word_rounds;
-    auto clamp = [](size_t i, size_t n) { return i < n ? i : n - 1; };
-    for (size_t r = 0; r < rounds; ++r) {
+    const V4* vsrc[NV];
+    size_t vlast[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        vsrc[j] = units[j] ? reinterpret_cast<const V4*>(vec[j].src) : reinterpret_cast<const V4*>(copy_zero);
+        vlast[j] = units[j] ? units[j] - 1 : 0;
+    }
+    const uint32_t* wsrc[N + NV];
+    size_t wlast[N + NV];
+#pragma unroll
+    for (int j = 0; j < N + NV; ++j) {
+        const WordCopy& cj = j < N ? c[j] : rest[j - N];
+        wsrc[j] = cj.words ? cj.src : copy_zero;
+        wlast[j] = cj.words ? cj.words - 1 : 0;
+    }
+    auto round = [&](size_t r) {
         const size_t iw = tid + r * KW * step, iv = tid + r * KV * step;
         V4 v[NV][KV];
         uint32_t w[N + NV][KW];
 #pragma unroll
-        for (int j = 0; j < NV; ++j) {
-            const V4* src = reinterpret_cast<const V4*>(vec[j].src);
+        for (int j = 0; j < NV; ++j)
 #pragma unroll
-            for (int k = 0; k < KV; ++k) v[j][k] = units[j] ? src[clamp(iv + k * step, units[j])] : V4{0, 0, 0, 0};
-        }
+            for (int k = 0; k < KV; ++k) {
+                const size_t i = iv + k * step;
+                v[j][k] = vsrc[j][i < vlast[j] ? i : vlast[j]];
+            }
 #pragma unroll
-        for (int j = 0; j < N + NV; ++j) {
-            const WordCopy& cj = j < N ? c[j] : rest[j - N];
+        for (int j = 0; j < N + NV; ++j)
 #pragma unroll
-            for (int k = 0; k < KW; ++k) w[j][k] = cj.words ? cj.src[clamp(iw + k * step, cj.words)] : 0u;
-        }
+            for (int k = 0; k < KW; ++k) {
+                const size_t i = iw + k * step;
+                w[j][k] = wsrc[j][i < wlast[j] ? i : wlast[j]];
+            }
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
             V4* dst = reinterpret_cast<V4*>(vec[j].dst);
@@ -165,7 +187,20 @@ __device__ __forceinline__ void copy_staged(const WordCopy (&c)[N], const WordCo
             for (int k = 0; k < KW; ++k)
                 if (iw + k * step < cj.words) cj.dst[iw + k * step] = w[j][k];
         }
-    }
+    };
+    if (rounds > 0) round(0);
+    for (size_t r = 1; r < rounds; ++r) round(r);
+}
+
+// Copy of `words` 4-byte words from LDS to global memory, in 16-byte units when both ends
+// are 16-byte aligned (the forward kernel's record).
+__device__ __forceinline__ void copy_from_lds(uint32_t* dst, const uint32_t* src, size_t words, int tid, int nthreads)
+{
+    using V4 = uint32_t __attribute__((ext_vector_type(4)));
+    const bool aligned = ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15) == 0;
+    const size_t units = aligned ? words / 4 : 0;
+    for (size_t i = tid; i < units; i += nthreads) reinterpret_cast<V4*>(dst)[i] = reinterpret_cast<const V4*>(src)[i];
+    for (size_t i = units * 4 + tid; i < words; i += nthreads) dst[i] = src[i];
 }
 
 // N word tables and one array in 16-byte units (copy_staged with one unit array).

@@ -234,6 +234,22 @@ class DragAdjoint:
         s._call("lbm_pull_step", s._lattice_code, s._dtype_code, pointer(src), pointer(dst), pointer(s._kind),
                 s._wall_ptr, pointer(field), pointer(s._body), *s._step_args())
 
+    def _step_forward(self, src, dst, field) -> None:
+        """One forward pull step from ``src`` to ``dst``. With link rules it evaluates them
+        inside the step (``lbm_pull_step_links``): ``dst`` gets the same bits as the rule
+        pre-pass followed by the pull, and only ``src`` keeps its rule targets unwritten. The
+        forward run never reads those slots again (the IBM moment rows read none, and the next
+        step overwrites the buffer), so drag history, records and checkpoints are the same.
+        The backward sweep's recompute keeps the pre-pass: the adjoint step reads the rule
+        values from the states it recomputes."""
+        s = self.solver
+        if s._links is not None and s.block[0] * s.block[1] >= 64:
+            name, args = s._pull_links_call(pointer(src), pointer(dst), s._stream, field=pointer(field))
+            s._call(name, *args)
+            return
+        self._rules_forward(src)
+        self._pull(src, dst, field)
+
     def _forward_core(self, length: int, drag, fields, records=None) -> None:
         """``length`` steps from solver._f (ending there: length is even or a copy closes it).
 
@@ -246,8 +262,7 @@ class DragAdjoint:
             cur, nxt = (a, b) if j % 2 == 0 else (b, a)
             field = s._field if fields is None else fields[j]
             self._ibm_forward(cur, field, drag[j].data_ptr(), None if records is None else records[j])
-            self._rules_forward(cur)
-            self._pull(cur, nxt, field)
+            self._step_forward(cur, nxt, field)
         if length % 2:
             a.copy_(b)
 

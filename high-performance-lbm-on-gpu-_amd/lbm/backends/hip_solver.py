@@ -525,10 +525,12 @@ class HipSolver(Solver):
         return [self._pre_pass_calls("pull", src, stream) + [(name, (L, D, src, dst, kind, wall, field, body, *args))]
                 for src, dst in ((a, b), (b, a))]
 
-    def _pull_links_call(self, src: int, dst: int, stream, force=None, out=None, partial=None) -> tuple:
+    def _pull_links_call(self, src: int, dst: int, stream, force=None, out=None, partial=None,
+                         field: int | None = None) -> tuple:
         """The pull step from ``src`` to ``dst`` with the link rules evaluated in the step and,
         when ``force`` (a force pack) is given, the force of the state at ``src`` into
-        ``out`` (one reduction group) or ``partial`` (several)."""
+        ``out`` (one reduction group) or ``partial`` (several). ``field`` (an address) replaces
+        the solver's IBM force field."""
         p = self.params
         wp, wm = self._rates
         fx, fy, fz = self._force3
@@ -545,8 +547,9 @@ class HipSolver(Solver):
             rows = (pointer(force.slot), pointer(force.coeff), pointer(force.constant), pointer(force.weight),
                     force.n, force.m, out, partial)
         return ("lbm_pull_step_links", (
-            self._lattice_code, self._dtype_code, src, dst, pointer(self._kind), self._wall_ptr, pointer(self._field),
-            pointer(self._body), p.nx, p.ny, self._nz, wp, wm, fx, fy, fz, self._force_mode, *rules, *rows, bx, by,
+            self._lattice_code, self._dtype_code, src, dst, pointer(self._kind), self._wall_ptr,
+            pointer(self._field) if field is None else field, pointer(self._body), p.nx, p.ny, self._nz, wp, wm, fx,
+            fy, fz, self._force_mode, *rules, *rows, bx, by,
             self._groups_per_cu, stream,
         ))
 

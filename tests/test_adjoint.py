@@ -323,6 +323,30 @@ def test_fused_adjoint_step_gives_the_same_bits_with_either_pre_pass():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_forward_with_rules_in_the_step_gives_the_same_drag_and_gradients(dtype):
+    """The forward run evaluates the link rules inside the pull step (one launch per step
+    after the IBM) instead of a pre-pass launch: drag history, loss and gradients are the
+    same bits as with the pre-pass and the plain pull."""
+    from lbm.adjoint import DragAdjoint
+
+    params = case(dtype=dtype, rules=True)
+    g = np.random.default_rng(5).standard_normal((2, *params.shape)) * 1e-6 * params.fluid
+    runs = []
+    for in_step in (True, False):
+        problem = DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params))
+        assert problem.solver._links is not None
+        if not in_step:
+            problem.solver._links = None
+        problem.set_body_force(g)
+        loss = problem.forward()
+        runs.append((loss, problem.drag_history, *problem.backward()))
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1:], runs[1][1:], strict=True):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_prefetching_ibm_adjoint_gives_the_same_bits_as_the_generic_kernel(dtype):
     """The IBM adjoint that reads every value it updates ahead of its first barrier (chosen
     when no rule term slot is a rule target or a transposed moment slot) and the generic

@@ -15,9 +15,11 @@
 #include "lbm_affine.h"
 #include "lbm_common.h"
 #include "lbm_ibm_core.h"
+#include "lbm_rule_layer.h"
 #include "lbm_step.h"
 
 #include <initializer_list>
+#include <type_traits>
 
 namespace lbm {
 
@@ -115,8 +117,9 @@ __device__ __forceinline__ uint32_t nbr(int x, int y, int z, const Grid& g)
     return ((uint32_t)zz * (uint32_t)g.ny + (uint32_t)yy) * (uint32_t)g.nx + (uint32_t)xx;
 }
 
-// The adjoint update of node (x, y, z) (inside the grid).
-template <class L, class T, ForceMode FM, bool MASK>
+// The adjoint update of node (x, y, z) (inside the grid). SKIP_REGION leaves the nodes marked
+// KIND_ADJ_REGION to the fused step's region workgroup.
+template <class L, class T, ForceMode FM, bool MASK, bool SKIP_REGION = false>
 __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T* __restrict__ lam_out,
                                               const T* __restrict__ f, const uint8_t* __restrict__ kind,
                                               const T* __restrict__ wall_u, const T* __restrict__ field,
@@ -141,7 +144,7 @@ __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T*
         hold_loaded(fs);
         hold_loaded(lam);
     }
-    if (MASK && (k & KIND_SOLID)) return;
+    if (MASK && (k & (SKIP_REGION ? (KIND_SOLID | KIND_ADJ_REGION) : KIND_SOLID))) return;
     uint32_t blocked = 0;
     if (MASK && (k & KIND_NEAR_SOLID)) {
         uint32_t nk[L::Q];  // all neighbour kinds read together, held, then tested
@@ -916,6 +919,102 @@ __global__ __launch_bounds__(ibm_adj_max_threads(L::D)) void k_ibm_adj_pull(
     if (c.x < g.nx && c.y < g.ny) pull_update<L, T, FM, MASK>(src, dst, kind, wall_u, field, body, g, r, c);
 }
 
+// ------------------------------------------------------------------ fused backward step
+//
+// One backward step of the pipelined sweep in one launch (float32, the prefetching IBM
+// adjoint; a one-dimensional grid of workgroups of bx x by <= 1024 lanes):
+//   workgroup 0        the IBM region: the adjoint step of the IBM stencil nodes (listed in
+//                      `region`), a barrier, then the IBM adjoint (ibm_adj_group_prefetched
+//                      without rules), whose lam reads find values this workgroup has just
+//                      written;
+//   workgroup 1        (with link rules) the rule region: the adjoint step of the writers of
+//                      every rule target and rule term slot (`rule_region`), a barrier, then
+//                      the transposed link rules (as lbm_adj_rules);
+//   next `chunks`      the partner (forward recompute) step's rule chunks, with write-back;
+//   next ptiles        the partner step's other nodes (pull_update, rule nodes left out);
+//   last atiles        the adjoint step of every node outside the regions.
+// Every lam slot of a fluid node is written by exactly one node's push; the slots the IBM
+// adjoint reads or updates are written by IBM-region nodes and those of the rule adjoint by
+// rule-region nodes (the host keeps the two regions and slot sets apart), so neither region
+// workgroup races with another workgroup; the partner step works on other buffers. The
+// operations are those of the two-launch form (adjoint step with its riding pre-pass, then
+// the IBM and rule adjoints with the riding pull), so the bits are the same.
+template <class L, class T, ForceMode FM>
+__global__ __launch_bounds__(1024) void k_adj_fused(LBM_IBM_ADJ_PARAMS, const T* __restrict__ lam_next,
+                                                    const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
+                                                    const T* __restrict__ field, const T* __restrict__ body,
+                                                    double* __restrict__ grad_body,
+                                                    const int32_t* __restrict__ region, int n_region,
+                                                    const int32_t* __restrict__ rule_region, int n_rule_region,
+                                                    Grid g, Relaxation<T> r, int atiles_x, int atiles_y,
+                                                    const T* __restrict__ p_src, T* __restrict__ p_dst,
+                                                    const T* __restrict__ p_field, RuleRows<T> p_rules, int ptiles_x,
+                                                    int ptiles_y, int ptiles)
+{
+    const int b = (int)blockIdx.x;
+    const int lanes = (int)(blockDim.x * blockDim.y);
+    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    T* nu_out = const_cast<T*>(nu);  // the IBM region writes nu, then the IBM adjoint reads it
+    auto region_step = [&](const int32_t* __restrict__ list, int count) {
+        for (int q = tid; q < count; q += lanes) {
+            const uint32_t node = (uint32_t)list[q];
+            const uint32_t rest = node / (uint32_t)g.nx;
+            adj_pull_node<L, T, FM, true>(lam_next, lam, F, kind, wall_u, field, body, grad_body, nu_out, g, r,
+                                          (int)(node - rest * (uint32_t)g.nx), (int)(rest % (uint32_t)g.ny),
+                                          (int)(rest / (uint32_t)g.ny));
+        }
+        __syncthreads();
+    };
+    const int specials = n_rules > 0 ? 2 : 1;
+    if (b == 0) {
+        region_step(region, n_region);
+        // The IBM adjoint alone: workgroup 1 runs the transposed link rules.
+        ibm_adj_group_prefetched<T, L::D>(F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g, nm,
+                                          start_g, entry_g, iters, hfx, hfy, hfz, n_total, nu, body_weight, lam,
+                                          tslot, tstart, trow, tcoeff, n_tslots, grad_w_g, scratch, nullptr, 0,
+                                          nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0, record, lds_trows, tid,
+                                          lanes);
+        return;
+    }
+    if (b == 1 && specials == 2) {
+        region_step(rule_region, n_rule_region);
+        extern __shared__ __align__(16) unsigned char adj_lds[];
+        T* a = reinterpret_cast<T*>(adj_lds);
+        for (int q = tid; q < n_rules; q += lanes) {
+            const int64_t t = rtarget[q];
+            a[q] = lam[t];
+            lam[t] = T(0);
+        }
+        __syncthreads();
+        scatter_rows(lam, ruslot, rustart, rurule, rucoeff, n_runique, a, tid, lanes);
+        return;
+    }
+    const int chunks = p_rules.count ? rule_chunk_count(p_rules.count, p_rules.chunk) : 0;
+    int t = b - specials;
+    if (t < chunks) {
+        extern __shared__ double dynamic_lds[];
+        rule_chunk<L, T, FM, true>(p_src, p_dst, kind, wall_u, p_field, body, g, r, p_rules, t, lanes, tid,
+                                   reinterpret_cast<T*>(dynamic_lds));
+        return;
+    }
+    t -= chunks;
+    if (t < ptiles) {
+        Coord c;
+        c.x = (t % ptiles_x) * (int)blockDim.x + (int)threadIdx.x;
+        c.y = ((t / ptiles_x) % ptiles_y) * (int)blockDim.y + (int)threadIdx.y;
+        c.z = t / (ptiles_x * ptiles_y);
+        if (c.x < g.nx && c.y < g.ny) pull_update<L, T, FM, true, true>(p_src, p_dst, kind, wall_u, p_field, body, g, r, c);
+        return;
+    }
+    t -= ptiles;
+    const int x = (t % atiles_x) * (int)blockDim.x + (int)threadIdx.x;
+    const int y = ((t / atiles_x) % atiles_y) * (int)blockDim.y + (int)threadIdx.y;
+    const int z = t / (atiles_x * atiles_y);
+    if (x < g.nx && y < g.ny)
+        adj_pull_node<L, T, FM, true, true>(lam_next, lam, F, kind, wall_u, field, body, grad_body, nu_out, g, r, x,
+                                            y, z);
+}
+
 #undef LBM_IBM_ADJ_PARAMS
 #undef LBM_IBM_ADJ_ARGS
 
@@ -1193,6 +1292,123 @@ LBM_API int lbm_ibm_adjoint_pull(int lattice, int dtype, const void* f, const in
                 else go(std::false_type{});
             });
             check_launch("IBM adjoint with pull step");
+        });
+    });
+}
+
+namespace {
+
+// The fused step's workgroup size bound and when it applies: float32 D2Q9, blocks of 64 ..
+// 1024 lanes, and the prefetching IBM adjoint (slots disjoint, rows and rule adjoints staged).
+bool adj_fused_ok(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int n_rules, int n_tslots,
+                  int slots_disjoint, int bx, int by)
+{
+    if (lattice != LBM_D2Q9 || dtype != LBM_F32 || bx <= 0 || by <= 0) return false;
+    const int lanes = bx * by;
+    if (lanes < RULE_CHUNK_MAX || lanes > 1024 || n_nodes <= 0 || n_markers <= 0 || iterations < 1) return false;
+    const IbmAdjShape sh = ibm_adj_shape<D2Q9, float>(n_nodes, n_markers, iterations, 0, n_tslots,
+                                                      slots_disjoint != 0);
+    return sh.rows_in_lds > 0 && (size_t)n_rules * sizeof(float) <= lds_limit();
+}
+
+}  // namespace
+
+LBM_API int lbm_adj_fused_supported(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int n_rules,
+                                    int n_tslots, int slots_disjoint, int bx, int by, int* ok)
+{
+    return guarded("lbm_adj_fused_supported", [&] {
+        require(ok != nullptr, "null output");
+        *ok = adj_fused_ok(lattice, dtype, n_nodes, n_markers, iterations, n_rules, n_tslots, slots_disjoint, bx, by)
+                  ? 1
+                  : 0;
+    });
+}
+
+LBM_API int lbm_adj_fused_step(
+    int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff, const void* constant, int m,
+    const int64_t* nodes, int n_nodes, const int* inverse, const void* weights, const void* areas, int n_markers,
+    const int* spread_start, const int* spread_entry, int iterations, double fx, double fy, double fz, int64_t n_total,
+    void* nu, const double* body_weight, void* lam, const int64_t* tslot, const int* tstart, const int* trow,
+    const void* tcoeff, int n_tslots, double* grad_w, const int64_t* rule_target, int n_rules,
+    const int64_t* rule_uslot, const int* rule_ustart, const int* rule_urule, const void* rule_ucoeff,
+    int n_rule_unique, const void* record, const void* lam_next, const uint8_t* kind, const void* wall_u,
+    const void* force_field, const void* body_field, int nx, int ny, int nz, double omega_plus, double omega_minus,
+    double* grad_body, const int32_t* region, int n_region, const int32_t* rule_region, int n_rule_region, int bx,
+    int by, const void* pull_src, void* pull_dst, const void* pull_field, const uint32_t* rule_node_info,
+    const int32_t* rule_node_first, int rule_nodes, const int64_t* prule_slot, const void* prule_coeff,
+    const void* prule_constant, int64_t n_prules, int prule_m, void* stream)
+{
+    return guarded("lbm_adj_fused_step", [&] {
+        require(adj_fused_ok(lattice, dtype, n_nodes, n_markers, iterations, n_rules, n_tslots, 1, bx, by),
+                "the fused adjoint step takes float32 D2Q9, 64 .. 1024 lanes and a body whose IBM adjoint stages "
+                "its tables, rule adjoints and transposed rows in LDS (lbm_adj_fused_supported)");
+        check_ibm_adj_args(m, n_nodes, n_markers, iterations, n_total, n_rules,
+                           {f, slot, coeff, constant, nodes, inverse, weights, areas, spread_start, spread_entry, nu,
+                            body_weight, lam, tslot, tstart, trow, tcoeff, grad_w},
+                           {rule_target, rule_uslot, rule_ustart, rule_urule, rule_ucoeff});
+        const Grid g = make_grid(lattice, nx, ny, nz);
+        require((int64_t)g.n == n_total, "the grid must have n_total nodes");
+        require(lam_next && kind && body_field && force_field && lam_next != lam, "null or aliased adjoint buffers");
+        require(n_region >= 0 && (n_region == 0 || region), "null region list");
+        require(n_rule_region >= 0 && (n_rule_region == 0 || rule_region), "null rule region list");
+        const bool partner = pull_src != nullptr;
+        if (partner) {
+            require(pull_dst && pull_field && pull_src != pull_dst && pull_src != f && pull_dst != f,
+                    "partner buffers must be distinct and non-null");
+            require(n_prules >= 0 && n_prules < (1ll << 31) && rule_nodes >= 0 && (n_prules > 0) == (rule_nodes > 0),
+                    "partner rules need rule nodes and nodes rows");
+            require(n_prules == 0 || (rule_node_info && rule_node_first && prule_slot && prule_coeff &&
+                                      prule_constant && prule_m >= 1 && prule_m <= 128),
+                    "partner link rules need the node list, slot, coeff, constant and 1 <= m <= 128");
+            require(reinterpret_cast<uintptr_t>(rule_node_info) % 16 == 0, "rule_node_info must be 16-byte aligned");
+        }
+        const double force[3] = {fx, fy, fz};
+        const dim3 block(bx, by, 1);
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            if constexpr (std::is_same<T, float>::value && L::D == 2) {
+                constexpr ForceMode FM = ForceMode::DENSE;
+                // The IBM adjoint runs without the rules (workgroup 1 takes them), whose adjoints
+                // a_r go to the start of workgroup 1's LDS.
+                const IbmAdjShape sh = ibm_adj_shape<L, T>(n_nodes, n_markers, iterations, 0, n_tslots, true);
+                const Relaxation<T> r = make_relaxation<L, T>(omega_plus, omega_minus, force);
+                const dim3 tiles = grid_of(g, block);
+                const int tx = (int)tiles.x, ty = (int)tiles.y;
+                const int64_t n_tiles = (int64_t)tiles.x * tiles.y * tiles.z;
+                int chunk = RULE_CHUNK_MIN;
+                int chunks = 0;
+                if (partner && rule_nodes > 0) {
+                    chunk = rule_chunk_nodes(rule_nodes, (int)std::min<int64_t>(n_tiles, 1 << 20));
+                    chunks = rule_chunk_count(rule_nodes, chunk);
+                }
+                const RuleRows<T> rr{reinterpret_cast<const uint4*>(rule_node_info), rule_node_first,
+                                     partner ? rule_nodes : 0, chunk, prule_slot, static_cast<const T*>(prule_coeff),
+                                     static_cast<const T*>(prule_constant), partner ? n_prules : 0, prule_m};
+                const int64_t ptiles = partner ? n_tiles : 0;
+                const int64_t blocks = (n_rules > 0 ? 2 : 1) + chunks + ptiles + n_tiles;
+                require(blocks < (1ll << 31), "grid too large");
+                const size_t lds = std::max({sh.lds, chunks ? sizeof(T) * (size_t)chunk * (L::Q - 1) : (size_t)0,
+                                             sizeof(T) * (size_t)n_rules});
+                require(lds <= lds_limit(), "the rule adjoints do not fit the workgroup's LDS");
+                allow_lds(k_adj_fused<L, T, FM>, lds);
+                k_adj_fused<L, T, FM><<<(unsigned)blocks, block, lds, as_stream(stream)>>>(
+                    static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), m,
+                    nodes, n_nodes, inverse, static_cast<const T*>(weights), static_cast<const T*>(areas), n_markers,
+                    spread_start, spread_entry, iterations, T(0.5 * fx), T(0.5 * fy), T(0.5 * fz), (uint32_t)n_total,
+                    static_cast<const T*>(nu), body_weight, static_cast<T*>(lam), tslot, tstart, trow,
+                    static_cast<const T*>(tcoeff), n_tslots, grad_w, nullptr, rule_target, n_rules, rule_uslot,
+                    rule_ustart, rule_urule, static_cast<const T*>(rule_ucoeff), n_rule_unique, nullptr,
+                    sh.rules_in_lds, static_cast<const T*>(record), sh.rows_in_lds, static_cast<const T*>(lam_next),
+                    kind, static_cast<const T*>(wall_u), static_cast<const T*>(force_field),
+                    static_cast<const T*>(body_field), grad_body, region, n_region, rule_region, n_rule_region, g, r,
+                    tx, ty,
+                    static_cast<const T*>(pull_src), static_cast<T*>(pull_dst), static_cast<const T*>(pull_field), rr,
+                    tx, ty, (int)ptiles);
+            } else {
+                require(false, "the fused adjoint step takes float32 D2Q9");
+            }
+            check_launch("fused adjoint step");
         });
     });
 }

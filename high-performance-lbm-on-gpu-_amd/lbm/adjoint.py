@@ -32,8 +32,10 @@ import ctypes
 
 import numpy as np
 
+from lbm.backends import graph_capture
 from lbm.backends._native import FORCE_DENSE, pointer
 from lbm.backends.hip_solver import HipSolver
+from lbm.geometry import neighbour_index
 from lbm.ibm import STENCIL, roma, roma_derivative
 from lbm.link_rules import compile_rules
 
@@ -131,6 +133,7 @@ class DragAdjoint:
         self._nu = torch.zeros((d, n), dtype=tdt, device=dev)
         self._setup_ibm_adjoint()
         self._setup_rule_adjoint()
+        self._setup_fused_step()
         self._graphs = {}
         self.set_initial_state(*self.init)
 
@@ -182,6 +185,46 @@ class DragAdjoint:
             "nu": len(uslot),
             "scratch": torch.zeros(max(1, pack.n), dtype=s._torch_dtype, device=dev),
         }
+
+    def _setup_fused_step(self) -> None:
+        """The one-launch backward step (``lbm_adj_fused_step``) when the library takes this
+        problem and the forward recompute can run its link rules inside the pull step (a rule
+        set in which no rule reads another's target). Its regions: the writers of the slots the
+        IBM adjoint reads or updates (the stencil nodes) and those of the rule adjoint's slots,
+        kept apart. Slot (k, y) of a fluid node is written by the push of node y + c_k when that
+        node is fluid, else by y's own bounce-back."""
+        s, torch = self.solver, self._torch
+        self._fused = None
+        ibm, r = s._ibm, self._rules
+        pack = s._rule_passes.get("pull")
+        if pack is not None and (s._links is None or not pack.direct):
+            return
+        bx, by = s.block
+        ok = ctypes.c_int()
+        s._lib.call("lbm_adj_fused_supported", s._lattice_code, s._dtype_code, ibm.n_nodes, ibm.n_markers,
+                    ibm.iterations, r["n"] if r else 0, len(self._tslot), self._slots_disjoint, bx, by,
+                    ctypes.byref(ok))
+        if not ok.value:
+            return
+        p, n = s.params, s._n
+        solid = np.zeros(n, dtype=bool) if p.solid is None else np.asarray(p.solid, dtype=bool).ravel()
+
+        def writers(slots):
+            k, y = slots // n, slots % n
+            downstream = neighbour_index(y, tuple(np.asarray(p.lattice.c)[k].T), p.shape)
+            return np.unique(np.where(solid[downstream], y, downstream))
+
+        region = writers(self._tslot.cpu().numpy())
+        rule_region = np.zeros(0, dtype=np.int64)
+        if r is not None:
+            rule_region = writers(np.concatenate([r["uslot"].cpu().numpy(), r["target"].cpu().numpy()]))
+        both = np.concatenate([region, rule_region])
+        if solid[both].any() or np.intersect1d(region, rule_region).size:
+            return
+        kind = s._kind.clone()
+        kind[torch.as_tensor(both, device=s.device)] |= 16  # KIND_ADJ_REGION
+        self._fused = {"region": torch.as_tensor(region.astype(np.int32), device=s.device),
+                       "rule_region": torch.as_tensor(rule_region.astype(np.int32), device=s.device), "kind": kind}
 
     # ------------------------------------------------------------------ controls
 
@@ -311,6 +354,10 @@ class DragAdjoint:
                 buf_p, fields_p, _ = partner
                 if not self.store_fields:
                     self._ibm_forward(buf_p[jp], fields_p[jp], self._bf_scratch.data_ptr())
+            if self._fused is not None:
+                self._fused_step(buf[j], fields[j], None if records is None else records[j], weights[j], lam_next,
+                                 lam_out, (partner[0][jp], partner[0][jp + 1], partner[1][jp]) if with_partner else None)
+                continue
             adj = (L, D, pointer(lam_next), pointer(lam_out), pointer(buf[j]), pointer(s._kind), s._wall_ptr,
                    pointer(fields[j]), pointer(s._body), nx, ny, nz, wp, wm, fx, fy, fz, FORCE_DENSE, s._mask_mode,
                    pointer(self._grad_body), pointer(self._nu), bx, by)
@@ -340,6 +387,39 @@ class DragAdjoint:
         if partner is not None and partner[2] > length:
             self._recompute(partner[0], partner[1], length, partner[2])
 
+    def _fused_step(self, f, field, record, weight, lam_next, lam_out, partner) -> None:
+        """One backward step in one launch; ``partner`` = (source, destination, field) of the
+        forward recompute step riding along, or None."""
+        s = self.solver
+        ibm, mp, r, fz = s._ibm, self._mp, self._rules, self._fused
+        fx, fy, fz_ = s._force3
+        wp, wm = s._rates
+        bx, by = s.block
+        if r is None:
+            rules = (None, 0, None, None, None, None, 0)
+        else:
+            rules = (pointer(r["target"]), r["n"], pointer(r["uslot"]), pointer(r["ustart"]), pointer(r["urule"]),
+                     pointer(r["ucoeff"]), r["nu"])
+        if partner is not None and r is not None and s._links is None:
+            raise RuntimeError("the fused backward step needs the solver's in-step link rules for the recompute")
+        if partner is None or s._links is None:
+            links = (None, None, 0, None, None, None, 0, 1)
+        else:
+            lk = s._links
+            links = (pointer(lk.node_info), pointer(lk.node_first), lk.nodes, pointer(lk.slot), pointer(lk.coeff),
+                     pointer(lk.constant), lk.n, lk.m)
+        src, dst, pfield = (None, None, None) if partner is None else tuple(pointer(t) for t in partner)
+        s._call("lbm_adj_fused_step", s._lattice_code, s._dtype_code, pointer(f), pointer(mp.slot), pointer(mp.coeff),
+                pointer(mp.constant), mp.m, pointer(s._ibm_nodes), ibm.n_nodes, pointer(s._ibm_inverse),
+                pointer(s._ibm_weights), pointer(s._ibm_areas), ibm.n_markers, pointer(s._ibm_start),
+                pointer(s._ibm_entry), ibm.iterations, fx, fy, fz_, s._n, pointer(self._nu), weight.data_ptr(),
+                pointer(lam_out), pointer(self._tslot), pointer(self._tstart), pointer(self._trow),
+                pointer(self._tcoeff), len(self._tslot), pointer(self._grad_w), *rules,
+                None if record is None else pointer(record), pointer(lam_next), pointer(fz["kind"]), s._wall_ptr,
+                pointer(field), pointer(s._body), s.params.nx, s.params.ny, s._nz, wp, wm, pointer(self._grad_body),
+                pointer(fz["region"]), len(fz["region"]), pointer(fz["rule_region"]), len(fz["rule_region"]), bx, by,
+                src, dst, pfield, *links, s._stream)
+
     @property
     def adjoint_block(self) -> tuple[int, int]:
         bx = min(256, 64 * -(-self.params.nx // 64))
@@ -366,7 +446,7 @@ class DragAdjoint:
             launch()  # eager warm-up: loads every kernel before capture
             torch.cuda.synchronize()
             graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(graph):
+            with graph_capture(torch, graph):
                 launch()
             self._graphs[key] = graph
             return

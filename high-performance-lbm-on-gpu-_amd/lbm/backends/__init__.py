@@ -16,7 +16,9 @@ The NumPy restatement of the reference solver lives in the repository's
 
 from __future__ import annotations
 
+import contextlib
 import functools
+import gc
 
 BACKENDS = ("hip-pull", "hip-aa", "hip-lds")
 ALIASES = {"cuda-naive": "hip-pull", "cuda-aa": "hip-aa", "cuda-tiled": "hip-lds"}
@@ -34,6 +36,23 @@ def _require_device():
     if not torch.cuda.is_available():
         raise BackendUnavailable("no ROCm GPU is visible (torch.cuda.is_available() is False)")
     return torch
+
+
+@contextlib.contextmanager
+def graph_capture(torch, graph):
+    """``torch.cuda.graph(graph)`` with Python's cyclic garbage collector off until the
+    capture has ended. ``torch.cuda.graph`` collects garbage before it starts capturing; a
+    collection that an allocation triggers inside the capture can run the destructor of
+    another object's HIP graph, a call that stream capture does not allow, and the process
+    aborts."""
+    enabled = gc.isenabled()
+    gc.disable()
+    try:
+        with torch.cuda.graph(graph):
+            yield
+    finally:
+        if enabled:
+            gc.enable()
 
 
 def load_library():

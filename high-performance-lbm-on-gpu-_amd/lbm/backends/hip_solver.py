@@ -24,7 +24,7 @@ import math
 
 import numpy as np
 
-from lbm.backends import BackendUnavailable, _require_device, load_library
+from lbm.backends import BackendUnavailable, _require_device, graph_capture, load_library
 from lbm.backends._native import DTYPE_CODES, FORCE_DENSE, FORCE_FIELD, FORCE_UNIFORM, pointer
 from lbm.diagnostics import MacroscopicFields
 from lbm.geometry import neighbour_index
@@ -484,7 +484,7 @@ class HipSolver(Solver):
             self._advance_eager(2)
             torch.cuda.synchronize()
             graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(graph):
+            with graph_capture(torch, graph):
                 self._advance_eager(2)
             self._graphs[key] = graph
         else:

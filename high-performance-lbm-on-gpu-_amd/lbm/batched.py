@@ -22,7 +22,7 @@ import ctypes
 
 import numpy as np
 
-from lbm.backends import _require_device, load_library
+from lbm.backends import _require_device, graph_capture, load_library
 from lbm.backends._native import DTYPE_CODES, pointer
 from lbm.backends.hip_solver import check_block, node_kinds
 from lbm.link_rules import compile_rules, reads_other_targets
@@ -226,7 +226,7 @@ class BatchedSolver:
                 self._pair()
                 torch.cuda.synchronize()
                 graph = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(graph):
+                with graph_capture(torch, graph):
                     self._pair()
                 self._graph = graph
             else:

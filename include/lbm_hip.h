@@ -372,6 +372,40 @@ LBM_API int lbm_ibm_adjoint_pull(int lattice, int dtype, const void* f, const in
                                  const void* body_field, int nx, int ny, int nz, double omega_plus,
                                  double omega_minus, int force_mode, int mask_mode, void* stream);
 
+/* Whether lbm_adj_fused_step takes this problem: float32 D2Q9, bx * by in 64 .. 1024, and an
+ * IBM adjoint that stages its tables, rule adjoints and transposed rows in LDS with
+ * slots_disjoint = 1 (*ok = 1). No counterpart in the reference. Host only. */
+LBM_API int lbm_adj_fused_supported(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int n_rules,
+                                    int n_tslots, int slots_disjoint, int bx, int by, int* ok);
+
+/* One backward step of the checkpointed adjoint in one launch (no counterpart in the
+ * reference): lbm_adj_pull_step (force mode dense, with kind bytes) from lam_next into lam,
+ * the IBM and link-rule adjoints of lbm_ibm_adjoint (same arguments; scratch buffers not
+ * needed, slots_disjoint = 1 required), and, when pull_src is not null, an independent
+ * forward step from pull_src to pull_dst with the link rules evaluated inside it (as
+ * lbm_pull_step_links, force field pull_field) that also writes each rule's value into the
+ * slot of pull_src the rule pre-pass would have written. `region` lists the nodes whose
+ * adjoint step the IBM workgroup runs itself (the writers of the transposed moment slots:
+ * the IBM stencil nodes), `rule_region` those of the workgroup that runs the transposed link
+ * rules (the writers of every rule target and rule term slot; each slot (k, y) of a fluid
+ * node has one writer, the node y + c_k when that node is fluid, else y). The two lists and
+ * their slot sets must not meet; the listed nodes' kind bytes carry bit 16 and the other
+ * workgroups skip them. Same bits as lbm_adj_pull_step_rules followed by
+ * lbm_ibm_adjoint_pull. One workgroup shape bx x by for every part. */
+LBM_API int lbm_adj_fused_step(
+    int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff, const void* constant, int m,
+    const int64_t* nodes, int n_nodes, const int* inverse, const void* weights, const void* areas, int n_markers,
+    const int* spread_start, const int* spread_entry, int iterations, double fx, double fy, double fz, int64_t n_total,
+    void* nu, const double* body_weight, void* lam, const int64_t* tslot, const int* tstart, const int* trow,
+    const void* tcoeff, int n_tslots, double* grad_w, const int64_t* rule_target, int n_rules,
+    const int64_t* rule_uslot, const int* rule_ustart, const int* rule_urule, const void* rule_ucoeff,
+    int n_rule_unique, const void* record, const void* lam_next, const uint8_t* kind, const void* wall_u,
+    const void* force_field, const void* body_field, int nx, int ny, int nz, double omega_plus, double omega_minus,
+    double* grad_body, const int32_t* region, int n_region, const int32_t* rule_region, int n_rule_region, int bx,
+    int by, const void* pull_src, void* pull_dst, const void* pull_field, const uint32_t* rule_node_info,
+    const int32_t* rule_node_first, int rule_nodes, const int64_t* prule_slot, const void* prule_coeff,
+    const void* prule_constant, int64_t n_prules, int prule_m, void* stream);
+
 /* Size in bytes of one lattice descriptor of the batched step (no counterpart in the
  * reference, which runs one lattice per solver). Host only. */
 LBM_API int lbm_batched_desc_size(int dtype, int64_t* bytes);

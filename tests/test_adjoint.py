@@ -335,6 +335,7 @@ def test_forward_with_rules_in_the_step_gives_the_same_drag_and_gradients(dtype)
     for in_step in (True, False):
         problem = DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params))
         assert problem.solver._links is not None
+        problem._fused = None  # the two-launch backward step in both runs
         if not in_step:
             problem.solver._links = None
         problem.set_body_force(g)
@@ -343,6 +344,32 @@ def test_forward_with_rules_in_the_step_gives_the_same_drag_and_gradients(dtype)
     assert runs[0][0] == runs[1][0]
     for a, b in zip(runs[0][1:], runs[1][1:], strict=True):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rules", [True, False], ids=["zou-he", "walls"])
+def test_fused_backward_step_gives_the_same_bits_as_two_launches(rules):
+    """The one-launch backward step (the IBM workgroup runs the adjoint step of its region
+    nodes, the forward recompute rides with its link rules inside the pull step and writes
+    them back) against the two launches per step: loss and gradients bit for bit, over three
+    checkpoint segments."""
+    from lbm.adjoint import DragAdjoint
+
+    params = case(dtype="float32", rules=rules)
+    g = np.random.default_rng(9).standard_normal((2, *params.shape)) * 1e-6 * params.fluid
+    runs = []
+    for fused in (True, False):
+        problem = DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params))
+        assert problem._fused is not None
+        if not fused:
+            problem._fused = None
+        problem.set_body_force(g)
+        loss = problem.forward()
+        runs.append((loss, *problem.backward()))
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1:], runs[1][1:], strict=True):
+        assert np.array_equal(a, b)
+        assert np.any(a != 0)
 
 
 @pytest.mark.gpu
@@ -362,6 +389,7 @@ def test_prefetching_ibm_adjoint_gives_the_same_bits_as_the_generic_kernel(dtype
         problem = DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params))
         assert problem._slots_disjoint == 1  # Zou-He rows read neither targets nor IBM slots
         problem._slots_disjoint = disjoint
+        problem._fused = None  # the two-launch backward step
         problem.set_body_force(g)
         problem.forward()
         grads.append(problem.backward())

@@ -196,6 +196,76 @@ __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T*
     }
 }
 
+// The adjoint update of a node with no solid upstream neighbour (an IBM stencil node) in two
+// parts, so its loads can go out with other loads: load() reads the kind byte, the pulled
+// populations, the adjoint populations and the node's forces; finish() is the rest of
+// adj_pull_node (same operations, so the same bits).
+template <class L, class T, ForceMode FM>
+struct InteriorAdjNode {
+    uint32_t node;
+    int x, y, z;
+    uint8_t k;
+    T fs[L::Q], lam[L::Q];
+    T body[3], field[3];
+
+    __device__ __forceinline__ void load(const T* __restrict__ lam_next, const T* __restrict__ f,
+                                         const uint8_t* __restrict__ kind, const T* __restrict__ fld,
+                                         const T* __restrict__ bdy, const Grid& g, uint32_t at)
+    {
+        node = at;
+        const uint32_t rest = node / (uint32_t)g.nx;
+        x = (int)(node - rest * (uint32_t)g.nx);
+        y = (int)(rest % (uint32_t)g.ny);
+        z = (int)(rest / (uint32_t)g.ny);
+        k = kind[node];
+        static_for<0, L::Q>([&](auto I) {
+            constexpr int i = I;
+            fs[i] = f[(size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)];
+            lam[i] = lam_next[(size_t)i * g.n + node];
+        });
+#pragma unroll
+        for (int a = 0; a < L::D; ++a) {
+            body[a] = FM == ForceMode::DENSE ? bdy[(size_t)a * g.n + node] : T(0);
+            field[a] = FM != ForceMode::UNIFORM ? fld[(size_t)a * g.n + node] : T(0);
+        }
+    }
+
+    __device__ __forceinline__ void finish(T* __restrict__ lam_out, double* __restrict__ grad_body,
+                                           T* __restrict__ nu_forced, const Grid& g, const Relaxation<T>& r)
+    {
+        T Fx = r.force[0], Fy = r.force[1], Fz = r.force[2];
+        if constexpr (FM == ForceMode::DENSE) {
+            Fx = Fx + body[0];
+            Fy = Fy + body[1];
+            if constexpr (L::D == 3) Fz = Fz + body[2];
+        }
+        if constexpr (FM != ForceMode::UNIFORM) {
+            if (k & KIND_FORCED) {
+                Fx = Fx + field[0];
+                Fy = Fy + field[1];
+                if constexpr (L::D == 3) Fz = Fz + field[2];
+            }
+        }
+        T mu[L::Q];
+        T Fbx, Fby, Fbz;
+        collide_adjoint<L, FM>(fs, lam, r, Fx, Fy, Fz, mu, Fbx, Fby, Fbz);
+        static_for<0, L::Q>([&](auto I) {
+            constexpr int i = I;
+            lam_out[(size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)] = mu[i];
+        });
+        if (grad_body) {
+            grad_body[node] += (double)Fbx;
+            grad_body[(size_t)g.n + node] += (double)Fby;
+            if constexpr (L::D == 3) grad_body[2 * (size_t)g.n + node] += (double)Fbz;
+        }
+        if (nu_forced && (k & KIND_FORCED)) {
+            nu_forced[node] = Fbx;
+            nu_forced[(size_t)g.n + node] = Fby;
+            if constexpr (L::D == 3) nu_forced[2 * (size_t)g.n + node] = Fbz;
+        }
+    }
+};
+
 template <class L, class T, ForceMode FM, bool MASK>
 __global__ __launch_bounds__(256) void k_adj_pull(const T* __restrict__ lam_next, T* __restrict__ lam_out,
                                                   const T* __restrict__ f, const uint8_t* __restrict__ kind,
@@ -620,8 +690,17 @@ __device__ __forceinline__ T row_list_sum(const int* __restrict__ row, const T* 
 // final before the kernel starts. Float32 only, with the rule adjoints a_r in LDS (float64
 // values would not fit the lanes' registers). Same arithmetic in the same order as
 // ibm_adj_group, so the same bits.
-template <class T, int D>
-__device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int tid, int nthreads)
+//
+// `before` runs before the first loads and `after` once the first level has arrived, followed
+// by a barrier when AFTER_BARRIER: a caller issues other loads in `before` so they share the
+// first latency, and finishes work in `after` whose lam and nu stores the second level reads.
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+template <class T, int D, bool AFTER_BARRIER = false, class Before = NoHook, class After = NoHook>
+__device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int tid, int nthreads,
+                                                         Before before = Before{}, After after = After{})
 {
     constexpr int S = D == 2 ? 9 : 27;
     constexpr int KT = ADJ_AHEAD_TSLOTS, KR = ADJ_AHEAD_RULE_SLOTS, E = ADJ_AHEAD_ENTRIES;
@@ -644,6 +723,7 @@ __device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int
     const IbmTables<T> tb{inv, w, a, st, ent};
     const int gw_count = nm * S;
 
+    before();
     // ---- level 1
     const bool has_rule = tid < n_rules, has_node = tid < nn;
     const int64_t target0 = has_rule ? rtarget[tid] : 0;
@@ -679,6 +759,8 @@ __device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int
         const WordCopy rec[1] = {word_copy(base, record, record ? ibm_record_elems(D, nn, nm, iters) : 0)};
         copy_staged<5, 1, 3>(tables, rec, tid, nthreads);
     }
+    after();
+    if constexpr (AFTER_BARRIER) __syncthreads();
 
     // ---- level 2
     int rr[KR][E];
@@ -924,9 +1006,10 @@ __global__ __launch_bounds__(ibm_adj_max_threads(L::D)) void k_ibm_adj_pull(
 // One backward step of the pipelined sweep in one launch (float32, the prefetching IBM
 // adjoint; a one-dimensional grid of workgroups of bx x by <= 1024 lanes):
 //   workgroup 0        the IBM region: the adjoint step of the IBM stencil nodes (listed in
-//                      `region`), a barrier, then the IBM adjoint (ibm_adj_group_prefetched
-//                      without rules), whose lam reads find values this workgroup has just
-//                      written;
+//                      `region`; none has a solid neighbour), their loads issued with the
+//                      IBM adjoint's first level, a barrier, then the rest of the IBM adjoint
+//                      (ibm_adj_group_prefetched without rules), whose lam reads find values
+//                      this workgroup has just written;
 //   workgroup 1        (with link rules) the rule region: the adjoint step of the writers of
 //                      every rule target and rule term slot (`rule_region`), a barrier, then
 //                      the transposed link rules (as lbm_adj_rules);
@@ -967,13 +1050,29 @@ __global__ __launch_bounds__(1024) void k_adj_fused(LBM_IBM_ADJ_PARAMS, const T*
     };
     const int specials = n_rules > 0 ? 2 : 1;
     if (b == 0) {
-        region_step(region, n_region);
-        // The IBM adjoint alone: workgroup 1 runs the transposed link rules.
-        ibm_adj_group_prefetched<T, L::D>(F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g, nm,
-                                          start_g, entry_g, iters, hfx, hfy, hfz, n_total, nu, body_weight, lam,
-                                          tslot, tstart, trow, tcoeff, n_tslots, grad_w_g, scratch, nullptr, 0,
-                                          nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0, record, lds_trows, tid,
-                                          lanes);
+        // The stencil nodes' adjoint step: the first `lanes` nodes load alongside the IBM
+        // adjoint's first level and finish once it has arrived; a barrier, then the IBM
+        // adjoint alone reads their lam and nu (workgroup 1 runs the transposed link rules).
+        InteriorAdjNode<L, T, FM> own;
+        const bool has = tid < n_region;
+        auto before = [&] {
+            if (has) own.load(lam_next, F, kind, field, body, g, (uint32_t)region[tid]);
+        };
+        auto after = [&] {
+            if (has) own.finish(lam, grad_body, nu_out, g, r);
+            for (int q = tid + lanes; q < n_region; q += lanes) {
+                const uint32_t node = (uint32_t)region[q];
+                const uint32_t rest = node / (uint32_t)g.nx;
+                adj_pull_node<L, T, FM, true>(lam_next, lam, F, kind, wall_u, field, body, grad_body, nu_out, g, r,
+                                              (int)(node - rest * (uint32_t)g.nx), (int)(rest % (uint32_t)g.ny),
+                                              (int)(rest / (uint32_t)g.ny));
+            }
+        };
+        ibm_adj_group_prefetched<T, L::D, true>(F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g,
+                                                nm, start_g, entry_g, iters, hfx, hfy, hfz, n_total, nu, body_weight,
+                                                lam, tslot, tstart, trow, tcoeff, n_tslots, grad_w_g, scratch,
+                                                nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0, record,
+                                                lds_trows, tid, lanes, before, after);
         return;
     }
     if (b == 1 && specials == 2) {

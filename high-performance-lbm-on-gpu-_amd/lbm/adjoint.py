@@ -221,6 +221,8 @@ class DragAdjoint:
         both = np.concatenate([region, rule_region])
         if solid[both].any() or np.intersect1d(region, rule_region).size:
             return
+        if (s._kind.cpu().numpy()[region] & 4).any():  # the IBM region takes no near-solid node
+            return
         kind = s._kind.clone()
         kind[torch.as_tensor(both, device=s.device)] |= 16  # KIND_ADJ_REGION
         self._fused = {"region": torch.as_tensor(region.astype(np.int32), device=s.device),

@@ -196,23 +196,33 @@ __device__ __forceinline__ void adj_pull_node(const T* __restrict__ lam_next, T*
     }
 }
 
-// The adjoint update of a node with no solid upstream neighbour (an IBM stencil node) in two
-// parts, so its loads can go out with other loads: load() reads the kind byte, the pulled
-// populations, the adjoint populations and the node's forces; finish() is the rest of
-// adj_pull_node (same operations, so the same bits).
-template <class L, class T, ForceMode FM>
-struct InteriorAdjNode {
-    uint32_t node;
+// The adjoint update of one fluid node (adj_pull_node) in two parts, so that all of its
+// loads go out in one level, with other loads: load() reads the kind byte, the pulled and the
+// adjoint populations, the node's forces and its float64 body-force gradients (no other
+// workgroup writes them in the launch); finish() is the rest of adj_pull_node (same
+// operations, so the same bits). Without MASKED the node must have no solid upstream
+// neighbour; with MASKED load() takes the set of directions whose upstream node is solid
+// (bit i for x - c_i), loads those populations from their bounce-back slots and the wall
+// terms, and the pushes of those directions go to the bounce-back slots.
+template <class L, class T, ForceMode FM, bool MASKED = false>
+struct AdjNodeParts {
+    static constexpr int QM = MASKED ? L::Q : 1;
+    uint32_t node, bits;
     int x, y, z;
     uint8_t k;
     T fs[L::Q], lam[L::Q];
     T body[3], field[3];
+    double gb[3];
+    T wt[QM];
 
     __device__ __forceinline__ void load(const T* __restrict__ lam_next, const T* __restrict__ f,
-                                         const uint8_t* __restrict__ kind, const T* __restrict__ fld,
-                                         const T* __restrict__ bdy, const Grid& g, uint32_t at)
+                                         const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
+                                         const T* __restrict__ fld, const T* __restrict__ bdy,
+                                         const double* __restrict__ grad_body, const Grid& g, uint32_t at,
+                                         uint32_t blocked = 0)
     {
         node = at;
+        bits = MASKED ? blocked : 0u;
         const uint32_t rest = node / (uint32_t)g.nx;
         x = (int)(node - rest * (uint32_t)g.nx);
         y = (int)(rest % (uint32_t)g.ny);
@@ -220,19 +230,34 @@ struct InteriorAdjNode {
         k = kind[node];
         static_for<0, L::Q>([&](auto I) {
             constexpr int i = I;
-            fs[i] = f[(size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)];
+            const bool bounce = MASKED && ((bits >> i) & 1u);
+            fs[i] = f[bounce ? (size_t)L::opp[i] * g.n + node : (size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)];
             lam[i] = lam_next[(size_t)i * g.n + node];
         });
 #pragma unroll
         for (int a = 0; a < L::D; ++a) {
             body[a] = FM == ForceMode::DENSE ? bdy[(size_t)a * g.n + node] : T(0);
             field[a] = FM != ForceMode::UNIFORM ? fld[(size_t)a * g.n + node] : T(0);
+            gb[a] = grad_body ? grad_body[(size_t)a * g.n + node] : 0.0;
+        }
+        if constexpr (MASKED) {
+            if (wall_u) {
+                static_for<0, L::Q>([&](auto I) { wt[I] = wall_term<L, I>(wall_u, neighbour<L, I, -1>(x, y, z, g), g.n); });
+            }
         }
     }
 
-    __device__ __forceinline__ void finish(T* __restrict__ lam_out, double* __restrict__ grad_body,
-                                           T* __restrict__ nu_forced, const Grid& g, const Relaxation<T>& r)
+    // mu (the adjoint pushes) and dL/dF of the node; blocked directions take the wall terms.
+    __device__ __forceinline__ void adjoint(const T* __restrict__ wall_u, const Relaxation<T>& r, T (&mu)[L::Q],
+                                            T& Fbx, T& Fby, T& Fbz)
     {
+        if constexpr (MASKED) {
+            if (wall_u) {
+                static_for<0, L::Q>([&](auto I) {
+                    if ((bits >> I) & 1u) fs[I] = fs[I] + wt[I];
+                });
+            }
+        }
         T Fx = r.force[0], Fy = r.force[1], Fz = r.force[2];
         if constexpr (FM == ForceMode::DENSE) {
             Fx = Fx + body[0];
@@ -246,23 +271,80 @@ struct InteriorAdjNode {
                 if constexpr (L::D == 3) Fz = Fz + field[2];
             }
         }
-        T mu[L::Q];
-        T Fbx, Fby, Fbz;
         collide_adjoint<L, FM>(fs, lam, r, Fx, Fy, Fz, mu, Fbx, Fby, Fbz);
+    }
+
+    __device__ __forceinline__ void gradients(double* __restrict__ grad_body, const Grid& g, T Fbx, T Fby, T Fbz) const
+    {
+        if (grad_body) {
+            grad_body[node] = gb[0] + (double)Fbx;
+            grad_body[(size_t)g.n + node] = gb[1] + (double)Fby;
+            if constexpr (L::D == 3) grad_body[2 * (size_t)g.n + node] = gb[2] + (double)Fbz;
+        }
+    }
+
+    // The pushes mu to their slots, the body-force gradients and nu at a forced node.
+    __device__ __forceinline__ void store(T* __restrict__ lam_out, const T (&mu)[L::Q], double* __restrict__ grad_body,
+                                          T* __restrict__ nu_forced, const Grid& g, T Fbx, T Fby, T Fbz) const
+    {
         static_for<0, L::Q>([&](auto I) {
             constexpr int i = I;
-            lam_out[(size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)] = mu[i];
+            const bool bounce = MASKED && ((bits >> i) & 1u);
+            lam_out[bounce ? (size_t)L::opp[i] * g.n + node : (size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)] = mu[i];
         });
-        if (grad_body) {
-            grad_body[node] += (double)Fbx;
-            grad_body[(size_t)g.n + node] += (double)Fby;
-            if constexpr (L::D == 3) grad_body[2 * (size_t)g.n + node] += (double)Fbz;
-        }
+        gradients(grad_body, g, Fbx, Fby, Fbz);
         if (nu_forced && (k & KIND_FORCED)) {
             nu_forced[node] = Fbx;
             nu_forced[(size_t)g.n + node] = Fby;
             if constexpr (L::D == 3) nu_forced[2 * (size_t)g.n + node] = Fbz;
         }
+    }
+
+    __device__ __forceinline__ void finish(T* __restrict__ lam_out, double* __restrict__ grad_body,
+                                           T* __restrict__ nu_forced, const T* __restrict__ wall_u, const Grid& g,
+                                           const Relaxation<T>& r)
+    {
+        T mu[L::Q];
+        T Fbx, Fby, Fbz;
+        adjoint(wall_u, r, mu, Fbx, Fby, Fbz);
+        store(lam_out, mu, grad_body, nu_forced, g, Fbx, Fby, Fbz);
+    }
+
+    // finish() for an IBM node that keeps its pushes: mu_i stays in lam[i] (the node adds the
+    // transposed moment rows itself in push_kept) and dL/dF (the node being forced) goes to
+    // nu_seed instead of nu; grad_body is updated as in finish().
+    __device__ __forceinline__ void finish_kept(T (&nu_seed)[L::D], double* __restrict__ grad_body, const Grid& g,
+                                                const Relaxation<T>& r)
+    {
+        static_assert(!MASKED, "an IBM node has no solid upstream neighbour");
+        T mu[L::Q];
+        T Fbx, Fby, Fbz;
+        adjoint(nullptr, r, mu, Fbx, Fby, Fbz);
+        static_for<0, L::Q>([&](auto I) { lam[I] = mu[I]; });
+        gradients(grad_body, g, Fbx, Fby, Fbz);
+        const bool forced = (k & KIND_FORCED) != 0;
+        nu_seed[0] = forced ? Fbx : T(0);
+        nu_seed[1] = forced ? Fby : T(0);
+        if constexpr (L::D == 3) nu_seed[2] = forced ? Fbz : T(0);
+    }
+
+    // The kept pushes plus the transposed moment rows of IBM node `row` (rows a * nn + row,
+    // a = 0 density with coefficient 1, a = 1 .. d momentum with c_i,a-1, the nonzero ones in
+    // increasing a, summed from 0 as the transposed row lists are): push i writes slot
+    // (i, x - c_i), which only this node's moment rows read.
+    __device__ __forceinline__ void push_kept(T* __restrict__ lam_out, const T* __restrict__ m_bar, int nn, int row,
+                                              const Grid& g) const
+    {
+        static_for<0, L::Q>([&](auto I) {
+            constexpr int i = I;
+            constexpr int c[3] = {L::cx[i], L::cy[i], L::cz[i]};
+            T acc = T(0);
+            acc += T(1) * m_bar[row];
+#pragma unroll
+            for (int a = 0; a < L::D; ++a)
+                if (c[a] != 0) acc += T(c[a]) * m_bar[(size_t)(1 + a) * nn + row];
+            lam_out[(size_t)i * g.n + nbr<L, i, -1>(x, y, z, g)] = lam[i] + acc;
+        });
     }
 };
 
@@ -694,13 +776,19 @@ __device__ __forceinline__ T row_list_sum(const int* __restrict__ row, const T* 
 // `before` runs before the first loads and `after` once the first level has arrived, followed
 // by a barrier when AFTER_BARRIER: a caller issues other loads in `before` so they share the
 // first latency, and finishes work in `after` whose lam and nu stores the second level reads.
+// With OWN the caller keeps the adjoint pushes of the IBM nodes (lane tid has node tid) and
+// adds the transposed moment rows itself: `after(nu_seed)` gives nu at node tid, and
+// `scatter(m_bar)` at the end replaces the transposed row lists, which are then neither read
+// nor staged (lds_trows 0); lam at the transposed slots and nu are not read at all.
 struct NoHook {
     __device__ void operator()() const {}
 };
 
-template <class T, int D, bool AFTER_BARRIER = false, class Before = NoHook, class After = NoHook>
+template <class T, int D, bool AFTER_BARRIER = false, bool OWN = false, class Before = NoHook, class After = NoHook,
+          class Scatter = NoHook>
 __device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int tid, int nthreads,
-                                                         Before before = Before{}, After after = After{})
+                                                         Before before = Before{}, After after = After{},
+                                                         Scatter scatter = Scatter{})
 {
     constexpr int S = D == 2 ? 9 : 27;
     constexpr int KT = ADJ_AHEAD_TSLOTS, KR = ADJ_AHEAD_RULE_SLOTS, E = ADJ_AHEAD_ENTRIES;
@@ -739,12 +827,15 @@ __device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int
         ue[k] = in ? rustart[t + 1] : 0;
     }
     int64_t tsl[KT];
+    int te = 0;
+    if constexpr (!OWN) {
 #pragma unroll
-    for (int k = 0; k < KT; ++k) {
-        const int t = tid + k * nthreads;
-        tsl[k] = n_tslots > 0 ? tslot[t < n_tslots ? t : 0] : 0;
+        for (int k = 0; k < KT; ++k) {
+            const int t = tid + k * nthreads;
+            tsl[k] = n_tslots > 0 ? tslot[t < n_tslots ? t : 0] : 0;
+        }
+        te = tstart[n_tslots];
     }
-    const int te = tstart[n_tslots];
     const double gw_old = tid < gw_count ? grad_w_g[tid] : 0.0;
     for (int e = tid; e < gw_count; e += nthreads) gw[e] = 0.0;
     {
@@ -759,7 +850,12 @@ __device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int
         const WordCopy rec[1] = {word_copy(base, record, record ? ibm_record_elems(D, nn, nm, iters) : 0)};
         copy_staged<5, 1, 3>(tables, rec, tid, nthreads);
     }
-    after();
+    T nu_seed[D];
+    if constexpr (OWN) {
+        after(nu_seed);
+    } else {
+        after();
+    }
     if constexpr (AFTER_BARRIER) __syncthreads();
 
     // ---- level 2
@@ -780,14 +876,19 @@ __device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int
     }
     const T lam0 = has_rule ? lam[target0] : T(0);
     T nu0[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) nu0[c] = has_node ? nu[(size_t)c * n_total + node0] : T(0);
     T told[KT];
+    if constexpr (OWN) {
 #pragma unroll
-    for (int k = 0; k < KT; ++k) told[k] = n_tslots > 0 ? lam[tsl[k]] : T(0);
+        for (int c = 0; c < D; ++c) nu0[c] = nu_seed[c];
+    } else {
+#pragma unroll
+        for (int c = 0; c < D; ++c) nu0[c] = has_node ? nu[(size_t)c * n_total + node0] : T(0);
+#pragma unroll
+        for (int k = 0; k < KT; ++k) told[k] = n_tslots > 0 ? lam[tsl[k]] : T(0);
+    }
     // The transposed rows go to LDS when their entries fit the plan (always, for moment rows).
-    const bool rows_staged = te <= lds_trows * adj_row_entries(D);
-    {
+    const bool rows_staged = !OWN && te <= lds_trows * adj_row_entries(D);
+    if constexpr (!OWN) {
         const WordCopy none[1] = {WordCopy{nullptr, nullptr, 0}};
         const WordCopy rows[3] = {
             word_copy(tst, tstart, (size_t)n_tslots + 1),
@@ -857,8 +958,12 @@ __device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int
         for (int t = tid + KT * nthreads; t < n_tslots; t += nthreads)
             lam[tslot[t]] = lam[tslot[t]] + row_list_sum(rw, cf, tst[t], tst[t + 1], m_bar);
     };
-    if (rows_staged) transposed(trw, tco);
-    else transposed(trow, tcoeff);
+    if constexpr (OWN) {
+        scatter(m_bar);
+    } else {
+        if (rows_staged) transposed(trw, tco);
+        else transposed(trow, tcoeff);
+    }
     if (n_runique > 0) {
 #pragma unroll
         for (int k = 0; k < KR; ++k)
@@ -1002,6 +1107,81 @@ __global__ __launch_bounds__(ibm_adj_max_threads(L::D)) void k_ibm_adj_pull(
 }
 
 // ------------------------------------------------------------------ fused backward step
+
+// The link-rule nodes of the fused backward step, for rule sets in which every slot a rule
+// reads or writes is written by the adjoint step of one node, the rule's node: info[k] =
+// {node, directions i whose upstream node x - c_i is solid | the node's rule target
+// directions << 16, 0, 0}, where rule target direction j is the push of the node that writes
+// the rule's target slot; coeff[k][i][j] (Q x Q) = the coefficient of the rule with target
+// direction j in the transposed row of the slot push i writes (zero where the rule does not
+// read that slot). Nodes sorted by index, in chunks of `chunk` nodes.
+template <class T>
+struct AdjRuleNodes {
+    const uint4* info;
+    const T* coeff;
+    int count;
+    int chunk;
+};
+
+// One chunk of the link-rule nodes by one workgroup: the adjoint step of each node with the
+// transposed link rules applied to its pushes before they are stored. As lbm_adj_rules after
+// the pushes: a_j = the push of target direction j, that push becomes zero, and push i gets
+// sum_j coeff[i][j] a_j over the nonzero coefficients in increasing j (the transposed row's
+// rule order: a node's rules are sorted by direction), added from 0 and only where the row
+// has entries, so the same bits.
+template <class L, class T, ForceMode FM>
+__device__ __forceinline__ void adj_rule_chunk(const T* __restrict__ lam_next, T* __restrict__ lam,
+                                               const T* __restrict__ f, const uint8_t* __restrict__ kind,
+                                               const T* __restrict__ wall_u, const T* __restrict__ field,
+                                               const T* __restrict__ body, double* __restrict__ grad_body,
+                                               T* __restrict__ nu_forced, const Grid& g, const Relaxation<T>& r,
+                                               const AdjRuleNodes<T>& ar, int c, int lanes, int tid,
+                                               T* __restrict__ cl)
+{
+    constexpr int Q = L::Q;
+    const int first = c * ar.chunk;
+    const int end = min(first + ar.chunk, ar.count);
+    const bool node_lane = first + tid < end;
+    uint4 info = make_uint4(0u, 0u, 0u, 0u);
+    if (node_lane) info = ar.info[first + tid];
+    AdjNodeParts<L, T, FM, true> nd;
+    if (node_lane) nd.load(lam_next, f, kind, wall_u, field, body, grad_body, g, info.x, info.y & 0xffffu);
+    {
+        const WordCopy none[1] = {WordCopy{nullptr, nullptr, 0}};
+        const WordCopy cm[1] = {word_copy(cl, ar.coeff + (size_t)first * Q * Q, (size_t)(end - first) * Q * Q)};
+        copy_staged<1, 1>(cm, none, tid, lanes);
+    }
+    __syncthreads();
+    if (!node_lane) return;
+    T mu[Q];
+    T Fbx, Fby, Fbz;
+    nd.adjoint(wall_u, r, mu, Fbx, Fby, Fbz);
+    const uint32_t targets = info.y >> 16;
+    const T* cm = cl + (size_t)tid * Q * Q;
+    // a_j for every direction (only the targets' are read); a select between the pushes and
+    // a zero would make the compiler select addresses and keep mu in scratch memory.
+    T a[Q];
+    static_for<0, Q>([&](auto J) {
+        a[J] = mu[J];
+        if ((targets >> J) & 1u) mu[J] = T(0);
+    });
+    static_for<0, Q>([&](auto I) {
+        T acc = T(0);
+        bool any = false;
+        static_for<0, Q>([&](auto J) {
+            if ((targets >> J) & 1u) {
+                const T cij = cm[(int)I * Q + (int)J];
+                if (cij != T(0)) {
+                    acc += cij * a[J];
+                    any = true;
+                }
+            }
+        });
+        if (any) mu[I] = mu[I] + acc;
+    });
+    nd.store(lam, mu, grad_body, nu_forced, g, Fbx, Fby, Fbz);
+}
+
 //
 // One backward step of the pipelined sweep in one launch (float32, the prefetching IBM
 // adjoint; a one-dimensional grid of workgroups of bx x by <= 1024 lanes):
@@ -1009,26 +1189,27 @@ __global__ __launch_bounds__(ibm_adj_max_threads(L::D)) void k_ibm_adj_pull(
 //                      `region`; none has a solid neighbour), their loads issued with the
 //                      IBM adjoint's first level, a barrier, then the rest of the IBM adjoint
 //                      (ibm_adj_group_prefetched without rules), whose lam reads find values
-//                      this workgroup has just written;
-//   workgroup 1        (with link rules) the rule region: the adjoint step of the writers of
-//                      every rule target and rule term slot (`rule_region`), a barrier, then
-//                      the transposed link rules (as lbm_adj_rules);
+//                      this workgroup has just written; with `keep` the stencil nodes keep
+//                      their pushes and nu in registers instead and add their moment rows'
+//                      adjoint at the end;
+//   next `achunks`     the link-rule nodes (adj_rule_chunk): their adjoint step with the
+//                      transposed link rules applied to their pushes;
 //   next `chunks`      the partner (forward recompute) step's rule chunks, with write-back;
 //   next ptiles        the partner step's other nodes (pull_update, rule nodes left out);
 //   last atiles        the adjoint step of every node outside the regions.
 // Every lam slot of a fluid node is written by exactly one node's push; the slots the IBM
-// adjoint reads or updates are written by IBM-region nodes and those of the rule adjoint by
-// rule-region nodes (the host keeps the two regions and slot sets apart), so neither region
-// workgroup races with another workgroup; the partner step works on other buffers. The
-// operations are those of the two-launch form (adjoint step with its riding pre-pass, then
-// the IBM and rule adjoints with the riding pull), so the bits are the same.
+// adjoint reads or updates are written by IBM-region nodes and those of the link rules by the
+// rules' own nodes (the host keeps the IBM region, the rule nodes and their slot sets apart),
+// so no workgroup reads a slot another one writes; the partner step works on other buffers.
+// The operations are those of the two-launch form (adjoint step with its riding pre-pass,
+// then the IBM and rule adjoints with the riding pull), so the bits are the same.
 template <class L, class T, ForceMode FM>
 __global__ __launch_bounds__(1024) void k_adj_fused(LBM_IBM_ADJ_PARAMS, const T* __restrict__ lam_next,
                                                     const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
                                                     const T* __restrict__ field, const T* __restrict__ body,
                                                     double* __restrict__ grad_body,
-                                                    const int32_t* __restrict__ region, int n_region,
-                                                    const int32_t* __restrict__ rule_region, int n_rule_region,
+                                                    const int32_t* __restrict__ region, int n_region, int keep,
+                                                    AdjRuleNodes<T> ar,
                                                     Grid g, Relaxation<T> r, int atiles_x, int atiles_y,
                                                     const T* __restrict__ p_src, T* __restrict__ p_dst,
                                                     const T* __restrict__ p_field, RuleRows<T> p_rules, int ptiles_x,
@@ -1038,28 +1219,17 @@ __global__ __launch_bounds__(1024) void k_adj_fused(LBM_IBM_ADJ_PARAMS, const T*
     const int lanes = (int)(blockDim.x * blockDim.y);
     const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
     T* nu_out = const_cast<T*>(nu);  // the IBM region writes nu, then the IBM adjoint reads it
-    auto region_step = [&](const int32_t* __restrict__ list, int count) {
-        for (int q = tid; q < count; q += lanes) {
-            const uint32_t node = (uint32_t)list[q];
-            const uint32_t rest = node / (uint32_t)g.nx;
-            adj_pull_node<L, T, FM, true>(lam_next, lam, F, kind, wall_u, field, body, grad_body, nu_out, g, r,
-                                          (int)(node - rest * (uint32_t)g.nx), (int)(rest % (uint32_t)g.ny),
-                                          (int)(rest / (uint32_t)g.ny));
-        }
-        __syncthreads();
-    };
-    const int specials = n_rules > 0 ? 2 : 1;
     if (b == 0) {
         // The stencil nodes' adjoint step: the first `lanes` nodes load alongside the IBM
         // adjoint's first level and finish once it has arrived; a barrier, then the IBM
         // adjoint alone reads their lam and nu (workgroup 1 runs the transposed link rules).
-        InteriorAdjNode<L, T, FM> own;
+        AdjNodeParts<L, T, FM> own;
         const bool has = tid < n_region;
         auto before = [&] {
-            if (has) own.load(lam_next, F, kind, field, body, g, (uint32_t)region[tid]);
+            if (has) own.load(lam_next, F, kind, wall_u, field, body, grad_body, g, (uint32_t)region[tid]);
         };
         auto after = [&] {
-            if (has) own.finish(lam, grad_body, nu_out, g, r);
+            if (has) own.finish(lam, grad_body, nu_out, wall_u, g, r);
             for (int q = tid + lanes; q < n_region; q += lanes) {
                 const uint32_t node = (uint32_t)region[q];
                 const uint32_t rest = node / (uint32_t)g.nx;
@@ -1068,6 +1238,26 @@ __global__ __launch_bounds__(1024) void k_adj_fused(LBM_IBM_ADJ_PARAMS, const T*
                                               (int)(rest / (uint32_t)g.ny));
             }
         };
+        if (keep) {
+            // The region is the IBM node list itself (node tid is IBM node tid) and its pushes
+            // are the transposed slots: each node keeps its pushes and nu in registers and adds
+            // its moment rows' adjoint at the end. No barrier after the first level: the rest
+            // reads nothing these nodes would have written.
+            auto after_kept = [&](T (&nu_seed)[L::D]) {
+#pragma unroll
+                for (int a = 0; a < L::D; ++a) nu_seed[a] = T(0);
+                if (has) own.finish_kept(nu_seed, grad_body, g, r);
+            };
+            auto scatter_kept = [&](const T* m_bar) {
+                if (has) own.push_kept(lam, m_bar, nn, tid, g);
+            };
+            ibm_adj_group_prefetched<T, L::D, false, true>(
+                F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g, nm, start_g, entry_g, iters, hfx,
+                hfy, hfz, n_total, nu, body_weight, lam, tslot, tstart, trow, tcoeff, n_tslots, grad_w_g, scratch,
+                nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0, record, 0, tid, lanes, before,
+                after_kept, scatter_kept);
+            return;
+        }
         ibm_adj_group_prefetched<T, L::D, true>(F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g,
                                                 nm, start_g, entry_g, iters, hfx, hfy, hfz, n_total, nu, body_weight,
                                                 lam, tslot, tstart, trow, tcoeff, n_tslots, grad_w_g, scratch,
@@ -1075,21 +1265,16 @@ __global__ __launch_bounds__(1024) void k_adj_fused(LBM_IBM_ADJ_PARAMS, const T*
                                                 lds_trows, tid, lanes, before, after);
         return;
     }
-    if (b == 1 && specials == 2) {
-        region_step(rule_region, n_rule_region);
+    int t = b - 1;
+    const int achunks = ar.count ? rule_chunk_count(ar.count, ar.chunk) : 0;
+    if (t < achunks) {
         extern __shared__ __align__(16) unsigned char adj_lds[];
-        T* a = reinterpret_cast<T*>(adj_lds);
-        for (int q = tid; q < n_rules; q += lanes) {
-            const int64_t t = rtarget[q];
-            a[q] = lam[t];
-            lam[t] = T(0);
-        }
-        __syncthreads();
-        scatter_rows(lam, ruslot, rustart, rurule, rucoeff, n_runique, a, tid, lanes);
+        adj_rule_chunk<L, T, FM>(lam_next, lam, F, kind, wall_u, field, body, grad_body, nu_out, g, r, ar, t, lanes,
+                                 tid, reinterpret_cast<T*>(adj_lds));
         return;
     }
+    t -= achunks;
     const int chunks = p_rules.count ? rule_chunk_count(p_rules.count, p_rules.chunk) : 0;
-    int t = b - specials;
     if (t < chunks) {
         extern __shared__ double dynamic_lds[];
         rule_chunk<L, T, FM, true>(p_src, p_dst, kind, wall_u, p_field, body, g, r, p_rules, t, lanes, tid,
@@ -1399,27 +1584,23 @@ namespace {
 
 // The fused step's workgroup size bound and when it applies: float32 D2Q9, blocks of 64 ..
 // 1024 lanes, and the prefetching IBM adjoint (slots disjoint, rows and rule adjoints staged).
-bool adj_fused_ok(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int n_rules, int n_tslots,
-                  int slots_disjoint, int bx, int by)
+bool adj_fused_ok(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int n_tslots, int bx, int by)
 {
     if (lattice != LBM_D2Q9 || dtype != LBM_F32 || bx <= 0 || by <= 0) return false;
     const int lanes = bx * by;
     if (lanes < RULE_CHUNK_MAX || lanes > 1024 || n_nodes <= 0 || n_markers <= 0 || iterations < 1) return false;
-    const IbmAdjShape sh = ibm_adj_shape<D2Q9, float>(n_nodes, n_markers, iterations, 0, n_tslots,
-                                                      slots_disjoint != 0);
-    return sh.rows_in_lds > 0 && (size_t)n_rules * sizeof(float) <= lds_limit();
+    const IbmAdjShape sh = ibm_adj_shape<D2Q9, float>(n_nodes, n_markers, iterations, 0, n_tslots, true);
+    return sh.rows_in_lds > 0;
 }
 
 }  // namespace
 
-LBM_API int lbm_adj_fused_supported(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int n_rules,
-                                    int n_tslots, int slots_disjoint, int bx, int by, int* ok)
+LBM_API int lbm_adj_fused_supported(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int n_tslots,
+                                    int bx, int by, int* ok)
 {
     return guarded("lbm_adj_fused_supported", [&] {
         require(ok != nullptr, "null output");
-        *ok = adj_fused_ok(lattice, dtype, n_nodes, n_markers, iterations, n_rules, n_tslots, slots_disjoint, bx, by)
-                  ? 1
-                  : 0;
+        *ok = adj_fused_ok(lattice, dtype, n_nodes, n_markers, iterations, n_tslots, bx, by) ? 1 : 0;
     });
 }
 
@@ -1428,28 +1609,32 @@ LBM_API int lbm_adj_fused_step(
     const int64_t* nodes, int n_nodes, const int* inverse, const void* weights, const void* areas, int n_markers,
     const int* spread_start, const int* spread_entry, int iterations, double fx, double fy, double fz, int64_t n_total,
     void* nu, const double* body_weight, void* lam, const int64_t* tslot, const int* tstart, const int* trow,
-    const void* tcoeff, int n_tslots, double* grad_w, const int64_t* rule_target, int n_rules,
-    const int64_t* rule_uslot, const int* rule_ustart, const int* rule_urule, const void* rule_ucoeff,
-    int n_rule_unique, const void* record, const void* lam_next, const uint8_t* kind, const void* wall_u,
-    const void* force_field, const void* body_field, int nx, int ny, int nz, double omega_plus, double omega_minus,
-    double* grad_body, const int32_t* region, int n_region, const int32_t* rule_region, int n_rule_region, int bx,
-    int by, const void* pull_src, void* pull_dst, const void* pull_field, const uint32_t* rule_node_info,
-    const int32_t* rule_node_first, int rule_nodes, const int64_t* prule_slot, const void* prule_coeff,
-    const void* prule_constant, int64_t n_prules, int prule_m, void* stream)
+    const void* tcoeff, int n_tslots, double* grad_w, const void* record, const void* lam_next, const uint8_t* kind,
+    const void* wall_u, const void* force_field, const void* body_field, int nx, int ny, int nz, double omega_plus,
+    double omega_minus, double* grad_body, const int32_t* region, int n_region, int keep_pushes,
+    const uint32_t* arule_info, const void* arule_coeff, int arule_nodes, int bx, int by, const void* pull_src,
+    void* pull_dst, const void* pull_field, const uint32_t* rule_node_info, const int32_t* rule_node_first,
+    int rule_nodes, const int64_t* prule_slot, const void* prule_coeff, const void* prule_constant, int64_t n_prules,
+    int prule_m, void* stream)
 {
     return guarded("lbm_adj_fused_step", [&] {
-        require(adj_fused_ok(lattice, dtype, n_nodes, n_markers, iterations, n_rules, n_tslots, 1, bx, by),
+        require(adj_fused_ok(lattice, dtype, n_nodes, n_markers, iterations, n_tslots, bx, by),
                 "the fused adjoint step takes float32 D2Q9, 64 .. 1024 lanes and a body whose IBM adjoint stages "
-                "its tables, rule adjoints and transposed rows in LDS (lbm_adj_fused_supported)");
-        check_ibm_adj_args(m, n_nodes, n_markers, iterations, n_total, n_rules,
+                "its tables and transposed rows in LDS (lbm_adj_fused_supported)");
+        check_ibm_adj_args(m, n_nodes, n_markers, iterations, n_total, 0,
                            {f, slot, coeff, constant, nodes, inverse, weights, areas, spread_start, spread_entry, nu,
                             body_weight, lam, tslot, tstart, trow, tcoeff, grad_w},
-                           {rule_target, rule_uslot, rule_ustart, rule_urule, rule_ucoeff});
+                           {});
         const Grid g = make_grid(lattice, nx, ny, nz);
         require((int64_t)g.n == n_total, "the grid must have n_total nodes");
         require(lam_next && kind && body_field && force_field && lam_next != lam, "null or aliased adjoint buffers");
         require(n_region >= 0 && (n_region == 0 || region), "null region list");
-        require(n_rule_region >= 0 && (n_rule_region == 0 || rule_region), "null rule region list");
+        require(arule_nodes >= 0 && (arule_nodes == 0 || (arule_info && arule_coeff)), "null link-rule node tables");
+        require(reinterpret_cast<uintptr_t>(arule_info) % 16 == 0, "arule_info must be 16-byte aligned");
+        require(!keep_pushes || (n_region == n_nodes && n_region <= bx * by &&
+                                 (int64_t)n_tslots == (int64_t)n_region * (lattice == LBM_D2Q9 ? 9 : 19)),
+                "keep_pushes needs the region to be the IBM node list, at most one node per lane, whose pushes are "
+                "the transposed slots");
         const bool partner = pull_src != nullptr;
         if (partner) {
             require(pull_dst && pull_field && pull_src != pull_dst && pull_src != f && pull_dst != f,
@@ -1468,40 +1653,44 @@ LBM_API int lbm_adj_fused_step(
             using T = decltype(Tt);
             if constexpr (std::is_same<T, float>::value && L::D == 2) {
                 constexpr ForceMode FM = ForceMode::DENSE;
-                // The IBM adjoint runs without the rules (workgroup 1 takes them), whose adjoints
-                // a_r go to the start of workgroup 1's LDS.
                 const IbmAdjShape sh = ibm_adj_shape<L, T>(n_nodes, n_markers, iterations, 0, n_tslots, true);
                 const Relaxation<T> r = make_relaxation<L, T>(omega_plus, omega_minus, force);
                 const dim3 tiles = grid_of(g, block);
                 const int tx = (int)tiles.x, ty = (int)tiles.y;
                 const int64_t n_tiles = (int64_t)tiles.x * tiles.y * tiles.z;
+                const int hosts = (int)std::min<int64_t>(n_tiles, 1 << 20);
                 int chunk = RULE_CHUNK_MIN;
                 int chunks = 0;
                 if (partner && rule_nodes > 0) {
-                    chunk = rule_chunk_nodes(rule_nodes, (int)std::min<int64_t>(n_tiles, 1 << 20));
+                    chunk = rule_chunk_nodes(rule_nodes, hosts);
                     chunks = rule_chunk_count(rule_nodes, chunk);
                 }
                 const RuleRows<T> rr{reinterpret_cast<const uint4*>(rule_node_info), rule_node_first,
                                      partner ? rule_nodes : 0, chunk, prule_slot, static_cast<const T*>(prule_coeff),
                                      static_cast<const T*>(prule_constant), partner ? n_prules : 0, prule_m};
+                const int achunk = arule_nodes > 0 ? rule_chunk_nodes(arule_nodes, hosts) : RULE_CHUNK_MIN;
+                const int achunks = arule_nodes > 0 ? rule_chunk_count(arule_nodes, achunk) : 0;
+                const AdjRuleNodes<T> ar{reinterpret_cast<const uint4*>(arule_info),
+                                         static_cast<const T*>(arule_coeff), arule_nodes, achunk};
                 const int64_t ptiles = partner ? n_tiles : 0;
-                const int64_t blocks = (n_rules > 0 ? 2 : 1) + chunks + ptiles + n_tiles;
+                const int64_t blocks = 1 + achunks + chunks + ptiles + n_tiles;
                 require(blocks < (1ll << 31), "grid too large");
-                const size_t lds = std::max({sh.lds, chunks ? sizeof(T) * (size_t)chunk * (L::Q - 1) : (size_t)0,
-                                             sizeof(T) * (size_t)n_rules});
-                require(lds <= lds_limit(), "the rule adjoints do not fit the workgroup's LDS");
+                // Kept pushes need no transposed rows in LDS.
+                const int trows = keep_pushes ? 0 : sh.rows_in_lds;
+                const size_t ibm_lds = adj_lds_plan(L::D, n_nodes, n_markers, iterations, sizeof(T), 0, trows).total;
+                const size_t lds = std::max({ibm_lds, chunks ? sizeof(T) * (size_t)chunk * (L::Q - 1) : (size_t)0,
+                                             achunks ? sizeof(T) * (size_t)achunk * L::Q * L::Q : (size_t)0});
+                require(lds <= lds_limit(), "the workgroups' tables do not fit the LDS");
                 allow_lds(k_adj_fused<L, T, FM>, lds);
                 k_adj_fused<L, T, FM><<<(unsigned)blocks, block, lds, as_stream(stream)>>>(
                     static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), m,
                     nodes, n_nodes, inverse, static_cast<const T*>(weights), static_cast<const T*>(areas), n_markers,
                     spread_start, spread_entry, iterations, T(0.5 * fx), T(0.5 * fy), T(0.5 * fz), (uint32_t)n_total,
                     static_cast<const T*>(nu), body_weight, static_cast<T*>(lam), tslot, tstart, trow,
-                    static_cast<const T*>(tcoeff), n_tslots, grad_w, nullptr, rule_target, n_rules, rule_uslot,
-                    rule_ustart, rule_urule, static_cast<const T*>(rule_ucoeff), n_rule_unique, nullptr,
-                    sh.rules_in_lds, static_cast<const T*>(record), sh.rows_in_lds, static_cast<const T*>(lam_next),
+                    static_cast<const T*>(tcoeff), n_tslots, grad_w, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
+                    nullptr, 0, nullptr, 0, static_cast<const T*>(record), trows, static_cast<const T*>(lam_next),
                     kind, static_cast<const T*>(wall_u), static_cast<const T*>(force_field),
-                    static_cast<const T*>(body_field), grad_body, region, n_region, rule_region, n_rule_region, g, r,
-                    tx, ty,
+                    static_cast<const T*>(body_field), grad_body, region, n_region, keep_pushes, ar, g, r, tx, ty,
                     static_cast<const T*>(pull_src), static_cast<T*>(pull_dst), static_cast<const T*>(pull_field), rr,
                     tx, ty, (int)ptiles);
             } else {

@@ -51,6 +51,85 @@ def transpose_rows(slots: np.ndarray, coeff: np.ndarray):
     return unique.astype(np.int64), start, rows.astype(np.int32), c
 
 
+def keeps_pushes(lattice, shape, region, nodes, tslot, tstart, trow, tcoeff, lanes: int) -> bool:
+    """Whether the fused backward step's IBM workgroup can keep its region's adjoint pushes:
+    the region is the IBM node list itself, at most one node per lane, and the transposed
+    slots are exactly the pushes of those nodes, the push of node x in direction i writing
+    slot (i, x - c_i) (flat i * n + (x - c_i), periodic wrap), whose transposed row list
+    holds the moment rows of node x only: rows a * nn + x (nn IBM nodes; a = 0 density with
+    coefficient 1, a = 1 .. d momentum with c_i,a-1), the nonzero ones, in increasing a."""
+    region, nodes, tslot = np.asarray(region), np.asarray(nodes), np.asarray(tslot)
+    nn, q = len(nodes), lattice.q
+    if not np.array_equal(region, nodes) or nn > lanes or len(tslot) != q * nn:
+        return False
+    c = np.asarray(lattice.c)
+    weight = np.vstack([np.ones(q), c.T])  # (d + 1, q): coefficient of direction i in row block a
+    upstream = np.stack([neighbour_index(region, tuple(-c[i]), shape) for i in range(q)], axis=1)
+    slots = (np.arange(q) * int(np.prod(shape)) + upstream).ravel()
+    at = np.minimum(np.searchsorted(tslot, slots), len(tslot) - 1)  # tslot is sorted and distinct
+    if not (np.all(np.diff(tslot) > 0) and np.array_equal(tslot[at], slots)):
+        return False
+    tstart, trow, tcoeff = np.asarray(tstart), np.asarray(trow), np.asarray(tcoeff)
+    for x in range(nn):
+        for i in range(q):
+            t = at[x * q + i]
+            blocks = np.flatnonzero(weight[:, i])
+            if not (np.array_equal(trow[tstart[t]:tstart[t + 1]], blocks * nn + x)
+                    and np.array_equal(tcoeff[tstart[t]:tstart[t + 1]], weight[blocks, i])):
+                return False
+    return True
+
+
+def slot_writers(lattice, shape, solid, slots):
+    """The adjoint push that writes each lam slot (k, y) of a fluid node y: (node, direction)
+    = (y + c_k, k) when y + c_k is fluid, else y's own bounce-back (y, opp k)."""
+    n = int(np.prod(shape))
+    k, y = np.asarray(slots) // n, np.asarray(slots) % n
+    downstream = neighbour_index(y, tuple(np.asarray(lattice.c)[k].T), shape)
+    fluid = ~solid[downstream]
+    return np.where(fluid, downstream, y), np.where(fluid, k, np.asarray(lattice.opposite)[k])
+
+
+def adjoint_rule_nodes(lattice, shape, solid, target, uslot, ustart, urule, ucoeff):
+    """The link-rule node tables of the fused backward step, or None when the rule set does
+    not allow them: every rule's target and term slots written by the adjoint step of one
+    node (the rule's node), no term slot a target, and each transposed row's entries in
+    increasing target direction with no (slot, rule) pair twice. Returns (nodes, info,
+    coeff): the rule nodes (sorted), info (count, 4) uint32 = {node, blocked directions |
+    target directions << 16, 0, 0} (blocked: x - c_i solid), coeff (count, q, q) with
+    coeff[k, i, j] = the coefficient of the rule whose target push is j in the transposed row
+    of the slot push i writes."""
+    solid = np.asarray(solid, dtype=bool).ravel()
+    target, uslot = np.asarray(target), np.asarray(uslot)
+    ustart, urule, ucoeff = np.asarray(ustart), np.asarray(urule), np.asarray(ucoeff)
+    if len(target) == 0 or np.intersect1d(uslot, target).size:
+        return None
+    tx, tj = slot_writers(lattice, shape, solid, target)  # per rule
+    ux, ui = slot_writers(lattice, shape, solid, uslot)  # per term slot
+    entry_slot = np.repeat(np.arange(len(uslot)), np.diff(ustart))
+    if np.any(tx[urule] != ux[entry_slot]) or np.any(solid[tx]):
+        return None
+    same_slot = entry_slot[1:] == entry_slot[:-1]
+    if np.any(same_slot & (tj[urule][1:] <= tj[urule][:-1])):  # increasing j, no rule twice in a row
+        return None
+    nodes, node_of_rule = np.unique(tx, return_inverse=True)
+    q = lattice.q
+    coeff = np.zeros((len(nodes), q, q))
+    coeff[node_of_rule[urule], ui[entry_slot], tj[urule]] = ucoeff
+    targets = np.zeros(len(nodes), dtype=np.int64)
+    np.bitwise_or.at(targets, node_of_rule, np.left_shift(1, tj))
+    if np.any(np.bincount(node_of_rule, minlength=len(nodes)) != [bin(t).count("1") for t in targets]):
+        return None  # two rules of one node with the same target push
+    c = np.asarray(lattice.c)
+    blocked = np.zeros(len(nodes), dtype=np.int64)
+    for i in range(q):
+        blocked |= solid[neighbour_index(nodes, tuple(-c[i]), shape)].astype(np.int64) << i
+    info = np.zeros((len(nodes), 4), dtype=np.uint32)
+    info[:, 0] = nodes
+    info[:, 1] = blocked | (targets << 16)
+    return nodes, info, coeff
+
+
 def marker_gradient(markers: np.ndarray, stencil_base: np.ndarray, grad_w: np.ndarray) -> np.ndarray:
     """dL/dX from dL/dw, w_ls = prod_a phi(p_ls,a - X_l,a): dw/dX_l,a = -phi'(r_a) prod_{b!=a} phi(r_b)."""
     n, d = markers.shape
@@ -188,45 +267,51 @@ class DragAdjoint:
 
     def _setup_fused_step(self) -> None:
         """The one-launch backward step (``lbm_adj_fused_step``) when the library takes this
-        problem and the forward recompute can run its link rules inside the pull step (a rule
-        set in which no rule reads another's target). Its regions: the writers of the slots the
-        IBM adjoint reads or updates (the stencil nodes) and those of the rule adjoint's slots,
-        kept apart. Slot (k, y) of a fluid node is written by the push of node y + c_k when that
-        node is fluid, else by y's own bounce-back."""
+        problem, the forward recompute can run its link rules inside the pull step (a rule set
+        in which no rule reads another's target) and the rule adjoint can run inside the rule
+        nodes' own adjoint step (``adjoint_rule_nodes``). Its IBM region: the writers of the
+        slots the IBM adjoint reads or updates (the stencil nodes), apart from the rule nodes.
+        When the region is the IBM node list itself, the IBM workgroup keeps its region's pushes
+        and nu (``keeps_pushes``)."""
         s, torch = self.solver, self._torch
         self._fused = None
         ibm, r = s._ibm, self._rules
         pack = s._rule_passes.get("pull")
-        if pack is not None and (s._links is None or not pack.direct):
+        if pack is not None and (s._links is None or not pack.direct or not self._slots_disjoint):
             return
         bx, by = s.block
         ok = ctypes.c_int()
         s._lib.call("lbm_adj_fused_supported", s._lattice_code, s._dtype_code, ibm.n_nodes, ibm.n_markers,
-                    ibm.iterations, r["n"] if r else 0, len(self._tslot), self._slots_disjoint, bx, by,
-                    ctypes.byref(ok))
+                    ibm.iterations, len(self._tslot), bx, by, ctypes.byref(ok))
         if not ok.value:
             return
         p, n = s.params, s._n
         solid = np.zeros(n, dtype=bool) if p.solid is None else np.asarray(p.solid, dtype=bool).ravel()
-
-        def writers(slots):
-            k, y = slots // n, slots % n
-            downstream = neighbour_index(y, tuple(np.asarray(p.lattice.c)[k].T), p.shape)
-            return np.unique(np.where(solid[downstream], y, downstream))
-
-        region = writers(self._tslot.cpu().numpy())
-        rule_region = np.zeros(0, dtype=np.int64)
+        region = np.unique(slot_writers(p.lattice, p.shape, solid, self._tslot.cpu().numpy())[0])
+        rule_nodes = np.zeros(0, dtype=np.int64)
+        info = coeff = None
         if r is not None:
-            rule_region = writers(np.concatenate([r["uslot"].cpu().numpy(), r["target"].cpu().numpy()]))
-        both = np.concatenate([region, rule_region])
-        if solid[both].any() or np.intersect1d(region, rule_region).size:
+            tables = adjoint_rule_nodes(p.lattice, p.shape, solid, r["target"].cpu().numpy(),
+                                        r["uslot"].cpu().numpy(), r["ustart"].cpu().numpy(),
+                                        r["urule"].cpu().numpy(), r["ucoeff"].cpu().numpy())
+            if tables is None:
+                return
+            rule_nodes, info, coeff = tables
+        both = np.concatenate([region, rule_nodes])
+        if solid[both].any() or np.intersect1d(region, rule_nodes).size:
             return
         if (s._kind.cpu().numpy()[region] & 4).any():  # the IBM region takes no near-solid node
             return
         kind = s._kind.clone()
         kind[torch.as_tensor(both, device=s.device)] |= 16  # KIND_ADJ_REGION
-        self._fused = {"region": torch.as_tensor(region.astype(np.int32), device=s.device),
-                       "rule_region": torch.as_tensor(rule_region.astype(np.int32), device=s.device), "kind": kind}
+        tslot = self._tslot.cpu().numpy()
+        keep = keeps_pushes(p.lattice, p.shape, region, ibm.nodes, tslot, self._tstart.cpu().numpy(),
+                            self._trow.cpu().numpy(), self._tcoeff.cpu().numpy(), bx * by)
+        self._fused = {"region": torch.as_tensor(region.astype(np.int32), device=s.device), "kind": kind,
+                       "keep": int(keep), "rule_nodes": len(rule_nodes),
+                       "rule_info": None if info is None else torch.as_tensor(info.view(np.int32), device=s.device),
+                       "rule_coeff": None if coeff is None else torch.as_tensor(coeff.astype(s._np_dtype),
+                                                                                device=s.device)}
 
     # ------------------------------------------------------------------ controls
 
@@ -397,11 +482,6 @@ class DragAdjoint:
         fx, fy, fz_ = s._force3
         wp, wm = s._rates
         bx, by = s.block
-        if r is None:
-            rules = (None, 0, None, None, None, None, 0)
-        else:
-            rules = (pointer(r["target"]), r["n"], pointer(r["uslot"]), pointer(r["ustart"]), pointer(r["urule"]),
-                     pointer(r["ucoeff"]), r["nu"])
         if partner is not None and r is not None and s._links is None:
             raise RuntimeError("the fused backward step needs the solver's in-step link rules for the recompute")
         if partner is None or s._links is None:
@@ -416,11 +496,11 @@ class DragAdjoint:
                 pointer(s._ibm_weights), pointer(s._ibm_areas), ibm.n_markers, pointer(s._ibm_start),
                 pointer(s._ibm_entry), ibm.iterations, fx, fy, fz_, s._n, pointer(self._nu), weight.data_ptr(),
                 pointer(lam_out), pointer(self._tslot), pointer(self._tstart), pointer(self._trow),
-                pointer(self._tcoeff), len(self._tslot), pointer(self._grad_w), *rules,
+                pointer(self._tcoeff), len(self._tslot), pointer(self._grad_w),
                 None if record is None else pointer(record), pointer(lam_next), pointer(fz["kind"]), s._wall_ptr,
                 pointer(field), pointer(s._body), s.params.nx, s.params.ny, s._nz, wp, wm, pointer(self._grad_body),
-                pointer(fz["region"]), len(fz["region"]), pointer(fz["rule_region"]), len(fz["rule_region"]), bx, by,
-                src, dst, pfield, *links, s._stream)
+                pointer(fz["region"]), len(fz["region"]), fz["keep"], pointer(fz["rule_info"]),
+                pointer(fz["rule_coeff"]), fz["rule_nodes"], bx, by, src, dst, pfield, *links, s._stream)
 
     @property
     def adjoint_block(self) -> tuple[int, int]:

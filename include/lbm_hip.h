@@ -373,38 +373,48 @@ LBM_API int lbm_ibm_adjoint_pull(int lattice, int dtype, const void* f, const in
                                  double omega_minus, int force_mode, int mask_mode, void* stream);
 
 /* Whether lbm_adj_fused_step takes this problem: float32 D2Q9, bx * by in 64 .. 1024, and an
- * IBM adjoint that stages its tables, rule adjoints and transposed rows in LDS with
- * slots_disjoint = 1 (*ok = 1). No counterpart in the reference. Host only. */
-LBM_API int lbm_adj_fused_supported(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int n_rules,
-                                    int n_tslots, int slots_disjoint, int bx, int by, int* ok);
+ * IBM adjoint that stages its tables and transposed rows in LDS (*ok = 1). No counterpart in
+ * the reference. Host only. */
+LBM_API int lbm_adj_fused_supported(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int n_tslots,
+                                    int bx, int by, int* ok);
 
 /* One backward step of the checkpointed adjoint in one launch (no counterpart in the
  * reference): lbm_adj_pull_step (force mode dense, with kind bytes) from lam_next into lam,
- * the IBM and link-rule adjoints of lbm_ibm_adjoint (same arguments; scratch buffers not
- * needed, slots_disjoint = 1 required), and, when pull_src is not null, an independent
+ * the IBM adjoint of lbm_ibm_adjoint (same arguments, no rules here; scratch not needed), the
+ * transposed link rules of lbm_adj_rules and, when pull_src is not null, an independent
  * forward step from pull_src to pull_dst with the link rules evaluated inside it (as
  * lbm_pull_step_links, force field pull_field) that also writes each rule's value into the
- * slot of pull_src the rule pre-pass would have written. `region` lists the nodes whose
- * adjoint step the IBM workgroup runs itself (the writers of the transposed moment slots:
- * the IBM stencil nodes), `rule_region` those of the workgroup that runs the transposed link
- * rules (the writers of every rule target and rule term slot; each slot (k, y) of a fluid
- * node has one writer, the node y + c_k when that node is fluid, else y). The two lists and
- * their slot sets must not meet; the listed nodes' kind bytes carry bit 16 and the other
- * workgroups skip them. Same bits as lbm_adj_pull_step_rules followed by
- * lbm_ibm_adjoint_pull. One workgroup shape bx x by for every part. */
+ * slot of pull_src the rule pre-pass would have written. Each lam slot (k, y) of a fluid node
+ * has one writer: the push of node y + c_k when that node is fluid, else y's own bounce-back.
+ * `region` lists the nodes whose adjoint step the IBM workgroup runs itself (the writers of
+ * the transposed moment slots: the IBM stencil nodes). keep_pushes = 1 declares that the
+ * region is the IBM node list itself (n_region = n_nodes <= bx * by) and that the transposed
+ * slots are exactly the pushes of those nodes, slot (i, x - c_i) listing the moment rows of
+ * node x only (density with coefficient 1, then momentum a with c_i,a where nonzero, in row
+ * order); the IBM workgroup then keeps the pushes and nu in registers, adds the moment rows
+ * itself and neither reads nor stages tslot, tstart, trow and tcoeff. The link rules come as
+ * arule_nodes rule nodes (arule_info 16-byte aligned): a rule set in which the target and
+ * every term slot of a rule are written by one node, the rule's node, and no term slot is a
+ * target; arule_info[4 k] = node (sorted), arule_info[4 k + 1] = the directions i whose
+ * upstream node is solid | the directions j whose push writes one of the node's rule targets
+ * << 16, the rest zero; arule_coeff[k][i][j] (q x q) = the coefficient of the rule with
+ * target push j in the transposed row of the slot push i writes, zero where none, with each
+ * transposed row's entries in increasing j. The IBM region and the rule nodes must not meet
+ * and their kind bytes carry bit 16 (the other workgroups skip them). Same bits as
+ * lbm_adj_pull_step_rules followed by lbm_ibm_adjoint_pull. One workgroup shape bx x by for
+ * every part. */
 LBM_API int lbm_adj_fused_step(
     int lattice, int dtype, const void* f, const int64_t* slot, const void* coeff, const void* constant, int m,
     const int64_t* nodes, int n_nodes, const int* inverse, const void* weights, const void* areas, int n_markers,
     const int* spread_start, const int* spread_entry, int iterations, double fx, double fy, double fz, int64_t n_total,
     void* nu, const double* body_weight, void* lam, const int64_t* tslot, const int* tstart, const int* trow,
-    const void* tcoeff, int n_tslots, double* grad_w, const int64_t* rule_target, int n_rules,
-    const int64_t* rule_uslot, const int* rule_ustart, const int* rule_urule, const void* rule_ucoeff,
-    int n_rule_unique, const void* record, const void* lam_next, const uint8_t* kind, const void* wall_u,
-    const void* force_field, const void* body_field, int nx, int ny, int nz, double omega_plus, double omega_minus,
-    double* grad_body, const int32_t* region, int n_region, const int32_t* rule_region, int n_rule_region, int bx,
-    int by, const void* pull_src, void* pull_dst, const void* pull_field, const uint32_t* rule_node_info,
-    const int32_t* rule_node_first, int rule_nodes, const int64_t* prule_slot, const void* prule_coeff,
-    const void* prule_constant, int64_t n_prules, int prule_m, void* stream);
+    const void* tcoeff, int n_tslots, double* grad_w, const void* record, const void* lam_next, const uint8_t* kind,
+    const void* wall_u, const void* force_field, const void* body_field, int nx, int ny, int nz, double omega_plus,
+    double omega_minus, double* grad_body, const int32_t* region, int n_region, int keep_pushes,
+    const uint32_t* arule_info, const void* arule_coeff, int arule_nodes, int bx, int by, const void* pull_src,
+    void* pull_dst, const void* pull_field, const uint32_t* rule_node_info, const int32_t* rule_node_first,
+    int rule_nodes, const int64_t* prule_slot, const void* prule_coeff, const void* prule_constant, int64_t n_prules,
+    int prule_m, void* stream);
 
 /* Size in bytes of one lattice descriptor of the batched step (no counterpart in the
  * reference, which runs one lattice per solver). Host only. */

@@ -98,6 +98,117 @@ def test_transposed_rows_are_the_transpose():
     np.testing.assert_allclose(got, expect, atol=1e-14)
 
 
+@pytest.mark.parametrize("rules", [True, False], ids=["zou-he", "walls"])
+def test_ibm_nodes_can_keep_their_pushes(rules):
+    """The fused backward step's IBM workgroup keeps the adjoint pushes of the IBM stencil
+    nodes and adds their moment rows itself when the transposed row lists are those rows:
+    slot (i, x - c_i) lists the density row of node x with coefficient 1 and its momentum rows
+    with c_i (nonzero ones), checked here by coordinates with periodic wrap. A region that is
+    not the node list or larger than a workgroup, a transposed row list with another
+    coefficient, row or order, or a slot list with a slot missing keeps nothing."""
+    from lbm.adjoint import keeps_pushes, transpose_rows
+    from lbm.ibm import ImmersedForcing
+
+    params = case(dtype="float32", rules=rules)
+    ibm = ImmersedForcing(params)
+    compiled = ibm.compiled("pull")
+    tslot, tstart, trow, tcoeff = transpose_rows(np.asarray(compiled.slots), np.asarray(compiled.coeff))
+    ny, nx = params.shape
+    nn = ibm.n_nodes
+    for q, node in enumerate(ibm.nodes):
+        y, x = divmod(int(node), nx)
+        for i, (cx, cy) in enumerate(D2Q9.c):
+            t = np.searchsorted(tslot, i * nx * ny + ((y - cy) % ny) * nx + (x - cx) % nx)
+            rows = list(trow[tstart[t]:tstart[t + 1]])
+            coeffs = list(tcoeff[tstart[t]:tstart[t + 1]])
+            want = [(a * nn + q, w) for a, w in enumerate((1, cx, cy)) if w != 0]
+            assert list(zip(rows, coeffs, strict=True)) == want
+    args = (D2Q9, params.shape, ibm.nodes, ibm.nodes, tslot, tstart, trow, tcoeff)
+    assert keeps_pushes(*args, 1024)
+    assert not keeps_pushes(*args, nn - 1)
+    assert not keeps_pushes(D2Q9, params.shape, ibm.nodes[1:], *args[3:], 1024)
+    bad = tcoeff.copy()
+    bad[7] = 2.0
+    assert not keeps_pushes(*args[:7], bad, 1024)
+    swapped = trow.copy()
+    swapped[[3, 4]] = swapped[[4, 3]]
+    assert not keeps_pushes(*args[:6], swapped, tcoeff, 1024)
+    assert not keeps_pushes(*args[:4], tslot[1:], tstart[1:] - tstart[1], trow[tstart[1]:], tcoeff[tstart[1]:], 1024)
+
+
+def _rule_csr(params):
+    """Rule targets and the transposed link rules (pull layout) of a case."""
+    from lbm.adjoint import transpose_rows
+    from lbm.link_rules import compile_rules
+
+    compiled = compile_rules(params.lattice, params.link_rules, "pull", params.solid, params.wall_velocity)
+    return (np.asarray(compiled.target),
+            *transpose_rows(np.asarray(compiled.values.slots), np.asarray(compiled.values.coeff)))
+
+
+def test_rule_node_tables_apply_the_transposed_link_rules_bit_for_bit():
+    """The fused backward step applies the transposed link rules inside each rule node's
+    adjoint step from per-node tables. On random float32 adjoint values, the tables applied
+    node by node (a_j = the push of each target direction j, which becomes zero; push i gets
+    sum over j of coeff[i][j] a_j in increasing j, added from 0, where the row has entries)
+    give the bits of the transposed rules applied to the whole slot array (a_r = lam at each
+    target, zeroed, then each term slot += its (rule, coefficient) list in order). The push
+    of node x in direction i writes (opp i, x) when x - c_i is solid, else (i, x - c_i),
+    computed here by coordinates."""
+    from lbm.adjoint import adjoint_rule_nodes
+
+    params = case(dtype="float32", rules=True)
+    target, uslot, ustart, urule, ucoeff = _rule_csr(params)
+    solid = np.asarray(params.solid, dtype=bool)
+    tables = adjoint_rule_nodes(D2Q9, params.shape, solid, target, uslot, ustart, urule, ucoeff)
+    assert tables is not None
+    nodes, info, coeff = tables
+    ny, nx = params.shape
+    n = nx * ny
+    rng = np.random.default_rng(4)
+    lam = rng.standard_normal(9 * n).astype(np.float32)
+
+    expect = lam.copy()
+    a = expect[target].copy()
+    expect[target] = 0
+    for u, s in enumerate(uslot):
+        acc = np.float32(0)
+        for e in range(ustart[u], ustart[u + 1]):
+            acc = np.float32(acc + np.float32(ucoeff[e]) * a[urule[e]])
+        expect[s] = np.float32(expect[s] + acc)
+
+    got = lam.copy()
+    for k, x in enumerate(nodes):
+        y0, x0 = divmod(int(x), nx)
+        assert info[k, 0] == x
+        slots = []
+        for i, (cx, cy) in enumerate(D2Q9.c):
+            up = ((y0 - cy) % ny, (x0 - cx) % nx)
+            blocked = bool(solid[up])
+            assert bool((info[k, 1] >> i) & 1) == blocked
+            slots.append(D2Q9.opposite[i] * n + x if blocked else i * n + up[0] * nx + up[1])
+        targets = [j for j in range(9) if (info[k, 1] >> (16 + j)) & 1]
+        aj = {j: got[slots[j]] for j in targets}
+        for j in targets:
+            got[slots[j]] = 0
+        for i in range(9):
+            acc, any_entry = np.float32(0), False
+            for j in targets:
+                c = np.float32(coeff[k, i, j])
+                if c != 0:
+                    acc, any_entry = np.float32(acc + c * aj[j]), True
+            if any_entry:
+                got[slots[i]] = np.float32(got[slots[i]] + acc)
+    assert np.array_equal(got.view(np.uint32), expect.view(np.uint32))
+
+    # a rule that reads a slot another node writes, or a term slot that is a target, gets no tables
+    moved = uslot.copy()
+    moved[0] = (moved[0] + nx) % (9 * n)
+    assert adjoint_rule_nodes(D2Q9, params.shape, solid, target, moved, ustart, urule, ucoeff) is None
+    assert adjoint_rule_nodes(D2Q9, params.shape, solid, target, np.append(uslot[:-1], target[0]), ustart,
+                              urule, ucoeff) is None
+
+
 @pytest.mark.parametrize("rules", [False, True], ids=["walls", "zou-he"])
 def test_torch_model_reproduces_the_oracle_drag(rules):
     params = case(rules=rules)
@@ -358,18 +469,21 @@ def test_fused_backward_step_gives_the_same_bits_as_two_launches(rules):
     params = case(dtype="float32", rules=rules)
     g = np.random.default_rng(9).standard_normal((2, *params.shape)) * 1e-6 * params.fluid
     runs = []
-    for fused in (True, False):
+    for mode in ("kept", "fused", "two launches"):
         problem = DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params))
-        assert problem._fused is not None
-        if not fused:
+        assert problem._fused is not None and problem._fused["keep"] == 1
+        if mode == "fused":  # the IBM workgroup writes its region's pushes and nu and reads them back
+            problem._fused["keep"] = 0
+        if mode == "two launches":
             problem._fused = None
         problem.set_body_force(g)
         loss = problem.forward()
         runs.append((loss, *problem.backward()))
-    assert runs[0][0] == runs[1][0]
-    for a, b in zip(runs[0][1:], runs[1][1:], strict=True):
-        assert np.array_equal(a, b)
-        assert np.any(a != 0)
+    for other in runs[1:]:
+        assert runs[0][0] == other[0]
+        for a, b in zip(runs[0][1:], other[1:], strict=True):
+            assert np.array_equal(a, b)
+            assert np.any(a != 0)
 
 
 @pytest.mark.gpu

@@ -6,6 +6,7 @@
 #include "lbm_affine.h"
 #include "lbm_common.h"
 #include "lbm_ibm_core.h"
+#include "lbm_rule_layer.h"
 
 namespace lbm {
 
@@ -42,29 +43,37 @@ inline int ibm_threads(int d, int nn)
 // stay in global memory (large bodies). RECORD: every iteration keeps its own u, U, F_l and
 // force slot and the forward part of the layout is copied to `record` at the end (written
 // in place when not staged), for the adjoint to read back. The arithmetic is the same in
-// all four variants.
-template <class T, int D, bool STAGED, bool RECORD>
-__global__ __launch_bounds__(ibm_max_threads(D)) void k_ibm(const T* __restrict__ F, const int64_t* __restrict__ slot,
-                                                     const T* __restrict__ coeff, const T* __restrict__ constant,
-                                                     int m, const int64_t* __restrict__ nodes, int nn,
-                                                     const int* __restrict__ inverse_g, const T* __restrict__ weights_g,
-                                                     const T* __restrict__ areas_g, int nm,
-                                                     const int* __restrict__ start_g,
-                                                     const int* __restrict__ entry_g, int iterations, T hfx, T hfy,
-                                                     T hfz, uint32_t n_total, T* __restrict__ field,
-                                                     T* __restrict__ body_force, T* __restrict__ scratch,
-                                                     T* __restrict__ record)
+// all four variants. `before` runs ahead of the first loads (a caller's own loads go out
+// with them) and `after(last)` once the field is written, `last` being the final force at
+// the stencil nodes (component a of node n at last[a nn + n]).
+struct IbmNoHook {
+    __device__ void operator()() const {}
+    template <class U>
+    __device__ void operator()(const U*) const
+    {
+    }
+};
+
+template <class T, int D, bool STAGED, bool RECORD, class Before = IbmNoHook, class After = IbmNoHook>
+__device__ __forceinline__ void ibm_force_group(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                                const T* __restrict__ coeff, const T* __restrict__ constant, int m,
+                                                const int64_t* __restrict__ nodes, int nn,
+                                                const int* __restrict__ inverse_g, const T* __restrict__ weights_g,
+                                                const T* __restrict__ areas_g, int nm, const int* __restrict__ start_g,
+                                                const int* __restrict__ entry_g, int iterations, T hfx, T hfy, T hfz,
+                                                uint32_t n_total, T* __restrict__ field, T* __restrict__ body_force,
+                                                T* __restrict__ scratch, T* __restrict__ record, int tid, int nthreads,
+                                                Before before = Before{}, After after = After{})
 {
     constexpr int S = D == 2 ? 9 : 27;
     extern __shared__ __align__(16) unsigned char ibm_lds[];
-    const int tid = threadIdx.x;
-    const int nthreads = blockDim.x;
     const int slots = RECORD ? iterations : 1;
     IbmForward<T> s;
     IbmTables<T> tb{inverse_g, weights_g, areas_g, start_g, entry_g};
     T* total;
     T* base;
     const int rows = (D + 1) * nn;
+    before();
     // With one moment row per lane (2D), the row's coefficients and slots go out before the
     // table copies, so its population gathers wait one latency after the tables, not two.
     constexpr int AHEAD = 9;
@@ -104,10 +113,113 @@ __global__ __launch_bounds__(ibm_max_threads(D)) void k_ibm(const T* __restrict_
     for (int n = tid; n < nn; n += nthreads)
         for (int a = 0; a < D; ++a) field[(size_t)a * n_total + nodes[n]] = last[(size_t)a * nn + n];
     if (tid < D) body_force[tid] = -total[tid];
+    after(last);
     if constexpr (STAGED && RECORD) {
         const WordCopy out = word_copy(record, base, ibm_record_elems(D, nn, nm, iterations));
         copy_from_lds(out.dst, out.src, out.words, tid, nthreads);
     }
+}
+
+template <class T, int D, bool STAGED, bool RECORD>
+__global__ __launch_bounds__(ibm_max_threads(D)) void k_ibm(const T* __restrict__ F, const int64_t* __restrict__ slot,
+                                                     const T* __restrict__ coeff, const T* __restrict__ constant,
+                                                     int m, const int64_t* __restrict__ nodes, int nn,
+                                                     const int* __restrict__ inverse_g, const T* __restrict__ weights_g,
+                                                     const T* __restrict__ areas_g, int nm,
+                                                     const int* __restrict__ start_g,
+                                                     const int* __restrict__ entry_g, int iterations, T hfx, T hfy,
+                                                     T hfz, uint32_t n_total, T* __restrict__ field,
+                                                     T* __restrict__ body_force, T* __restrict__ scratch,
+                                                     T* __restrict__ record)
+{
+    ibm_force_group<T, D, STAGED, RECORD>(F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g, nm,
+                                          start_g, entry_g, iterations, hfx, hfy, hfz, n_total, field, body_force,
+                                          scratch, record, (int)threadIdx.x, (int)blockDim.x);
+}
+
+// ------------------------------------------------------------------ forcing inside the step
+//
+// One forward step of a body-forced lattice in one launch: the IBM forcing of the state
+// `src` and the pull step from `src` to `dst` with that forcing (what lbm_ibm_force followed
+// by lbm_pull_step_links computes; one-dimensional grid, one workgroup shape):
+//   workgroup 0     the forcing (ibm_force_group, staged) and then the pull update of the IBM
+//                   stencil nodes, node n by lane n: their populations are loaded with the
+//                   forcing's first loads and their force is the one just computed, read
+//                   from LDS (the value the field then holds);
+//   next `chunks`   the link-rule nodes (rule_chunk, as k_pull_links);
+//   the rest        the pull update of every other node (tiles; the stencil nodes carry
+//                   KIND_ADJ_REGION in this launch's kind bytes and are skipped).
+// Only the stencil nodes are forced (KIND_FORCED), so no other workgroup reads the field the
+// forcing writes; none has a solid upstream neighbour or link rules (the host checks), so
+// their update is the plain pull with the same operations as pull_update: the same bits.
+template <class L, class T, ForceMode FM, bool RECORD>
+__global__ __launch_bounds__(1024) void k_ibm_step(const T* __restrict__ src, const int64_t* __restrict__ slot,
+                                                   const T* __restrict__ coeff, const T* __restrict__ constant, int m,
+                                                   const int64_t* __restrict__ nodes, int nn,
+                                                   const int* __restrict__ inverse_g, const T* __restrict__ weights_g,
+                                                   const T* __restrict__ areas_g, int nm,
+                                                   const int* __restrict__ start_g, const int* __restrict__ entry_g,
+                                                   int iterations, T hfx, T hfy, T hfz, T* __restrict__ field,
+                                                   T* __restrict__ body_force, T* __restrict__ record,
+                                                   T* __restrict__ dst, const uint8_t* __restrict__ kind,
+                                                   const T* __restrict__ wall_u, const T* __restrict__ body, Grid g,
+                                                   Relaxation<T> r, RuleRows<T> rules, int gx, int gy)
+{
+    const int b = (int)blockIdx.x;
+    const int lanes = (int)(blockDim.x * blockDim.y);
+    const int tid = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    if (b == 0) {
+        const bool has = tid < nn;
+        uint32_t node = 0;
+        uint8_t k = 0;
+        T f[L::Q], bd[3];
+        auto before = [&] {
+            if (!has) return;
+            node = (uint32_t)nodes[tid];
+            const uint32_t rest = node / (uint32_t)g.nx;
+            const int x = (int)(node - rest * (uint32_t)g.nx), y = (int)(rest % (uint32_t)g.ny),
+                      z = (int)(rest / (uint32_t)g.ny);
+            k = kind[node];
+            static_for<0, L::Q>([&](auto I) { f[I] = src[(size_t)I * g.ns + neighbour<L, I, -1>(x, y, z, g)]; });
+#pragma unroll
+            for (int a = 0; a < L::D; ++a) bd[a] = FM == ForceMode::DENSE ? body[(size_t)a * g.n + node] : T(0);
+        };
+        auto after = [&](const T* last) {
+            if (!has) return;
+            // node_force with the field value of this node taken from the forcing
+            T Fx = r.force[0], Fy = r.force[1], Fz = r.force[2];
+            if constexpr (FM == ForceMode::DENSE) {
+                Fx = Fx + bd[0];
+                Fy = Fy + bd[1];
+                if constexpr (L::D == 3) Fz = Fz + bd[2];
+            }
+            if (k & KIND_FORCED) {
+                Fx = Fx + last[tid];
+                Fy = Fy + last[(size_t)nn + tid];
+                if constexpr (L::D == 3) Fz = Fz + last[2 * (size_t)nn + tid];
+            }
+            collide<L, FM>(f, r, Fx, Fy, Fz);
+            static_for<0, L::Q>([&](auto I) { dst[(size_t)I * g.ns + node] = f[I]; });
+        };
+        ibm_force_group<T, L::D, true, RECORD>(src, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g,
+                                               nm, start_g, entry_g, iterations, hfx, hfy, hfz, g.n, field, body_force,
+                                               nullptr, record, tid, lanes, before, after);
+        return;
+    }
+    const int chunks = rules.count ? rule_chunk_count(rules.count, rules.chunk) : 0;
+    if (b - 1 < chunks) {
+        extern __shared__ double dynamic_lds[];
+        rule_chunk<L, T, FM>(src, dst, kind, wall_u, field, body, g, r, rules, b - 1, lanes, tid,
+                             reinterpret_cast<T*>(dynamic_lds));
+        return;
+    }
+    const int tile = b - 1 - chunks;
+    Coord c;
+    c.x = (tile % gx) * (int)blockDim.x + (int)threadIdx.x;
+    c.y = ((tile / gx) % gy) * (int)blockDim.y + (int)threadIdx.y;
+    c.z = tile / (gx * gy);
+    if (c.x >= g.nx || c.y >= g.ny) return;
+    pull_update<L, T, FM, true, true, true>(src, dst, kind, wall_u, field, body, g, r, c);
 }
 
 // ------------------------------------------------------------------ many workgroups
@@ -341,11 +453,105 @@ void ibm_force_multi(const T* f, const int64_t* slot, const T* coeff, const T* c
     }
 }
 
+// Whether k_ibm_step takes the forcing: D2Q9, bx * by in 64 .. 1024 lanes, at least one lane
+// per stencil node, and the staged forcing (with `record`'s slots) in the LDS.
+bool ibm_step_ok(int lattice, int dtype, int n_nodes, int n_markers, int iterations, int record, int bx, int by)
+{
+    if (lattice != LBM_D2Q9 || (dtype != LBM_F32 && dtype != LBM_F64) || bx <= 0 || by <= 0) return false;
+    const int lanes = bx * by;
+    if (lanes < RULE_CHUNK_MAX || lanes > 1024 || n_nodes <= 0 || n_nodes > lanes || n_markers <= 0 ||
+        iterations < 1)
+        return false;
+    const size_t real = dtype == LBM_F32 ? sizeof(float) : sizeof(double);
+    return ibm_lds_bytes(2, n_nodes, n_markers, record ? iterations : 1, real) <= lds_limit();
+}
+
 }  // namespace lbm
 
 using namespace lbm;
 
 extern "C" {
+
+LBM_API int lbm_ibm_pull_step_supported(int lattice, int dtype, int n_nodes, int n_markers, int iterations,
+                                        int record, int bx, int by, int* ok)
+{
+    return guarded("lbm_ibm_pull_step_supported", [&] {
+        require(ok != nullptr, "null output");
+        *ok = ibm_step_ok(lattice, dtype, n_nodes, n_markers, iterations, record, bx, by) ? 1 : 0;
+    });
+}
+
+LBM_API int lbm_ibm_pull_step(int lattice, int dtype, const void* src, void* dst, const uint8_t* kind,
+                              const void* wall, void* field, const void* body, int nx, int ny, int nz,
+                              double omega_plus, double omega_minus, double fx, double fy, double fz, int force_mode,
+                              const int64_t* slot, const void* coeff, const void* constant, int m,
+                              const int64_t* nodes, int n_nodes, const int* inverse, const void* weights,
+                              const void* areas, int n_markers, const int* spread_start, const int* spread_entry,
+                              int iterations, void* body_force, void* record, const uint32_t* rule_node_info,
+                              const int32_t* rule_node_first, int rule_nodes, const int64_t* rule_slot,
+                              const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m, int bx,
+                              int by, void* stream)
+{
+    return guarded("lbm_ibm_pull_step", [&] {
+        require(ibm_step_ok(lattice, dtype, n_nodes, n_markers, iterations, record != nullptr, bx, by),
+                "the forcing inside the step takes D2Q9, 64 .. 1024 lanes, a lane per stencil node and a forcing "
+                "that fits the LDS (lbm_ibm_pull_step_supported)");
+        const Grid g = make_grid(lattice, nx, ny, nz);
+        require(src && dst && src != dst && kind && field && body_force, "null or aliased step buffers");
+        require(force_mode == 1 || force_mode == 2, "the forced step uses force mode field (1) or dense (2)");
+        require(force_mode != 2 || body != nullptr, "dense force mode needs a body field");
+        require(m >= 1 && m <= 128, "moment rows need 1 <= m <= 128 terms");
+        require(slot && coeff && constant && nodes && inverse && weights && areas && spread_start && spread_entry,
+                "null IBM table");
+        require(n_rules >= 0 && n_rules < (1ll << 31), "rule count out of range");
+        require(rule_nodes >= 0 && (n_rules > 0) == (rule_nodes > 0), "rows need rule nodes and nodes rows");
+        require(n_rules == 0 || (rule_node_info && rule_node_first && rule_slot && rule_coeff && rule_constant &&
+                                 rule_m >= 1 && rule_m <= 128),
+                "link rules need the node list, slot, coeff, constant and 1 <= m <= 128");
+        require(reinterpret_cast<uintptr_t>(rule_node_info) % 16 == 0, "rule_node_info must be 16-byte aligned");
+        const double force[3] = {fx, fy, fz};
+        const dim3 block((unsigned)bx, (unsigned)by, 1);
+        dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
+            using L = decltype(Lt);
+            using T = decltype(Tt);
+            if constexpr (L::D == 2) {
+                const Relaxation<T> r = make_relaxation<L, T>(omega_plus, omega_minus, force);
+                const dim3 tiles = grid_of(g, block);
+                const int64_t n_tiles = (int64_t)tiles.x * tiles.y * tiles.z;
+                const int chunk = rule_chunk_nodes(rule_nodes, (int)std::min<int64_t>(n_tiles, 1 << 20));
+                const int chunks = rule_nodes ? rule_chunk_count(rule_nodes, chunk) : 0;
+                const RuleRows<T> rr{reinterpret_cast<const uint4*>(rule_node_info), rule_node_first, rule_nodes,
+                                     chunk, rule_slot, static_cast<const T*>(rule_coeff),
+                                     static_cast<const T*>(rule_constant), n_rules, rule_m};
+                require(1 + chunks + n_tiles < (1ll << 31), "grid too large for a one-dimensional launch");
+                const size_t lds =
+                    std::max(ibm_lds_bytes(L::D, n_nodes, n_markers, record ? iterations : 1, sizeof(T)),
+                             chunks ? sizeof(T) * (size_t)chunk * (L::Q - 1) : (size_t)0);
+                auto go = [&](auto fm, auto recorded) {
+                    constexpr ForceMode FM = decltype(fm)::value;
+                    constexpr bool RE = decltype(recorded)::value;
+                    allow_lds(k_ibm_step<L, T, FM, RE>, lds);
+                    k_ibm_step<L, T, FM, RE><<<(unsigned)(1 + chunks + n_tiles), block, lds, as_stream(stream)>>>(
+                        static_cast<const T*>(src), slot, static_cast<const T*>(coeff),
+                        static_cast<const T*>(constant), m, nodes, n_nodes, inverse, static_cast<const T*>(weights),
+                        static_cast<const T*>(areas), n_markers, spread_start, spread_entry, iterations,
+                        T(0.5 * fx), T(0.5 * fy), T(0.5 * fz), static_cast<T*>(field), static_cast<T*>(body_force),
+                        static_cast<T*>(record), static_cast<T*>(dst), kind, static_cast<const T*>(wall),
+                        static_cast<const T*>(body), g, r, rr, (int)tiles.x, (int)tiles.y);
+                };
+                using Fi = std::integral_constant<ForceMode, ForceMode::FIELD>;
+                using De = std::integral_constant<ForceMode, ForceMode::DENSE>;
+                if (force_mode == 1 && record) go(Fi{}, std::true_type{});
+                else if (force_mode == 1) go(Fi{}, std::false_type{});
+                else if (record) go(De{}, std::true_type{});
+                else go(De{}, std::false_type{});
+            } else {
+                require(false, "the forcing inside the step takes D2Q9");
+            }
+            check_launch("IBM forcing inside the pull step");
+        });
+    });
+}
 
 LBM_API int lbm_ibm_scratch_size(int lattice, int dtype, int n_nodes, int n_markers, int64_t* bytes)
 {

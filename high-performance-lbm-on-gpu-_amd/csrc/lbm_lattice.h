@@ -72,9 +72,10 @@ constexpr uint8_t KIND_NEAR_SOLID = 4;
 // A fluid node with link rules that lbm_pull_step_links updates in its rule layer (the
 // host sets it on exactly the nodes of the rule-node list it passes; other kernels ignore it).
 constexpr uint8_t KIND_RULES = 8;
-// A node whose adjoint step the fused adjoint launch runs in its IBM workgroup (the nodes
-// whose pushed adjoint populations the IBM and link-rule adjoints read); set only in the
-// adjoint's own copy of the kind bytes.
+// A node that a fused launch updates in a workgroup of its own, so the tile workgroups skip
+// it: in the fused backward step the IBM stencil nodes and the link-rule nodes, in the fused
+// forward step (k_ibm_step) the IBM stencil nodes. Set only in the fused steps' own copies
+// of the kind bytes.
 constexpr uint8_t KIND_ADJ_REGION = 16;
 
 }  // namespace lbm

@@ -101,14 +101,17 @@ __device__ __forceinline__ uint32_t flat(const Coord& c, const Grid& g)
 // batched kernels, so both produce the same bits). Population planes are g.ns elements
 // apart (n for one lattice; padded in the batched pools); wall and force fields g.n.
 // SKIP_RULES: nodes whose kind has KIND_RULES are left to the rule layer of the step
-// (k_pull_links), which updates them with their link rules.
-template <class L, class T, ForceMode FM, bool MASK, bool SKIP_RULES = false>
+// (k_pull_links), which updates them with their link rules. SKIP_REGION: nodes whose kind
+// has KIND_ADJ_REGION are left to a region workgroup of a fused step.
+template <class L, class T, ForceMode FM, bool MASK, bool SKIP_RULES = false, bool SKIP_REGION = false>
 __device__ __forceinline__ void pull_update(const T* __restrict__ src, T* __restrict__ dst,
                                             const uint8_t* __restrict__ kind, const T* __restrict__ wall_u,
                                             const T* __restrict__ field, const T* __restrict__ body, const Grid& g,
                                             const Relaxation<T>& r, const Coord& c)
 {
-    static_assert(MASK || !SKIP_RULES, "the rule nodes are marked in the kind bytes");
+    static_assert(MASK || !(SKIP_RULES || SKIP_REGION), "the skipped nodes are marked in the kind bytes");
+    constexpr uint8_t skip =
+        KIND_SOLID | (SKIP_RULES ? KIND_RULES : uint8_t(0)) | (SKIP_REGION ? KIND_ADJ_REGION : uint8_t(0));
     const uint32_t node = flat(c, g);
     // The node's kind byte and all Q pulled populations are loaded together (the pulled
     // slots are valid addresses whatever the node is; a solid node discards them). Only a
@@ -119,7 +122,7 @@ __device__ __forceinline__ void pull_update(const T* __restrict__ src, T* __rest
     T f[L::Q];
     static_for<0, L::Q>([&](auto I) { f[I] = src[(size_t)I * g.ns + neighbour<L, I, -1>(c.x, c.y, c.z, g)]; });
     if constexpr (MASK) hold_loaded(f);
-    if (MASK && (k & (SKIP_RULES ? (KIND_SOLID | KIND_RULES) : KIND_SOLID))) return;
+    if (MASK && (k & skip)) return;
     if (MASK && (k & KIND_NEAR_SOLID)) {
         const uint32_t blocked = blocked_upstream<L>(kind, c.x, c.y, c.z, g);
         if (blocked) {

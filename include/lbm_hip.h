@@ -278,6 +278,34 @@ LBM_API int lbm_ibm_force_multi(int lattice, int dtype, const void* f, const int
                                 const int* spread_entry, int iterations, double fx, double fy, double fz,
                                 int64_t n_total, void* field, void* body_force, void* scratch, void* stream);
 
+/* Whether lbm_ibm_pull_step takes this body: D2Q9 (float32 or float64), bx * by in
+ * 64 .. 1024 lanes, at least one lane per stencil node, and the one-workgroup forcing
+ * (with `record` slots when record = 1) in the LDS (*ok = 1). No counterpart in the
+ * reference. Host only. */
+LBM_API int lbm_ibm_pull_step_supported(int lattice, int dtype, int n_nodes, int n_markers, int iterations,
+                                        int record, int bx, int by, int* ok);
+
+/* One forward step of a body-forced lattice in one launch: lbm_ibm_force of the state src
+ * (field, body_force and record as there; scratch not needed) followed by
+ * lbm_pull_step_links from src to dst with that field (force mode 1 or 2, link rules as
+ * there, no force rows), with the same outputs bit for bit. The stencil nodes' own pull
+ * update runs in the forcing's workgroup after the forcing, every other node in the other
+ * workgroups; `kind` must carry bit 16 on exactly the stencil nodes (which the other
+ * workgroups then skip), and no stencil node may be solid, next to a solid node or a rule
+ * node. Replaces one iteration of the reference's pull loop: its pre-pass (link rules, then
+ * the IBM force field and its scatter; src/lbm/backends/cuda_solver.py:286-292,
+ * src/lbm/ibm.py:189-214) and its kernel launch (src/lbm/backends/cuda_solver.py:279-284). */
+LBM_API int lbm_ibm_pull_step(int lattice, int dtype, const void* src, void* dst, const uint8_t* kind,
+                              const void* wall, void* field, const void* body, int nx, int ny, int nz,
+                              double omega_plus, double omega_minus, double fx, double fy, double fz, int force_mode,
+                              const int64_t* slot, const void* coeff, const void* constant, int m,
+                              const int64_t* nodes, int n_nodes, const int* inverse, const void* weights,
+                              const void* areas, int n_markers, const int* spread_start, const int* spread_entry,
+                              int iterations, void* body_force, void* record, const uint32_t* rule_node_info,
+                              const int32_t* rule_node_first, int rule_nodes, const int64_t* rule_slot,
+                              const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m, int bx,
+                              int by, void* stream);
+
 /* Bytes of one step's record of the forcing intermediates (optional `record` argument of
  * lbm_ibm_force, read back by lbm_ibm_adjoint), a multiple of 16; records at 16-byte aligned
  * addresses are copied with 16-byte accesses. No counterpart in the reference. Host only. */

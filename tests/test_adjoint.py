@@ -449,6 +449,7 @@ def test_forward_with_rules_in_the_step_gives_the_same_drag_and_gradients(dtype)
         problem._fused = None  # the two-launch backward step in both runs
         if not in_step:
             problem.solver._links = None
+            problem._forcing_kind = None
         problem.set_body_force(g)
         loss = problem.forward()
         runs.append((loss, problem.drag_history, *problem.backward()))
@@ -484,6 +485,34 @@ def test_fused_backward_step_gives_the_same_bits_as_two_launches(rules):
         for a, b in zip(runs[0][1:], other[1:], strict=True):
             assert np.array_equal(a, b)
             assert np.any(a != 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rules", [True, False], ids=["zou-he", "walls"])
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_forcing_inside_the_step_gives_the_same_bits_as_two_launches(rules, dtype):
+    """The one-launch forward step (the IBM forcing, then the stencil nodes' pull update in
+    the forcing's workgroup, every other node in the others) against the forcing launch
+    followed by the pull step: drag history, per-step fields and records, loss and gradients
+    bit for bit."""
+    from lbm.adjoint import DragAdjoint
+
+    params = case(dtype=dtype, rules=rules)
+    g = np.random.default_rng(5).standard_normal((2, *params.shape)) * 1e-6 * params.fluid
+    runs = []
+    for inside in (True, False):
+        problem = DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params))
+        assert problem._forcing_kind is not None
+        if not inside:
+            problem._forcing_kind = None
+        problem.set_body_force(g)
+        loss = problem.forward()
+        runs.append((loss, problem.drag_history, problem._field_hist.cpu().numpy(), problem._records.cpu().numpy(),
+                     *problem.backward()))
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1:], runs[1][1:], strict=True):
+        assert np.array_equal(a, b)
+    assert np.any(runs[0][2] != 0) and np.any(runs[0][3] != 0)
 
 
 @pytest.mark.gpu

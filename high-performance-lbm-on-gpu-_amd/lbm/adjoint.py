@@ -34,7 +34,7 @@ import numpy as np
 
 from lbm.backends import graph_capture
 from lbm.backends._native import FORCE_DENSE, pointer
-from lbm.backends.hip_solver import KIND_NEAR_SOLID, KIND_RULES, KIND_SOLID, HipSolver
+from lbm.backends.hip_solver import KIND_REGION, HipSolver
 from lbm.geometry import neighbour_index
 from lbm.ibm import STENCIL, roma, roma_derivative
 from lbm.link_rules import compile_rules
@@ -304,7 +304,7 @@ class DragAdjoint:
         if (s._kind.cpu().numpy()[region] & 4).any():  # the IBM region takes no near-solid node
             return
         kind = s._kind.clone()
-        kind[torch.as_tensor(both, device=s.device)] |= 16  # KIND_ADJ_REGION
+        kind[torch.as_tensor(both, device=s.device)] |= KIND_REGION
         tslot = self._tslot.cpu().numpy()
         keep = keeps_pushes(p.lattice, p.shape, region, ibm.nodes, tslot, self._tstart.cpu().numpy(),
                             self._trow.cpu().numpy(), self._tcoeff.cpu().numpy(), bx * by)
@@ -315,26 +315,11 @@ class DragAdjoint:
                                                                                 device=s.device)}
 
     def _setup_forcing_in_step(self) -> None:
-        """The one-launch forward step (``lbm_ibm_pull_step``: the IBM forcing and the pull
-        step with its link rules) when the library takes this body at the solver's block, the
-        rules (if any) run inside the pull step, and no stencil node is solid, next to a
-        solid node or a rule node; its kind bytes mark the stencil nodes (bit 16)."""
-        s, ibm, torch = self.solver, self.solver._ibm, self._torch
-        self._forcing_kind = None
-        bx, by = s.block
-        if s._rule_passes.get("pull") is not None and s._links is None:
-            return
-        ok = ctypes.c_int()
-        for record in (0, 1):
-            s._lib.call("lbm_ibm_pull_step_supported", s._lattice_code, s._dtype_code, ibm.n_nodes, ibm.n_markers,
-                        ibm.iterations, record, bx, by, ctypes.byref(ok))
-            if not ok.value:
-                return
-        if (s._kind.cpu().numpy()[ibm.nodes] & (KIND_SOLID | KIND_NEAR_SOLID | KIND_RULES)).any():
-            return
-        kind = s._kind.clone()
-        kind[torch.as_tensor(ibm.nodes, device=s.device)] |= 16  # KIND_ADJ_REGION
-        self._forcing_kind = kind
+        """The one-launch forward step (``lbm_ibm_pull_step``: the IBM forcing and the pull step
+        with its link rules) when the solver can take it with the forcing's record
+        (``HipSolver._forcing_in_step_kind``)."""
+        s = self.solver
+        self._forcing_kind = s._forcing_in_step_kind(record=True) if s._forcing_in_step_kind() is not None else None
 
     # ------------------------------------------------------------------ controls
 
@@ -390,22 +375,10 @@ class DragAdjoint:
     def _forced_step(self, src, dst, field, body_out, record=None) -> None:
         """The IBM forcing of ``src`` and the pull step to ``dst`` in one launch: the same
         field, drag, record and next state as ``_ibm_forward`` followed by ``_step_forward``."""
-        s, ibm, mp = self.solver, self.solver._ibm, self._mp
-        wp, wm = s._rates
-        fx, fy, fz = s._force3
-        bx, by = s.block
-        lk = s._links
-        if lk is not None:
-            rules = (pointer(lk.node_info), pointer(lk.node_first), lk.nodes, pointer(lk.slot), pointer(lk.coeff),
-                     pointer(lk.constant), lk.n, lk.m)
-        else:
-            rules = (None, None, 0, None, None, None, 0, 1)
-        s._call("lbm_ibm_pull_step", s._lattice_code, s._dtype_code, pointer(src), pointer(dst),
-                pointer(self._forcing_kind), s._wall_ptr, pointer(field), pointer(s._body), s.params.nx, s.params.ny,
-                s._nz, wp, wm, fx, fy, fz, s._force_mode, pointer(mp.slot), pointer(mp.coeff), pointer(mp.constant),
-                mp.m, pointer(s._ibm_nodes), ibm.n_nodes, pointer(s._ibm_inverse), pointer(s._ibm_weights),
-                pointer(s._ibm_areas), ibm.n_markers, pointer(s._ibm_start), pointer(s._ibm_entry), ibm.iterations,
-                body_out, pointer(record), *rules, bx, by, s._stream)
+        s = self.solver
+        name, args = s._ibm_step_call(pointer(src), pointer(dst), self._forcing_kind, s._stream, field=pointer(field),
+                                      body_force=body_out, record=pointer(record))
+        s._call(name, *args)
 
     def _step_forward(self, src, dst, field) -> None:
         """One forward pull step from ``src`` to ``dst``. With link rules it evaluates them

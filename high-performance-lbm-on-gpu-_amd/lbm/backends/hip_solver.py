@@ -197,6 +197,9 @@ class _AffinePack:
 
 
 KIND_RULES = 8
+# Nodes a fused launch updates in a workgroup of its own (set only in that launch's copy of
+# the kind bytes; the tile workgroups skip them).
+KIND_REGION = 16
 
 
 # The pull step's link rules go inside the step (one launch per step) up to this many nodes;
@@ -284,6 +287,7 @@ class HipSolver(Solver):
         self._block = block
         self.graphs = graphs
         self._graphs = {}
+        self._forcing_kinds = {}
         p = params
         self._np_dtype = np.dtype(p.dtype)
         self._torch_dtype = {"float32": torch.float32, "float64": torch.float64}[p.dtype]
@@ -518,12 +522,63 @@ class HipSolver(Solver):
                     + self._pre_pass_calls("aa_even", f, stream)
                     + [("lbm_aa_even", (L, D, f, kind, field, body, *args))]]
         a, b = pointer(self._f), pointer(self._f_next)
+        forcing = self._forcing_in_step_kind()
+        if forcing is not None:
+            return [[self._ibm_step_call(src, dst, forcing, stream)] for src, dst in ((a, b), (b, a))]
         if self._links is not None and self.block[0] * self.block[1] >= 64:
             return [self._ibm_calls("pull", src, stream) + [self._pull_links_call(src, dst, stream)]
                     for src, dst in ((a, b), (b, a))]
         name = "lbm_lds_step" if self.variant == "lds" else "lbm_pull_step"
         return [self._pre_pass_calls("pull", src, stream) + [(name, (L, D, src, dst, kind, wall, field, body, *args))]
                 for src, dst in ((a, b), (b, a))]
+
+    def _forcing_in_step_kind(self, record: bool = False):
+        """The kind bytes of ``lbm_ibm_pull_step`` (the stencil nodes carry KIND_REGION) when the
+        pull variant can run its IBM forcing inside the step at the current block, the link
+        rules (if any) in the step and no stencil node solid, next to a solid node or a rule
+        node; else None. ``record``: the forcing also writes its intermediates."""
+        if self.variant != "pull" or self._ibm is None or ("pull" in self._rule_passes and self._links is None):
+            return None
+        key = (self.block, record, self._kind.data_ptr())
+        if key not in self._forcing_kinds:
+            import ctypes
+
+            ibm, (bx, by) = self._ibm, self.block
+            ok = ctypes.c_int()
+            self._lib.call("lbm_ibm_pull_step_supported", self._lattice_code, self._dtype_code, ibm.n_nodes,
+                           ibm.n_markers, ibm.iterations, int(record), bx, by, ctypes.byref(ok))
+            kind = None
+            blocked = KIND_SOLID | KIND_NEAR_SOLID | KIND_RULES
+            if ok.value and not (self._kind.cpu().numpy()[ibm.nodes] & blocked).any():
+                kind = self._kind.clone()
+                kind[self._ibm_nodes] |= KIND_REGION
+            self._forcing_kinds[key] = kind
+        return self._forcing_kinds[key]
+
+    def _ibm_step_call(self, src: int, dst: int, kind, stream, field: int | None = None,
+                       body_force: int | None = None, record: int | None = None) -> tuple:
+        """``lbm_ibm_pull_step`` from ``src`` to ``dst`` (the IBM forcing of ``src``, then the pull
+        step with its link rules) with the kind bytes of ``_forcing_in_step_kind``; ``field``,
+        ``body_force`` (addresses) replace the solver's, ``record`` receives the intermediates."""
+        p, ibm, mp = self.params, self._ibm, self._ibm_pack("pull")
+        wp, wm = self._rates
+        fx, fy, fz = self._force3
+        bx, by = self.block
+        lk = self._links
+        if lk is not None:
+            rules = (pointer(lk.node_info), pointer(lk.node_first), lk.nodes, pointer(lk.slot), pointer(lk.coeff),
+                     pointer(lk.constant), lk.n, lk.m)
+        else:
+            rules = (None, None, 0, None, None, None, 0, 1)
+        return ("lbm_ibm_pull_step", (
+            self._lattice_code, self._dtype_code, src, dst, pointer(kind), self._wall_ptr,
+            pointer(self._field) if field is None else field, pointer(self._body), p.nx, p.ny, self._nz, wp, wm,
+            fx, fy, fz, self._force_mode, pointer(mp.slot), pointer(mp.coeff), pointer(mp.constant), mp.m,
+            pointer(self._ibm_nodes), ibm.n_nodes, pointer(self._ibm_inverse), pointer(self._ibm_weights),
+            pointer(self._ibm_areas), ibm.n_markers, pointer(self._ibm_start), pointer(self._ibm_entry),
+            ibm.iterations, pointer(self._body_force) if body_force is None else body_force, record, *rules, bx, by,
+            stream,
+        ))
 
     def _pull_links_call(self, src: int, dst: int, stream, force=None, out=None, partial=None,
                          field: int | None = None) -> tuple:
@@ -823,5 +878,5 @@ class HipSolver(Solver):
         return out
 
 
-__all__ = ["HipSolver", "DeviceArrays", "KIND_RULES", "link_rule_index", "default_rules_in_step", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block",
+__all__ = ["HipSolver", "DeviceArrays", "KIND_RULES", "KIND_REGION", "link_rule_index", "default_rules_in_step", "VARIANTS", "TWO_D_ONLY", "BackendUnavailable", "node_kinds", "default_block",
            "default_groups_per_cu"]

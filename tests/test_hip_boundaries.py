@@ -130,6 +130,47 @@ def test_ibm_pull_is_bit_identical_to_the_oracle(d):
     np.testing.assert_array_equal(gpu.immersed_force, ref.immersed_force)
 
 
+def _ibm_open_channel(dtype):
+    """IBM cylinder in a 64 x 32 channel with a parabolic Zou-He inlet and a pressure outlet."""
+    from lbm.boundaries import channel_walls
+    from lbm.geometry import circle
+    from lbm.ibm import ImmersedBoundary
+    from lbm.lattice import D2Q9
+    from lbm.open_boundaries import open_channel
+
+    nx, ny = 64, 32
+    y = np.arange(ny) - 0.5
+    inflow = np.zeros((2, ny))
+    inflow[0] = np.where((y > 0) & (y < ny - 2), 4 * 0.04 * y * (ny - 2 - y) / (ny - 2) ** 2, 0.0)
+    solid, rules = open_channel(D2Q9, channel_walls(nx, ny), inflow)
+    body = ImmersedBoundary.from_ball(circle(19.47, 16.31, 4.0), spacing=1.0, iterations=3)
+    return SimulationParams(nx=nx, ny=ny, tau=0.6, collision="trt", solid=solid, dtype=dtype, immersed=body,
+                            link_rules=rules)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("case", ["walls-lid", "zou-he"])
+def test_forcing_inside_the_pull_step_equals_forcing_then_step(case, dtype):
+    """hip-pull runs the IBM forcing inside its step (lbm_ibm_pull_step) when the workgroup
+    holds a lane per stencil node: against the forcing launch followed by the step (with the
+    link rules inside it), populations, field and body force after 30 steps (graph replays
+    and one eager step) are the same bits."""
+    params = ibm_case(2, dtype) if case == "walls-lid" else _ibm_open_channel(dtype)
+    runs = []
+    for inside in (True, False):
+        s = make_solver(params, backend="hip-pull")
+        if not inside:
+            s._forcing_in_step_kind = lambda record=False: None
+        calls = s._step_calls(s._stream)[0]
+        assert (calls[0][0] == "lbm_ibm_pull_step") == inside and len(calls) == (1 if inside else 2)
+        s.initialize(*random_state(params, 2))
+        s.run(31)
+        runs.append((s.populations(), s._field.cpu().numpy(), s.immersed_force))
+    for a, b in zip(*runs, strict=True):
+        assert np.array_equal(a, b)
+    assert np.any(runs[0][1] != 0)
+
+
 def test_body_at_rest_in_fluid_at_rest_feels_no_force():
     from lbm import SimulationParams
     from lbm.geometry import circle

@@ -45,9 +45,12 @@ __device__ __forceinline__ void collide_adjoint(const T (&f)[L::Q], const T (&la
         Gy = Fy;
         Gz = Fz;
     }
-    const T ux = (jx + T(0.5) * Gx) / rho;
-    const T uy = (jy + T(0.5) * Gy) / rho;
-    const T uz = L::D == 3 ? (jz + T(0.5) * Gz) / rho : T(0);
+    // One reciprocal for the seven quotients by rho (the forward divides; the adjoint's
+    // values agree with it to rounding, which the gradient checks allow for).
+    const T inv_rho = T(1) / rho;
+    const T ux = (jx + T(0.5) * Gx) * inv_rho;
+    const T uy = (jy + T(0.5) * Gy) * inv_rho;
+    const T uz = L::D == 3 ? (jz + T(0.5) * Gz) * inv_rho : T(0);
     const T alpha = r.half_wp + r.half_wm;
     const T beta = r.half_wp - r.half_wm;
     const bool guo = (FM != ForceMode::UNIFORM) || r.has_force;
@@ -90,11 +93,11 @@ __device__ __forceinline__ void collide_adjoint(const T (&f)[L::Q], const T (&la
     uby -= T(3) * uy * gsum;
     ubz -= T(3) * uz * gsum;
     if constexpr (L::D == 2) ubz = T(0);
-    const T jbx = ubx / rho, jby = uby / rho, jbz = ubz / rho;
-    fbx += ubx / (T(2) * rho);
-    fby += uby / (T(2) * rho);
-    fbz += ubz / (T(2) * rho);
-    rho_bar -= (ubx * ux + uby * uy + ubz * uz) / rho;
+    const T jbx = ubx * inv_rho, jby = uby * inv_rho, jbz = ubz * inv_rho;
+    fbx += T(0.5) * jbx;
+    fby += T(0.5) * jby;
+    fbz += T(0.5) * jbz;
+    rho_bar -= (ubx * ux + uby * uy + ubz * uz) * inv_rho;
     drho_bar += rho_bar;
     static_for<0, L::Q>([&](auto I) {
         using Di = Dir<L, I>;

@@ -45,7 +45,9 @@ inline int ibm_threads(int d, int nn)
 // in place when not staged), for the adjoint to read back. The arithmetic is the same in
 // all four variants. `before` runs ahead of the first loads (a caller's own loads go out
 // with them) and `after(last)` once the field is written, `last` being the final force at
-// the stencil nodes (component a of node n at last[a nn + n]).
+// the stencil nodes (component a of node n at last[a nn + n]). With ROWS the caller's
+// `rows(moments)` writes the moment rows' values itself (the rows are then neither read
+// nor gathered here).
 struct IbmNoHook {
     __device__ void operator()() const {}
     template <class U>
@@ -54,7 +56,8 @@ struct IbmNoHook {
     }
 };
 
-template <class T, int D, bool STAGED, bool RECORD, class Before = IbmNoHook, class After = IbmNoHook>
+template <class T, int D, bool STAGED, bool RECORD, class Before = IbmNoHook, class After = IbmNoHook,
+          bool ROWS = false, class Rows = IbmNoHook>
 __device__ __forceinline__ void ibm_force_group(const T* __restrict__ F, const int64_t* __restrict__ slot,
                                                 const T* __restrict__ coeff, const T* __restrict__ constant, int m,
                                                 const int64_t* __restrict__ nodes, int nn,
@@ -63,7 +66,7 @@ __device__ __forceinline__ void ibm_force_group(const T* __restrict__ F, const i
                                                 const int* __restrict__ entry_g, int iterations, T hfx, T hfy, T hfz,
                                                 uint32_t n_total, T* __restrict__ field, T* __restrict__ body_force,
                                                 T* __restrict__ scratch, T* __restrict__ record, int tid, int nthreads,
-                                                Before before = Before{}, After after = After{})
+                                                Before before = Before{}, After after = After{}, Rows rows_hook = Rows{})
 {
     constexpr int S = D == 2 ? 9 : 27;
     extern __shared__ __align__(16) unsigned char ibm_lds[];
@@ -77,9 +80,9 @@ __device__ __forceinline__ void ibm_force_group(const T* __restrict__ F, const i
     // With one moment row per lane (2D), the row's coefficients and slots go out before the
     // table copies, so its population gathers wait one latency after the tables, not two.
     constexpr int AHEAD = 9;
-    const bool ahead = D == 2 && m <= AHEAD && rows <= nthreads;
+    const bool ahead = !ROWS && D == 2 && m <= AHEAD && rows <= nthreads;
     AffineAhead<T, AHEAD> pre;
-    if constexpr (D == 2) {
+    if constexpr (D == 2 && !ROWS) {
         if (ahead && tid < rows) affine_fetch(slot, coeff, constant, rows, m, (int64_t)tid, pre);
     }
     if constexpr (STAGED) {
@@ -102,7 +105,9 @@ __device__ __forceinline__ void ibm_force_group(const T* __restrict__ F, const i
     }
     const T half_force[3] = {hfx, hfy, hfz};
     // P0: affine moment rows, pairwise over the m padded terms.
-    if (ahead) {
+    if constexpr (ROWS) {
+        rows_hook(s.moments);
+    } else if (ahead) {
         if (tid < rows) s.moments[tid] = affine_finish(F, pre, m);
     } else {
         for (int r = tid; r < rows; r += nthreads) s.moments[r] = affine_row(F, slot, coeff, constant, rows, m, r);
@@ -163,7 +168,7 @@ __global__ __launch_bounds__(1024) void k_ibm_step(const T* __restrict__ src, co
                                                    T* __restrict__ body_force, T* __restrict__ record,
                                                    T* __restrict__ dst, const uint8_t* __restrict__ kind,
                                                    const T* __restrict__ wall_u, const T* __restrict__ body, Grid g,
-                                                   Relaxation<T> r, RuleRows<T> rules, int gx, int gy)
+                                                   Relaxation<T> r, RuleRows<T> rules, int gx, int gy, int own_rows)
 {
     const int b = (int)blockIdx.x;
     const int lanes = (int)(blockDim.x * blockDim.y);
@@ -201,6 +206,42 @@ __global__ __launch_bounds__(1024) void k_ibm_step(const T* __restrict__ src, co
             collide<L, FM>(f, r, Fx, Fy, Fz);
             static_for<0, L::Q>([&](auto I) { dst[(size_t)I * g.ns + node] = f[I]; });
         };
+        if (own_rows) {
+            // The moment rows of stencil node n are sums over the populations lane n has
+            // pulled (density: every direction with coefficient 1; momentum a: the directions
+            // with c_ia != 0 in lattice order, coefficient c_ia, the row padded with zero
+            // terms to q), summed as affine_finish sums a row of q terms: the same bits.
+            auto rows_hook = [&](T* moments) {
+                if (!has) return;
+                auto row = [&](const T (&t)[L::Q]) {
+                    T acc = ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+#pragma unroll
+                    for (int k = 8; k < L::Q; ++k) acc = acc + t[k];
+                    return T(0) + acc;
+                };
+                static_assert(L::Q == 9, "rows of nine terms: one chunk of eight, then one term");
+                T t[L::Q];
+                static_for<0, L::Q>([&](auto I) { t[I] = T(1) * f[I]; });
+                moments[tid] = row(t);
+                static_for<0, L::D>([&](auto A) {
+                    constexpr int a = A;
+                    int used = 0;
+                    static_for<0, L::Q>([&](auto I) {
+                        constexpr int i = I;
+                        constexpr int c = a == 0 ? L::cx[i] : (a == 1 ? L::cy[i] : L::cz[i]);
+                        if constexpr (c != 0) t[used++] = T(c) * f[i];
+                    });
+#pragma unroll
+                    for (int k = 0; k < L::Q; ++k)
+                        if (k >= used) t[k] = T(0);
+                    moments[(size_t)(1 + a) * nn + tid] = row(t);
+                });
+            };
+            ibm_force_group<T, L::D, true, RECORD, decltype(before), decltype(after), true>(
+                src, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g, nm, start_g, entry_g,
+                iterations, hfx, hfy, hfz, g.n, field, body_force, nullptr, record, tid, lanes, before, after, rows_hook);
+            return;
+        }
         ibm_force_group<T, L::D, true, RECORD>(src, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g,
                                                nm, start_g, entry_g, iterations, hfx, hfy, hfz, g.n, field, body_force,
                                                nullptr, record, tid, lanes, before, after);
@@ -489,8 +530,8 @@ LBM_API int lbm_ibm_pull_step(int lattice, int dtype, const void* src, void* dst
                               const void* areas, int n_markers, const int* spread_start, const int* spread_entry,
                               int iterations, void* body_force, void* record, const uint32_t* rule_node_info,
                               const int32_t* rule_node_first, int rule_nodes, const int64_t* rule_slot,
-                              const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m, int bx,
-                              int by, void* stream)
+                              const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m,
+                              int rows_in_lanes, int bx, int by, void* stream)
 {
     return guarded("lbm_ibm_pull_step", [&] {
         require(ibm_step_ok(lattice, dtype, n_nodes, n_markers, iterations, record != nullptr, bx, by),
@@ -509,6 +550,7 @@ LBM_API int lbm_ibm_pull_step(int lattice, int dtype, const void* src, void* dst
                                  rule_m >= 1 && rule_m <= 128),
                 "link rules need the node list, slot, coeff, constant and 1 <= m <= 128");
         require(reinterpret_cast<uintptr_t>(rule_node_info) % 16 == 0, "rule_node_info must be 16-byte aligned");
+        require(!rows_in_lanes || m == (lattice == LBM_D2Q9 ? 9 : 19), "rows_in_lanes needs rows of q terms");
         const double force[3] = {fx, fy, fz};
         const dim3 block((unsigned)bx, (unsigned)by, 1);
         dispatch(lattice, dtype, [&](auto Lt, auto Tt) {
@@ -537,7 +579,7 @@ LBM_API int lbm_ibm_pull_step(int lattice, int dtype, const void* src, void* dst
                         static_cast<const T*>(areas), n_markers, spread_start, spread_entry, iterations,
                         T(0.5 * fx), T(0.5 * fy), T(0.5 * fz), static_cast<T*>(field), static_cast<T*>(body_force),
                         static_cast<T*>(record), static_cast<T*>(dst), kind, static_cast<const T*>(wall),
-                        static_cast<const T*>(body), g, r, rr, (int)tiles.x, (int)tiles.y);
+                        static_cast<const T*>(body), g, r, rr, (int)tiles.x, (int)tiles.y, rows_in_lanes);
                 };
                 using Fi = std::integral_constant<ForceMode, ForceMode::FIELD>;
                 using De = std::integral_constant<ForceMode, ForceMode::DENSE>;

@@ -317,9 +317,9 @@ class DragAdjoint:
     def _setup_forcing_in_step(self) -> None:
         """The one-launch forward step (``lbm_ibm_pull_step``: the IBM forcing and the pull step
         with its link rules) when the solver can take it with the forcing's record
-        (``HipSolver._forcing_in_step_kind``)."""
+        (``HipSolver._forcing_in_step``)."""
         s = self.solver
-        self._forcing_kind = s._forcing_in_step_kind(record=True) if s._forcing_in_step_kind() is not None else None
+        self._forcing = s._forcing_in_step(record=True) if s._forcing_in_step() is not None else None
 
     # ------------------------------------------------------------------ controls
 
@@ -376,7 +376,7 @@ class DragAdjoint:
         """The IBM forcing of ``src`` and the pull step to ``dst`` in one launch: the same
         field, drag, record and next state as ``_ibm_forward`` followed by ``_step_forward``."""
         s = self.solver
-        name, args = s._ibm_step_call(pointer(src), pointer(dst), self._forcing_kind, s._stream, field=pointer(field),
+        name, args = s._ibm_step_call(pointer(src), pointer(dst), self._forcing, s._stream, field=pointer(field),
                                       body_force=body_out, record=pointer(record))
         s._call(name, *args)
 
@@ -408,7 +408,7 @@ class DragAdjoint:
             cur, nxt = (a, b) if j % 2 == 0 else (b, a)
             field = s._field if fields is None else fields[j]
             record = None if records is None else records[j]
-            if self._forcing_kind is not None and (s._links is not None or "pull" not in s._rule_passes):
+            if self._forcing is not None and (s._links is not None or "pull" not in s._rule_passes):
                 self._forced_step(cur, nxt, field, drag[j].data_ptr(), record)
                 continue
             self._ibm_forward(cur, field, drag[j].data_ptr(), record)

@@ -21,6 +21,7 @@ holds f*_i (``state_layout = "aa_odd"``).
 from __future__ import annotations
 
 import math
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -28,6 +29,7 @@ from lbm.backends import BackendUnavailable, _require_device, graph_capture, loa
 from lbm.backends._native import DTYPE_CODES, FORCE_DENSE, FORCE_FIELD, FORCE_UNIFORM, pointer
 from lbm.diagnostics import MacroscopicFields
 from lbm.geometry import neighbour_index
+from lbm.ibm import rows_are_pulled_moments
 from lbm.lattice import VelocitySet
 from lbm.link_rules import CompiledAffine, compile_rules, reads_other_targets
 from lbm.solver import SimulationDiverged, Solver
@@ -202,6 +204,15 @@ KIND_RULES = 8
 KIND_REGION = 16
 
 
+@dataclass(frozen=True)
+class ForcingStep:
+    """The one-launch forward step of an IBM body (``lbm_ibm_pull_step``): its kind bytes (the
+    stencil nodes carry KIND_REGION) and whether the stencil nodes' lanes sum the moment rows."""
+
+    kind: object
+    rows_in_lanes: bool
+
+
 # The pull step's link rules go inside the step (one launch per step) up to this many nodes;
 # above, the saved pre-pass launch is worth less than the rule layer's scattered accesses
 # to the end planes of open ducts. Bouzidi body with Zou-He ends, fp32, MLUPS pre-pass ->
@@ -287,7 +298,7 @@ class HipSolver(Solver):
         self._block = block
         self.graphs = graphs
         self._graphs = {}
-        self._forcing_kinds = {}
+        self._forcing_steps = {}
         p = params
         self._np_dtype = np.dtype(p.dtype)
         self._torch_dtype = {"float32": torch.float32, "float64": torch.float64}[p.dtype]
@@ -522,7 +533,7 @@ class HipSolver(Solver):
                     + self._pre_pass_calls("aa_even", f, stream)
                     + [("lbm_aa_even", (L, D, f, kind, field, body, *args))]]
         a, b = pointer(self._f), pointer(self._f_next)
-        forcing = self._forcing_in_step_kind()
+        forcing = self._forcing_in_step()
         if forcing is not None:
             return [[self._ibm_step_call(src, dst, forcing, stream)] for src, dst in ((a, b), (b, a))]
         if self._links is not None and self.block[0] * self.block[1] >= 64:
@@ -532,34 +543,36 @@ class HipSolver(Solver):
         return [self._pre_pass_calls("pull", src, stream) + [(name, (L, D, src, dst, kind, wall, field, body, *args))]
                 for src, dst in ((a, b), (b, a))]
 
-    def _forcing_in_step_kind(self, record: bool = False):
-        """The kind bytes of ``lbm_ibm_pull_step`` (the stencil nodes carry KIND_REGION) when the
-        pull variant can run its IBM forcing inside the step at the current block, the link
-        rules (if any) in the step and no stencil node solid, next to a solid node or a rule
-        node; else None. ``record``: the forcing also writes its intermediates."""
+    def _forcing_in_step(self, record: bool = False) -> ForcingStep | None:
+        """What ``lbm_ibm_pull_step`` needs when the pull variant can run its IBM forcing inside
+        the step at the current block (the link rules, if any, in the step; no stencil node
+        solid, next to a solid node or a rule node), else None. ``record``: the forcing also
+        writes its intermediates."""
         if self.variant != "pull" or self._ibm is None or ("pull" in self._rule_passes and self._links is None):
             return None
         key = (self.block, record, self._kind.data_ptr())
-        if key not in self._forcing_kinds:
+        if key not in self._forcing_steps:
             import ctypes
 
             ibm, (bx, by) = self._ibm, self.block
             ok = ctypes.c_int()
             self._lib.call("lbm_ibm_pull_step_supported", self._lattice_code, self._dtype_code, ibm.n_nodes,
                            ibm.n_markers, ibm.iterations, int(record), bx, by, ctypes.byref(ok))
-            kind = None
+            step = None
             blocked = KIND_SOLID | KIND_NEAR_SOLID | KIND_RULES
             if ok.value and not (self._kind.cpu().numpy()[ibm.nodes] & blocked).any():
                 kind = self._kind.clone()
                 kind[self._ibm_nodes] |= KIND_REGION
-            self._forcing_kinds[key] = kind
-        return self._forcing_kinds[key]
+                step = ForcingStep(kind, rows_are_pulled_moments(self.params.lattice, ibm.nodes, self.params.shape,
+                                                                 ibm.compiled("pull")))
+            self._forcing_steps[key] = step
+        return self._forcing_steps[key]
 
-    def _ibm_step_call(self, src: int, dst: int, kind, stream, field: int | None = None,
+    def _ibm_step_call(self, src: int, dst: int, forcing: ForcingStep, stream, field: int | None = None,
                        body_force: int | None = None, record: int | None = None) -> tuple:
         """``lbm_ibm_pull_step`` from ``src`` to ``dst`` (the IBM forcing of ``src``, then the pull
-        step with its link rules) with the kind bytes of ``_forcing_in_step_kind``; ``field``,
-        ``body_force`` (addresses) replace the solver's, ``record`` receives the intermediates."""
+        step with its link rules) as ``_forcing_in_step`` set it up; ``field``, ``body_force``
+        (addresses) replace the solver's, ``record`` receives the intermediates."""
         p, ibm, mp = self.params, self._ibm, self._ibm_pack("pull")
         wp, wm = self._rates
         fx, fy, fz = self._force3
@@ -571,13 +584,13 @@ class HipSolver(Solver):
         else:
             rules = (None, None, 0, None, None, None, 0, 1)
         return ("lbm_ibm_pull_step", (
-            self._lattice_code, self._dtype_code, src, dst, pointer(kind), self._wall_ptr,
+            self._lattice_code, self._dtype_code, src, dst, pointer(forcing.kind), self._wall_ptr,
             pointer(self._field) if field is None else field, pointer(self._body), p.nx, p.ny, self._nz, wp, wm,
             fx, fy, fz, self._force_mode, pointer(mp.slot), pointer(mp.coeff), pointer(mp.constant), mp.m,
             pointer(self._ibm_nodes), ibm.n_nodes, pointer(self._ibm_inverse), pointer(self._ibm_weights),
             pointer(self._ibm_areas), ibm.n_markers, pointer(self._ibm_start), pointer(self._ibm_entry),
-            ibm.iterations, pointer(self._body_force) if body_force is None else body_force, record, *rules, bx, by,
-            stream,
+            ibm.iterations, pointer(self._body_force) if body_force is None else body_force, record, *rules,
+            int(forcing.rows_in_lanes), bx, by, stream,
         ))
 
     def _pull_links_call(self, src: int, dst: int, stream, force=None, out=None, partial=None,

@@ -292,7 +292,12 @@ LBM_API int lbm_ibm_pull_step_supported(int lattice, int dtype, int n_nodes, int
  * update runs in the forcing's workgroup after the forcing, every other node in the other
  * workgroups; `kind` must carry bit 16 on exactly the stencil nodes (which the other
  * workgroups then skip), and no stencil node may be solid, next to a solid node or a rule
- * node. Replaces one iteration of the reference's pull loop: its pre-pass (link rules, then
+ * node. rows_in_lanes = 1 declares that the moment rows are those of lbm.ibm.moment_rows in
+ * the pull layout (row a n_nodes + k of stencil node k: density, every direction in lattice
+ * order with coefficient 1; momentum a, the directions with c_i,a != 0 in lattice order with
+ * coefficient c_i,a, zero-padded to q terms; constant 0): each stencil node's lane then sums
+ * them from the populations it pulls instead of gathering them (the same bits).
+ * Replaces one iteration of the reference's pull loop: its pre-pass (link rules, then
  * the IBM force field and its scatter; src/lbm/backends/cuda_solver.py:286-292,
  * src/lbm/ibm.py:189-214) and its kernel launch (src/lbm/backends/cuda_solver.py:279-284). */
 LBM_API int lbm_ibm_pull_step(int lattice, int dtype, const void* src, void* dst, const uint8_t* kind,
@@ -303,8 +308,8 @@ LBM_API int lbm_ibm_pull_step(int lattice, int dtype, const void* src, void* dst
                               const void* areas, int n_markers, const int* spread_start, const int* spread_entry,
                               int iterations, void* body_force, void* record, const uint32_t* rule_node_info,
                               const int32_t* rule_node_first, int rule_nodes, const int64_t* rule_slot,
-                              const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m, int bx,
-                              int by, void* stream);
+                              const void* rule_coeff, const void* rule_constant, int64_t n_rules, int rule_m,
+                              int rows_in_lanes, int bx, int by, void* stream);
 
 /* Bytes of one step's record of the forcing intermediates (optional `record` argument of
  * lbm_ibm_force, read back by lbm_ibm_adjoint), a multiple of 16; records at 16-byte aligned

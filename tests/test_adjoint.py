@@ -449,7 +449,7 @@ def test_forward_with_rules_in_the_step_gives_the_same_drag_and_gradients(dtype)
         problem._fused = None  # the two-launch backward step in both runs
         if not in_step:
             problem.solver._links = None
-            problem._forcing_kind = None
+            problem._forcing = None
         problem.set_body_force(g)
         loss = problem.forward()
         runs.append((loss, problem.drag_history, *problem.backward()))
@@ -502,9 +502,9 @@ def test_forcing_inside_the_step_gives_the_same_bits_as_two_launches(rules, dtyp
     runs = []
     for inside in (True, False):
         problem = DragAdjoint(params, steps=STEPS, window=WINDOW, checkpoint_every=K, init=init_fields(params))
-        assert problem._forcing_kind is not None
+        assert problem._forcing is not None
         if not inside:
-            problem._forcing_kind = None
+            problem._forcing = None
         problem.set_body_force(g)
         loss = problem.forward()
         runs.append((loss, problem.drag_history, problem._field_hist.cpu().numpy(), problem._records.cpu().numpy(),

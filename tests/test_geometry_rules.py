@@ -218,6 +218,39 @@ def test_ibm_stencils_must_stay_inside_and_clear_of_walls():
         SimulationParams(nx=40, ny=30, tau=0.8, immersed=ImmersedBoundary.from_ball(sphere(9, 9, 9, 3)))
 
 
+def test_pull_moment_rows_are_sums_of_the_pulled_populations():
+    """The one-launch forward step sums each stencil node's moment rows from the populations
+    its lane pulls. The compiled pull rows are checked here by coordinates with periodic wrap:
+    row a n + k has, in lattice order, the directions i with weight w_a,i != 0 (1 for the
+    density, c_i,a for momentum a) on slot (i, x_k - c_i), then zero coefficients, constant
+    0. A changed coefficient, slot, padding term or constant is refused."""
+    from dataclasses import replace
+
+    from lbm.ibm import rows_are_pulled_moments
+
+    params = SimulationParams(nx=40, ny=30, tau=0.8, solid=channel_walls(40, 30),
+                              immersed=ImmersedBoundary.from_ball(circle(17.3, 14.6, 4.2)))
+    ibm = ImmersedForcing(params)
+    rows = ibm.compiled("pull")
+    nn, (ny, nx) = ibm.n_nodes, params.shape
+    for k, node in enumerate(ibm.nodes):
+        y, x = divmod(int(node), nx)
+        for a, weight in enumerate(([1] * 9, [c[0] for c in D2Q9.c], [c[1] for c in D2Q9.c])):
+            used = [i for i in range(9) if weight[i] != 0]
+            r = a * nn + k
+            for t, i in enumerate(used):
+                cx, cy = D2Q9.c[i]
+                assert rows.slots[r, t] == i * nx * ny + ((y - cy) % ny) * nx + (x - cx) % nx
+                assert rows.coeff[r, t] == weight[i]
+            assert np.all(rows.coeff[r, len(used):] == 0) and rows.constant[r] == 0
+    assert rows_are_pulled_moments(D2Q9, ibm.nodes, params.shape, rows)
+    for field, index, value in (("coeff", (nn, 0), 2.0), ("slots", (3, 4), 0), ("coeff", (2 * nn + 1, 8), 1.0),
+                                ("constant", (5,), 1e-3)):
+        planted = np.array(getattr(rows, field), copy=True)
+        planted[index] = value
+        assert not rows_are_pulled_moments(D2Q9, ibm.nodes, params.shape, replace(rows, **{field: planted}))
+
+
 # ---------------------------------------------------------------- lattices
 
 

@@ -152,22 +152,31 @@ def _ibm_open_channel(dtype):
 @pytest.mark.parametrize("case", ["walls-lid", "zou-he"])
 def test_forcing_inside_the_pull_step_equals_forcing_then_step(case, dtype):
     """hip-pull runs the IBM forcing inside its step (lbm_ibm_pull_step) when the workgroup
-    holds a lane per stencil node: against the forcing launch followed by the step (with the
-    link rules inside it), populations, field and body force after 30 steps (graph replays
-    and one eager step) are the same bits."""
+    holds a lane per stencil node, each stencil node's lane summing its moment rows from the
+    populations it pulls: against the rows gathered from memory and against the forcing
+    launch followed by the step (with the link rules inside it), populations, field and
+    body force after 31 steps (graph replays and one eager step) are the same bits."""
+    from dataclasses import replace
+
     params = ibm_case(2, dtype) if case == "walls-lid" else _ibm_open_channel(dtype)
     runs = []
-    for inside in (True, False):
+    for mode in ("rows in lanes", "rows gathered", "two launches"):
         s = make_solver(params, backend="hip-pull")
-        if not inside:
-            s._forcing_in_step_kind = lambda record=False: None
+        forcing = s._forcing_in_step()
+        assert forcing is not None and forcing.rows_in_lanes
+        if mode == "rows gathered":
+            s._forcing_in_step = lambda record=False, f=replace(forcing, rows_in_lanes=False): f
+        if mode == "two launches":
+            s._forcing_in_step = lambda record=False: None
+        inside = mode != "two launches"
         calls = s._step_calls(s._stream)[0]
         assert (calls[0][0] == "lbm_ibm_pull_step") == inside and len(calls) == (1 if inside else 2)
         s.initialize(*random_state(params, 2))
         s.run(31)
         runs.append((s.populations(), s._field.cpu().numpy(), s.immersed_force))
-    for a, b in zip(*runs, strict=True):
-        assert np.array_equal(a, b)
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other, strict=True):
+            assert np.array_equal(a, b)
     assert np.any(runs[0][1] != 0)
 
 

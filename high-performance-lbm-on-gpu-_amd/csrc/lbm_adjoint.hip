@@ -58,51 +58,73 @@ __device__ __forceinline__ void collide_adjoint(const T (&f)[L::Q], const T (&la
     T drho_bar = T(0), rho_bar = T(0), gsum = T(0);
     T ubx = T(0), uby = T(0), ubz = T(0);
     T fbx = T(0), fby = T(0), fbz = T(0);
-    const T usq = T(1.5) * (ux * ux + uy * uy + uz * uz);
+    const T usq = T(1.5) * fma(ux, ux, fma(uy, uy, uz * uz));
+    // c_i . v from the compile-time components (no products by 0 or +-1).
+    auto dot = [](auto I, T vx, T vy, T vz) {
+        using Di = Dir<L, decltype(I)::value>;
+        T d = T(0);
+        bool first = true;
+        auto add = [&](int c, T v) {
+            if (c == 0) return;
+            const T t = c > 0 ? v : -v;
+            d = first ? t : d + t;
+            first = false;
+        };
+        add(Di::x, vx);
+        add(Di::y, vy);
+        if constexpr (L::D == 3) add(Di::z, vz);
+        return d;
+    };
+    // a += c_i,axis * v for the three axes (skipped where the component is 0).
+    auto push = [](auto I, T& ax, T& ay, T& az, T v) {
+        using Di = Dir<L, decltype(I)::value>;
+        if constexpr (Di::x != 0) ax = Di::x > 0 ? ax + v : ax - v;
+        if constexpr (Di::y != 0) ay = Di::y > 0 ? ay + v : ay - v;
+        if constexpr (L::D == 3 && Di::z != 0) az = Di::z > 0 ? az + v : az - v;
+    };
+    // The expressions below are those of the comment above with products and sums fused
+    // (the adjoint is checked against float64 finite differences and autograd, not bit
+    // for bit against a host code).
+    const T G3x = T(3) * Gx, G3y = T(3) * Gy, G3z = T(3) * Gz;
+    const T am3 = T(3) * r.a_minus;
     static_for<0, L::Q>([&](auto I) {
         constexpr int i = I;
         using Di = Dir<L, i>;
-        const T nbar = -alpha * lam[i] - beta * lam[L::opp[i]];
+        const T nbar = fma(-alpha, lam[i], -(beta * lam[L::opp[i]]));
         fbar[i] = lam[i] + nbar;
-        const T feqbar = -nbar;
-        const T w = T(Di::w);
-        const T cu = T(Di::x) * ux + T(Di::y) * uy + T(Di::z) * uz;
-        const T g = T(3) * cu + T(4.5) * cu * cu - usq;
-        drho_bar += feqbar * w;
-        rho_bar += feqbar * w * g;
-        const T gbar = feqbar * w * rho;
+        const T fw = -nbar * T(Di::w);  // feqbar w
+        const T cu = dot(I, ux, uy, uz);
+        const T g = fma(T(4.5) * cu, cu, fma(T(3), cu, -usq));
+        drho_bar += fw;
+        rho_bar = fma(fw, g, rho_bar);
+        const T gbar = fw * rho;
         gsum += gbar;
-        const T s = gbar * (T(3) + T(9) * cu);
-        ubx += s * T(Di::x);
-        uby += s * T(Di::y);
-        ubz += s * T(Di::z);
+        push(I, ubx, uby, ubz, gbar * fma(T(9), cu, T(3)));
         if (guo) {
-            const T cF = T(Di::x) * Gx + T(Di::y) * Gy + T(Di::z) * Gz;
-            const T lw = lam[i] * w;
+            const T cF = dot(I, Gx, Gy, Gz);
+            const T lw = lam[i] * T(Di::w);
             const T lp = lw * r.a_plus;
-            ubx += lp * (T(9) * cF * T(Di::x) - T(3) * Gx);
-            uby += lp * (T(9) * cF * T(Di::y) - T(3) * Gy);
-            ubz += lp * (T(9) * cF * T(Di::z) - T(3) * Gz);
-            const T am3 = T(3) * r.a_minus;
-            fbx += lp * (T(9) * cu * T(Di::x) - T(3) * ux) + lw * am3 * T(Di::x);
-            fby += lp * (T(9) * cu * T(Di::y) - T(3) * uy) + lw * am3 * T(Di::y);
-            fbz += lp * (T(9) * cu * T(Di::z) - T(3) * uz) + lw * am3 * T(Di::z);
+            ubx = fma(-lp, G3x, ubx);
+            uby = fma(-lp, G3y, uby);
+            if constexpr (L::D == 3) ubz = fma(-lp, G3z, ubz);
+            push(I, ubx, uby, ubz, T(9) * cF * lp);
+            fbx = fma(-lp, T(3) * ux, fbx);
+            fby = fma(-lp, T(3) * uy, fby);
+            if constexpr (L::D == 3) fbz = fma(-lp, T(3) * uz, fbz);
+            push(I, fbx, fby, fbz, fma(T(9) * cu, lp, lw * am3));
         }
     });
-    ubx -= T(3) * ux * gsum;
-    uby -= T(3) * uy * gsum;
-    ubz -= T(3) * uz * gsum;
-    if constexpr (L::D == 2) ubz = T(0);
+    const T g3 = T(3) * gsum;
+    ubx = fma(-g3, ux, ubx);
+    uby = fma(-g3, uy, uby);
+    ubz = L::D == 3 ? fma(-g3, uz, ubz) : T(0);
     const T jbx = ubx * inv_rho, jby = uby * inv_rho, jbz = ubz * inv_rho;
-    fbx += T(0.5) * jbx;
-    fby += T(0.5) * jby;
-    fbz += T(0.5) * jbz;
-    rho_bar -= (ubx * ux + uby * uy + ubz * uz) * inv_rho;
+    fbx = fma(T(0.5), jbx, fbx);
+    fby = fma(T(0.5), jby, fby);
+    fbz = fma(T(0.5), jbz, fbz);
+    rho_bar = fma(-fma(ubx, ux, fma(uby, uy, ubz * uz)), inv_rho, rho_bar);
     drho_bar += rho_bar;
-    static_for<0, L::Q>([&](auto I) {
-        using Di = Dir<L, I>;
-        fbar[I] += drho_bar + T(Di::x) * jbx + T(Di::y) * jby + T(Di::z) * jbz;
-    });
+    static_for<0, L::Q>([&](auto I) { fbar[I] += drho_bar + dot(I, jbx, jby, jbz); });
     Fbx = fbx;
     Fby = fby;
     Fbz = L::D == 3 ? fbz : T(0);

@@ -337,16 +337,18 @@ struct AdjNodeParts {
 
     // finish() for an IBM node that keeps its pushes: mu_i stays in lam[i] (the node adds the
     // transposed moment rows itself in push_kept) and dL/dF (the node being forced) goes to
-    // nu_seed instead of nu; grad_body is updated as in finish().
-    __device__ __forceinline__ void finish_kept(T (&nu_seed)[L::D], double* __restrict__ grad_body, const Grid& g,
-                                                const Relaxation<T>& r)
+    // nu_seed instead of nu; the body-force gradients are summed as in finish() and stored
+    // by push_kept (a store here would hold the next barrier until it completes).
+    __device__ __forceinline__ void finish_kept(T (&nu_seed)[L::D], const Relaxation<T>& r)
     {
         static_assert(!MASKED, "an IBM node has no solid upstream neighbour");
         T mu[L::Q];
         T Fbx, Fby, Fbz;
         adjoint(nullptr, r, mu, Fbx, Fby, Fbz);
         static_for<0, L::Q>([&](auto I) { lam[I] = mu[I]; });
-        gradients(grad_body, g, Fbx, Fby, Fbz);
+        gb[0] = gb[0] + (double)Fbx;
+        gb[1] = gb[1] + (double)Fby;
+        if constexpr (L::D == 3) gb[2] = gb[2] + (double)Fbz;
         const bool forced = (k & KIND_FORCED) != 0;
         nu_seed[0] = forced ? Fbx : T(0);
         nu_seed[1] = forced ? Fby : T(0);
@@ -357,9 +359,13 @@ struct AdjNodeParts {
     // a = 0 density with coefficient 1, a = 1 .. d momentum with c_i,a-1, the nonzero ones in
     // increasing a, summed from 0 as the transposed row lists are): push i writes slot
     // (i, x - c_i), which only this node's moment rows read.
-    __device__ __forceinline__ void push_kept(T* __restrict__ lam_out, const T* __restrict__ m_bar, int nn, int row,
-                                              const Grid& g) const
+    __device__ __forceinline__ void push_kept(T* __restrict__ lam_out, double* __restrict__ grad_body,
+                                              const T* __restrict__ m_bar, int nn, int row, const Grid& g) const
     {
+        if (grad_body) {
+#pragma unroll
+            for (int a = 0; a < L::D; ++a) grad_body[(size_t)a * g.n + node] = gb[a];
+        }
         static_for<0, L::Q>([&](auto I) {
             constexpr int i = I;
             constexpr int c[3] = {L::cx[i], L::cy[i], L::cz[i]};
@@ -1271,10 +1277,10 @@ __global__ __launch_bounds__(1024) void k_adj_fused(LBM_IBM_ADJ_PARAMS, const T*
             auto after_kept = [&](T (&nu_seed)[L::D]) {
 #pragma unroll
                 for (int a = 0; a < L::D; ++a) nu_seed[a] = T(0);
-                if (has) own.finish_kept(nu_seed, grad_body, g, r);
+                if (has) own.finish_kept(nu_seed, r);
             };
             auto scatter_kept = [&](const T* m_bar) {
-                if (has) own.push_kept(lam, m_bar, nn, tid, g);
+                if (has) own.push_kept(lam, grad_body, m_bar, nn, tid, g);
             };
             ibm_adj_group_prefetched<T, L::D, false, true>(
                 F, slot, coeff, constant, m, nodes, nn, inverse_g, weights_g, areas_g, nm, start_g, entry_g, iters, hfx,

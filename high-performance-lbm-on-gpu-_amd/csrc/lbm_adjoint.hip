@@ -1139,6 +1139,53 @@ __global__ __launch_bounds__(ibm_adj_max_threads(L::D)) void k_ibm_adj_pull(
 
 // ------------------------------------------------------------------ fused backward step
 
+// What the IBM workgroup of the next backward step reads first: the state it pulls from,
+// its record and its field (null when there is no next step in the same sweep), and the
+// tile workgroup (`block`, -1 for none) that reads them ahead.
+template <class T>
+struct AdjNext {
+    const T* f;
+    const T* record;
+    const T* field;
+    int block;
+};
+
+// Reads, once its own nodes are done, what the next step's IBM workgroup will read first
+// (the region list; kind, body force, field and pulled populations of the region nodes; the
+// record; the stencil tables), so that workgroup finds them in the L2 cache of its XCD
+// (the host picks a block on the same XCD) instead of HBM. The values are summed and held,
+// so the loads are issued; nothing is written.
+template <class L, class T>
+__device__ __forceinline__ void adj_prefetch_next(const AdjNext<T>& next, const int32_t* __restrict__ region,
+                                                  int n_region, const uint8_t* __restrict__ kind,
+                                                  const T* __restrict__ body, const int* __restrict__ inverse,
+                                                  const T* __restrict__ weights, const T* __restrict__ areas,
+                                                  const int* __restrict__ start, const int* __restrict__ entry,
+                                                  int nn, int nm, int iters, const Grid& g, int tid, int lanes)
+{
+    constexpr int S = L::D == 2 ? 9 : 27;
+    uint32_t sum = 0;
+    for (int q = tid; q < n_region; q += lanes) {
+        const uint32_t node = (uint32_t)region[q];
+        const uint32_t rest = node / (uint32_t)g.nx;
+        const int x = (int)(node - rest * (uint32_t)g.nx), y = (int)(rest % (uint32_t)g.ny),
+                  z = (int)(rest / (uint32_t)g.ny);
+        T v = T(0);
+        static_for<0, L::Q>([&](auto I) { v = v + next.f[(size_t)I * g.n + nbr<L, I, -1>(x, y, z, g)]; });
+#pragma unroll
+        for (int a = 0; a < L::D; ++a) v = v + body[(size_t)a * g.n + node] + next.field[(size_t)a * g.n + node];
+        sum += (uint32_t)kind[node] + (v != v ? 1u : 0u);
+    }
+    const size_t rec = ibm_record_elems(L::D, nn, nm, iters);
+    T v = T(0);
+    for (size_t w = tid; w < rec; w += lanes) v = v + next.record[w];
+    for (int w = tid; w < nm * S; w += lanes) v = v + weights[w] + T(inverse[w] + entry[w]);
+    for (int w = tid; w < nm; w += lanes) v = v + areas[w];
+    for (int w = tid; w <= nn; w += lanes) sum += (uint32_t)start[w];
+    sum += v != v ? 1u : 0u;
+    hold_value(sum);
+}
+
 // The link-rule nodes of the fused backward step, for rule sets in which every slot a rule
 // reads or writes is written by the adjoint step of one node, the rule's node: info[k] =
 // {node, directions i whose upstream node x - c_i is solid | the node's rule target
@@ -1244,7 +1291,7 @@ __global__ __launch_bounds__(1024) void k_adj_fused(LBM_IBM_ADJ_PARAMS, const T*
                                                     Grid g, Relaxation<T> r, int atiles_x, int atiles_y,
                                                     const T* __restrict__ p_src, T* __restrict__ p_dst,
                                                     const T* __restrict__ p_field, RuleRows<T> p_rules, int ptiles_x,
-                                                    int ptiles_y, int ptiles)
+                                                    int ptiles_y, int ptiles, AdjNext<T> next)
 {
     const int b = (int)blockIdx.x;
     const int lanes = (int)(blockDim.x * blockDim.y);
@@ -1328,6 +1375,9 @@ __global__ __launch_bounds__(1024) void k_adj_fused(LBM_IBM_ADJ_PARAMS, const T*
     if (x < g.nx && y < g.ny)
         adj_pull_node<L, T, FM, true, true>(lam_next, lam, F, kind, wall_u, field, body, grad_body, nu_out, g, r, x,
                                             y, z);
+    if (b == next.block)
+        adj_prefetch_next<L, T>(next, region, n_region, kind, body, inverse_g, weights_g, areas_g, start_g, entry_g,
+                                nn, nm, iters, g, tid, lanes);
 }
 
 #undef LBM_IBM_ADJ_PARAMS
@@ -1646,7 +1696,7 @@ LBM_API int lbm_adj_fused_step(
     const uint32_t* arule_info, const void* arule_coeff, int arule_nodes, int bx, int by, const void* pull_src,
     void* pull_dst, const void* pull_field, const uint32_t* rule_node_info, const int32_t* rule_node_first,
     int rule_nodes, const int64_t* prule_slot, const void* prule_coeff, const void* prule_constant, int64_t n_prules,
-    int prule_m, void* stream)
+    int prule_m, const void* next_f, const void* next_record, const void* next_field, void* stream)
 {
     return guarded("lbm_adj_fused_step", [&] {
         require(adj_fused_ok(lattice, dtype, n_nodes, n_markers, iterations, n_tslots, bx, by),
@@ -1712,6 +1762,15 @@ LBM_API int lbm_adj_fused_step(
                 const size_t lds = std::max({ibm_lds, chunks ? sizeof(T) * (size_t)chunk * (L::Q - 1) : (size_t)0,
                                              achunks ? sizeof(T) * (size_t)achunk * L::Q * L::Q : (size_t)0});
                 require(lds <= lds_limit(), "the workgroups' tables do not fit the LDS");
+                // The next step's reads go through the last adjoint tile on the XCD of workgroup
+                // 0 (workgroups are dealt to the 8 XCDs in turn), when all are given.
+                AdjNext<T> next{static_cast<const T*>(next_f), static_cast<const T*>(next_record),
+                                static_cast<const T*>(next_field), -1};
+                if (next_f && next_record && next_field) {
+                    int64_t last = blocks - 1;
+                    last -= last % 8;
+                    if (last >= blocks - n_tiles) next.block = (int)last;
+                }
                 allow_lds(k_adj_fused<L, T, FM>, lds);
                 k_adj_fused<L, T, FM><<<(unsigned)blocks, block, lds, as_stream(stream)>>>(
                     static_cast<const T*>(f), slot, static_cast<const T*>(coeff), static_cast<const T*>(constant), m,
@@ -1723,7 +1782,7 @@ LBM_API int lbm_adj_fused_step(
                     kind, static_cast<const T*>(wall_u), static_cast<const T*>(force_field),
                     static_cast<const T*>(body_field), grad_body, region, n_region, keep_pushes, ar, g, r, tx, ty,
                     static_cast<const T*>(pull_src), static_cast<T*>(pull_dst), static_cast<const T*>(pull_field), rr,
-                    tx, ty, (int)ptiles);
+                    tx, ty, (int)ptiles, next);
             } else {
                 require(false, "the fused adjoint step takes float32 D2Q9");
             }

@@ -462,8 +462,12 @@ class DragAdjoint:
                 if not self.store_fields:
                     self._ibm_forward(buf_p[jp], fields_p[jp], self._bf_scratch.data_ptr())
             if self._fused is not None:
+                following = None
+                if j > 0 and records is not None:
+                    following = (buf[j - 1], records[j - 1], fields[j - 1])
                 self._fused_step(buf[j], fields[j], None if records is None else records[j], weights[j], lam_next,
-                                 lam_out, (partner[0][jp], partner[0][jp + 1], partner[1][jp]) if with_partner else None)
+                                 lam_out, (partner[0][jp], partner[0][jp + 1], partner[1][jp]) if with_partner else None,
+                                 following)
                 continue
             adj = (L, D, pointer(lam_next), pointer(lam_out), pointer(buf[j]), pointer(s._kind), s._wall_ptr,
                    pointer(fields[j]), pointer(s._body), nx, ny, nz, wp, wm, fx, fy, fz, FORCE_DENSE, s._mask_mode,
@@ -494,9 +498,10 @@ class DragAdjoint:
         if partner is not None and partner[2] > length:
             self._recompute(partner[0], partner[1], length, partner[2])
 
-    def _fused_step(self, f, field, record, weight, lam_next, lam_out, partner) -> None:
+    def _fused_step(self, f, field, record, weight, lam_next, lam_out, partner, following=None) -> None:
         """One backward step in one launch; ``partner`` = (source, destination, field) of the
-        forward recompute step riding along, or None."""
+        forward recompute step riding along, or None; ``following`` = (state, record, field) of
+        the backward step after this one (read ahead into the cache), or None."""
         s = self.solver
         ibm, mp, r, fz = s._ibm, self._mp, self._rules, self._fused
         fx, fy, fz_ = s._force3
@@ -520,7 +525,8 @@ class DragAdjoint:
                 None if record is None else pointer(record), pointer(lam_next), pointer(fz["kind"]), s._wall_ptr,
                 pointer(field), pointer(s._body), s.params.nx, s.params.ny, s._nz, wp, wm, pointer(self._grad_body),
                 pointer(fz["region"]), len(fz["region"]), fz["keep"], pointer(fz["rule_info"]),
-                pointer(fz["rule_coeff"]), fz["rule_nodes"], bx, by, src, dst, pfield, *links, s._stream)
+                pointer(fz["rule_coeff"]), fz["rule_nodes"], bx, by, src, dst, pfield, *links,
+                *((None, None, None) if following is None else tuple(pointer(t) for t in following)), s._stream)
 
     @property
     def adjoint_block(self) -> tuple[int, int]:

@@ -61,7 +61,7 @@ SIGNATURES: dict[str, list] = {
     "lbm_adj_fused_supported": [_i, _i, _i, _i, _i, _i, _i, _i, POINTER(c_int)],
     "lbm_adj_fused_step": [_i, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _p, _i, _p, _p, _i, _d, _d, _d, _i64, _p, _p,
                            _p, _p, _p, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _p, _p, _i, _i, _p,
-                           _p, _i, _i, _i, _p, _p, _p, _p, _p, _i, _p, _p, _p, _i64, _i, _p],
+                           _p, _i, _i, _i, _p, _p, _p, _p, _p, _i, _p, _p, _p, _i64, _i, _p, _p, _p, _p],
     "lbm_ibm_scratch_size": [_i, _i, _i, _i, POINTER(c_int64)],
     "lbm_ibm_pull_step_supported": [_i, _i, _i, _i, _i, _i, _i, _i, POINTER(c_int)],
     "lbm_ibm_pull_step": [_i, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _d, _d, _d, _d, _d, _i, _p, _p, _p, _i, _p, _i,

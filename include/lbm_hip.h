@@ -433,7 +433,10 @@ LBM_API int lbm_adj_fused_supported(int lattice, int dtype, int n_nodes, int n_m
  * << 16, the rest zero; arule_coeff[k][i][j] (q x q) = the coefficient of the rule with
  * target push j in the transposed row of the slot push i writes, zero where none, with each
  * transposed row's entries in increasing j. The IBM region and the rule nodes must not meet
- * and their kind bytes carry bit 16 (the other workgroups skip them). Same bits as
+ * and their kind bytes carry bit 16 (the other workgroups skip them). next_f, next_record
+ * and next_field (may be null) are the state, record and field of the backward step that
+ * follows (the preceding time step): one tile workgroup that finishes early reads what that
+ * step's IBM workgroup will read first, so it comes from the L2 cache. Same bits as
  * lbm_adj_pull_step_rules followed by lbm_ibm_adjoint_pull. One workgroup shape bx x by for
  * every part. */
 LBM_API int lbm_adj_fused_step(
@@ -447,7 +450,7 @@ LBM_API int lbm_adj_fused_step(
     const uint32_t* arule_info, const void* arule_coeff, int arule_nodes, int bx, int by, const void* pull_src,
     void* pull_dst, const void* pull_field, const uint32_t* rule_node_info, const int32_t* rule_node_first,
     int rule_nodes, const int64_t* prule_slot, const void* prule_coeff, const void* prule_constant, int64_t n_prules,
-    int prule_m, void* stream);
+    int prule_m, const void* next_f, const void* next_record, const void* next_field, void* stream);
 
 /* Size in bytes of one lattice descriptor of the batched step (no counterpart in the
  * reference, which runs one lattice per solver). Host only. */

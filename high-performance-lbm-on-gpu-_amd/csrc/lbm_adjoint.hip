@@ -598,13 +598,12 @@ __host__ __device__ __forceinline__ size_t ibm_adj_layout(T* base, int d, int nn
 // nm 3^d lanes instead of 3^d entries in turn on nm lanes; every sum keeps its order.
 template <class T, int D>
 __device__ __forceinline__ void ibm_adj_reverse(const IbmAdjScratch<T> s, const IbmTables<T> tb, int nn, int nm,
-                                                int iters, const double* __restrict__ body_weight,
-                                                double* __restrict__ grad_w, int tid, int nthreads)
+                                                int iters, const T (&total_bar)[3], double* __restrict__ grad_w,
+                                                int tid, int nthreads)
 {
     constexpr int S = D == 2 ? 9 : 27;
     const int entries = nm * S;
     const int back = nthreads - 1 - tid;
-    const T total_bar[3] = {T(-body_weight[0]), T(-body_weight[1]), T(D == 3 ? -body_weight[2] : 0.0)};
     for (int it = iters - 1; it >= 0; --it) {
         const T* fo = s.force + (size_t)it * D * nn;
         const T* u = s.u + (size_t)it * D * nn;
@@ -868,6 +867,8 @@ __device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int
         te = tstart[n_tslots];
     }
     const double gw_old = tid < gw_count ? grad_w_g[tid] : 0.0;
+    // dL/d(body force) of the step, read with the first level (the sweep needs it first).
+    const T total_bar[3] = {T(-body_weight[0]), T(-body_weight[1]), T(D == 3 ? -body_weight[2] : 0.0)};
     for (int e = tid; e < gw_count; e += nthreads) gw[e] = 0.0;
     {
         const WordCopy tables[5] = {
@@ -976,7 +977,7 @@ __device__ __forceinline__ void ibm_adj_group_prefetched(LBM_IBM_ADJ_PARAMS, int
         __syncthreads();
         ibm_iterate<T, D>(s, tb, nn, nm, iters, true, hf, nullptr, tid, nthreads);
     }
-    ibm_adj_reverse<T, D>(s, tb, nn, nm, iters, body_weight, gw, tid, nthreads);
+    ibm_adj_reverse<T, D>(s, tb, nn, nm, iters, total_bar, gw, tid, nthreads);
 
     // ---- transposed moment rows (from LDS when staged), then every deferred store
     const T* m_bar = s.m_bar;
@@ -1102,7 +1103,8 @@ __device__ __forceinline__ void ibm_adj_group(LBM_IBM_ADJ_PARAMS, int tid, int n
         __syncthreads();
         ibm_iterate<T, D>(s, tb, nn, nm, iters, true, hf, nullptr, tid, nthreads);
     }
-    ibm_adj_reverse<T, D>(s, tb, nn, nm, iters, body_weight, grad_w, tid, nthreads);
+    const T total_bar[3] = {T(-body_weight[0]), T(-body_weight[1]), T(D == 3 ? -body_weight[2] : 0.0)};
+    ibm_adj_reverse<T, D>(s, tb, nn, nm, iters, total_bar, grad_w, tid, nthreads);
     scatter_rows(lam, tslot, tstart, trow, tcoeff, n_tslots, s.m_bar, tid, nthreads);
     if constexpr (STAGED) {
         for (int e = tid; e < nm * S; e += nthreads) grad_w_g[e] += grad_w[e];
@@ -1178,7 +1180,17 @@ __device__ __forceinline__ void adj_prefetch_next(const AdjNext<T>& next, const 
     }
     const size_t rec = ibm_record_elems(L::D, nn, nm, iters);
     T v = T(0);
-    for (size_t w = tid; w < rec; w += lanes) v = v + next.record[w];
+    // Eight loads in flight per lane and pass (clamped indices), then their sum.
+    for (size_t w0 = tid; w0 < rec; w0 += (size_t)8 * lanes) {
+        T a[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const size_t w = w0 + (size_t)k * lanes;
+            a[k] = next.record[w < rec ? w : rec - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v = v + a[k];
+    }
     for (int w = tid; w < nm * S; w += lanes) v = v + weights[w] + T(inverse[w] + entry[w]);
     for (int w = tid; w < nm; w += lanes) v = v + areas[w];
     for (int w = tid; w <= nn; w += lanes) sum += (uint32_t)start[w];

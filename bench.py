@@ -159,8 +159,12 @@ def batched_measure(torch, count: int = 64, steps: int = 100) -> dict:
     launches replayed from a captured graph and launched eagerly from a prepared list, the
     faster of the two counted."""
     from lbm import make_solver
+    from lbm.backends.hip_solver import GRAPH_PAIRS
     from lbm.batched import BatchedSolver
     from lbm.cases.batched_channels import batched_channels, initial_fields
+
+    # untimed warm-up that also captures both graph lengths the timed run replays
+    warm = 2 * (GRAPH_PAIRS + 1)
 
     t_build = time.perf_counter()
     params = batched_channels(count=count, seed=0)
@@ -168,7 +172,7 @@ def batched_measure(torch, count: int = 64, steps: int = 100) -> dict:
     batch = BatchedSolver(params, autotune=True)
     for i, p in enumerate(params):
         batch.initialize(i, *initial_fields(p))
-    batch.run(4)
+    batch.run(warm)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch.run(steps)
@@ -181,7 +185,7 @@ def batched_measure(torch, count: int = 64, steps: int = 100) -> dict:
         for graphs in (True, False):
             solver = make_solver(p, backend="hip-pull", graphs=graphs)
             solver.initialize(*initial_fields(p))
-            solver.run(4, check_every=4)
+            solver.run(warm, check_every=warm)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             solver._advance(steps)

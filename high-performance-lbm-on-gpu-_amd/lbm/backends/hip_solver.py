@@ -221,6 +221,11 @@ class ForcingStep:
 # 28.9 -> 25.7 K; the 880 x 164 channel 15.5 -> 6.9 us per step.
 RULES_IN_STEP_MAX_NODES = 2**22
 
+# Pairs of steps per replayed HIP graph: runs of GRAPH_PAIRS pairs replay one graph, the
+# remaining pairs a one-pair graph. Each replay costs the host a graph launch and leaves the
+# GPU idle for a few microseconds in between, so longer graphs keep short steps back to back.
+GRAPH_PAIRS = 8
+
 
 def default_rules_in_step(params) -> bool:
     return params.n_nodes <= RULES_IN_STEP_MAX_NODES
@@ -271,7 +276,9 @@ class HipSolver(Solver):
         groups_per_cu: Cap on resident workgroups per compute unit (0: none); default from
             ``default_groups_per_cu``.
         autotune: Time a few block shapes and caps on this grid and keep the fastest.
-        graphs: Capture two steps (pre-passes and kernels) into a HIP graph and replay it.
+        graphs: Capture steps (pre-passes and kernels) into HIP graphs and replay them:
+            runs of ``2 * GRAPH_PAIRS`` steps from one graph, the remaining pairs from a
+            two-step graph.
         rules_in_step: Pull variant with link rules: evaluate them inside the step
             (``lbm_pull_step_links``) rather than in a pre-pass launch (same bits); default
             from ``default_rules_in_step``. The per-step force of ``record_forces`` rides in
@@ -481,26 +488,29 @@ class HipSolver(Solver):
                              f"got {steps}")
         if self.graphs and steps >= 2:
             pairs, rest = divmod(steps, 2)
-            for _ in range(pairs):
-                self._advance_pair_graphed()
+            while pairs:
+                k = GRAPH_PAIRS if pairs >= GRAPH_PAIRS else 1
+                self._advance_pairs_graphed(k)
+                pairs -= k
             if rest:
                 self._advance_eager(rest)
             return
         self._advance_eager(steps)
 
-    def _advance_pair_graphed(self) -> None:
-        """Two steps from a captured HIP graph (buffers return to their places). The
-        two-lattice variants keep one graph per buffer holding the state, since an odd
-        number of steps run in between swaps the roles of the two buffers."""
+    def _advance_pairs_graphed(self, pairs: int) -> None:
+        """2 * ``pairs`` steps from a captured HIP graph (buffers return to their places);
+        the first call for a length launches the steps directly and captures them for the
+        next. The two-lattice variants keep the graphs per buffer holding the state, since
+        an odd number of steps run in between swaps the roles of the two buffers."""
         torch = self._torch
-        key = self._f.data_ptr()
+        key = (self._f.data_ptr(), pairs)
         graph = self._graphs.get(key)
         if graph is None:
-            self._advance_eager(2)
+            self._advance_eager(2 * pairs)
             torch.cuda.synchronize()
             graph = torch.cuda.CUDAGraph()
             with graph_capture(torch, graph):
-                self._advance_eager(2)
+                self._advance_eager(2 * pairs)
             self._graphs[key] = graph
         else:
             graph.replay()

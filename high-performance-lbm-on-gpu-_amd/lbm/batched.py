@@ -10,7 +10,9 @@ of the same velocity set and precision. All lattices share three device pools
    no rule reads a slot another rule writes; else ``lbm_rules_apply_widths``);
 2. one persistent launch of ``lbm_batched_pull_step`` over the (lattice, tile) work table.
 
-Two steps (the pool ping-pong) are captured in a HIP graph and replayed. Each lattice's
+Runs of steps (an even number, so the pools return to their places) are captured in HIP
+graphs and replayed: 2 * GRAPH_PAIRS steps per graph, the remaining pairs from a two-step
+graph. Each lattice's
 populations equal, bit for bit, what ``HipSolver(params, "pull")`` computes for it alone
 (``tests/test_batched.py``). Block shape and the number of persistent workgroups come from
 ``lbm.backends.autotune.tune_batched`` when ``autotune=True``.
@@ -24,7 +26,7 @@ import numpy as np
 
 from lbm.backends import _require_device, graph_capture, load_library
 from lbm.backends._native import DTYPE_CODES, pointer
-from lbm.backends.hip_solver import check_block, node_kinds
+from lbm.backends.hip_solver import GRAPH_PAIRS, check_block, node_kinds
 from lbm.link_rules import compile_rules, reads_other_targets
 
 DEFAULT_GROUPS_PER_CU = 8
@@ -111,7 +113,7 @@ class BatchedSolver:
         self._fb = torch.zeros_like(self._fa)
         self._build_rules()
         self._build_tiles()
-        self._graph = None
+        self._graphs = {}
         for i in range(n_l):
             self.initialize(i)
         if autotune:
@@ -168,7 +170,7 @@ class BatchedSolver:
         tiles = np.concatenate(parts).astype(np.int32)
         self._tiles = self._torch.as_tensor(tiles.ravel(), device=self.device)
         self._n_tiles = len(tiles)
-        self._graph = None
+        self._graphs = {}
 
     # ------------------------------------------------------------------ state
 
@@ -215,26 +217,36 @@ class BatchedSolver:
         self._step(self._fb, self._fa)
 
     def run(self, steps: int) -> None:
-        """Advance every lattice by ``steps`` steps (pairs replayed from one captured graph)."""
+        """Advance every lattice by ``steps`` steps: runs of GRAPH_PAIRS pairs replayed from one
+        captured graph, the remaining pairs from a one-pair graph (the first run of each
+        length is launched directly and captured for the next), an odd last step launched."""
         torch = self._torch
         pairs, odd = divmod(int(steps), 2)
-        for _ in range(pairs):
+        while pairs:
+            k = GRAPH_PAIRS if pairs >= GRAPH_PAIRS else 1
+            pairs -= k
             if not self.graphs:
-                self._pair()
+                for _ in range(k):
+                    self._pair()
                 continue
-            if self._graph is None:
-                self._pair()
+            # the launches carry the grid size and the occupancy cap
+            key = (k, self.groups, self.groups_per_cu)
+            graph = self._graphs.get(key)
+            if graph is None:
+                for _ in range(k):
+                    self._pair()
                 torch.cuda.synchronize()
                 graph = torch.cuda.CUDAGraph()
                 with graph_capture(torch, graph):
-                    self._pair()
-                self._graph = graph
+                    for _ in range(k):
+                        self._pair()
+                self._graphs[key] = graph
             else:
-                self._graph.replay()
+                graph.replay()
         if odd:
             self._step(self._fa, self._fb)
             self._fa, self._fb = self._fb, self._fa
-            self._graph = None
+            self._graphs = {}
         self.time_step += int(steps)
 
     def synchronize(self) -> None:

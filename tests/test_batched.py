@@ -68,6 +68,53 @@ def test_graphed_single_solver_equals_eager():
     np.testing.assert_array_equal(alone(p, 13, graphs=True), alone(p, 13, graphs=False))
 
 
+# runs that capture, then replay, the long and the one-pair graphs, with odd runs between
+# them (which swap the buffers) and direct work between the replays
+def _graph_runs():
+    from lbm.backends.hip_solver import GRAPH_PAIRS
+
+    long = 2 * GRAPH_PAIRS
+    return [long + 2, 2 * long + 5, 4, long + 1, 2 * long + 2]
+
+
+@pytest.mark.parametrize("variant", ["pull", "aa"])
+def test_long_graph_replays_of_one_solver_equal_direct_launches(variant):
+    p = small_batch()[0]
+    runs = _graph_runs()
+    if variant == "aa":
+        runs = [r + r % 2 for r in runs]
+    out = []
+    for graphs in (True, False):
+        solver = make_solver(p, backend=f"hip-{variant}", graphs=graphs)
+        solver.initialize(*initial_fields(p))
+        states = []
+        for steps in runs:
+            solver.run(steps)
+            states.append(solver.populations())
+        out.append(states)
+    for replayed, direct in zip(*out):
+        np.testing.assert_array_equal(replayed, direct)
+
+
+def test_long_graph_replays_of_a_batch_equal_direct_launches():
+    from lbm.batched import BatchedSolver
+
+    params = small_batch()
+    out = []
+    for graphs in (True, False):
+        batch = BatchedSolver(params, graphs=graphs)
+        for i, p in enumerate(params):
+            batch.initialize(i, *initial_fields(p))
+        states = []
+        for steps in _graph_runs():
+            batch.run(steps)
+            states.append([batch.populations(i) for i in range(len(params))])
+        out.append(states)
+    for replayed, direct in zip(*out):
+        for a, b in zip(replayed, direct):
+            np.testing.assert_array_equal(a, b)
+
+
 def test_autotuned_solver_keeps_its_state():
     from tests.helpers import mixed_params, random_state
 

@@ -1154,9 +1154,10 @@ struct AdjNext {
 
 // Reads, once its own nodes are done, what the next step's IBM workgroup will read first
 // (the region list; kind, body force, field and pulled populations of the region nodes; the
-// record; the stencil tables), so that workgroup finds them in the L2 cache of its XCD
-// (the host picks a block on the same XCD) instead of HBM. The values are summed and held,
-// so the loads are issued; nothing is written.
+// record; the stencil tables), so that workgroup finds them in a cache (the L2 of its XCD,
+// the host picking a block on the same XCD, or the Infinity Cache all XCDs share) instead of
+// HBM. None of it is written in this launch. The values are summed and held, so the loads
+// are issued; nothing is written.
 template <class L, class T>
 __device__ __forceinline__ void adj_prefetch_next(const AdjNext<T>& next, const int32_t* __restrict__ region,
                                                   int n_region, const uint8_t* __restrict__ kind,

@@ -96,16 +96,32 @@ def default_block(params, variant: str) -> tuple[int, int]:
     kernel keeps bx <= 256: its halo ring costs 2 (bx + 2) + 2 by cells per bx * by, so
     256 x 4 and 128 x 8 win (D2Q9 fp32 4096^2: 76 K MLUPS against 62 K for 64 x 4 and 55 K
     for 32 x 8). D3Q19: bx up to 256 with the measured lanes per workgroup of
-    ``_SHAPE_3D`` (``_SHAPE_3D_MASKED`` for lattices with solid or forced nodes)."""
+    ``_SHAPE_3D`` (``_SHAPE_3D_MASKED`` for lattices with solid or forced nodes).
+
+    A row longer than the widest tile is split into equal tiles (``_even_tiles``) for
+    all-fluid lattices and for the AA pattern in 3D: at 1536^2 D2Q9 two tiles of 768 ran
+    13-14 % faster than 1024 + 512, at 384^3 D3Q19 two of 192 10 % faster than 256 + 128,
+    at 320^3 3-5 %, and the 384 x 96 x 96 duct 13 % (AA). Masked lattices of the pull
+    variant keep power-of-two workgroups, which the force layer of the pull step with its
+    link rules needs (``record_forces``)."""
     if params.d == 2:
         masked = _masked(params) and variant != "lds"
         small = params.nx * params.ny <= _D2Q9_SMALL_MASKED_NODES
         lanes = _D2Q9_MASKED_LANES if (masked and not small) else _D2Q9_LANES
         row = 256 if variant == "lds" else (_D2Q9_MASKED_LANES if masked else lanes)
-        bx = min(row, 64 * math.ceil(params.nx / 64))
+        bx = min(row, 64 * math.ceil(params.nx / 64)) if _masked(params) else _even_tiles(params.nx, row)
         return bx, max(1, min(params.ny, lanes // bx))
-    bx = min(256, 64 * math.ceil(params.nx / 64))
+    balanced = variant == "aa" or not _masked(params)
+    bx = _even_tiles(params.nx, 256) if balanced else min(256, 64 * math.ceil(params.nx / 64))
     return bx, max(1, min(params.ny, _shape_3d(params, variant)[0] // bx))
+
+
+def _even_tiles(nx: int, widest: int) -> int:
+    """Tile width for rows of nx nodes: the fewest tiles of at most ``widest`` lanes, each a
+    whole number of 64-lane wavefronts and as equal as that allows, so the last tile of a
+    row is not left mostly idle (nx = 384, widest 256: 192 + 192 rather than 256 + 128)."""
+    tiles = math.ceil(nx / widest)
+    return 64 * math.ceil(nx / (64 * tiles))
 
 
 INFINITY_CACHE_BYTES = 256 * 2**20

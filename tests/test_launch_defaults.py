@@ -67,3 +67,27 @@ def test_default_blocks_respect_the_kernel_lane_limits(n, dtype):
             bx, by = default_block(p, variant)
             limit = 512 if (d == 3 and dtype == "float64") else 1024
             assert bx % 64 == 0 and 1 <= bx * by <= limit
+
+
+@pytest.mark.parametrize(("nx", "widest", "width"), [
+    (384, 256, 192), (320, 256, 192), (256, 256, 256), (512, 256, 256), (640, 256, 256),
+    (100, 1024, 128), (1536, 1024, 768), (5120, 1024, 1024), (1100, 1024, 576),
+])
+def test_rows_split_into_even_tiles(nx, widest, width):
+    from lbm.backends.hip_solver import _even_tiles
+
+    bx = _even_tiles(nx, widest)
+    assert bx == width and bx % 64 == 0 and bx <= widest
+    tiles = -(-nx // bx)
+    assert tiles == -(-nx // widest)  # no more tiles than the widest width needs
+    assert tiles * bx - nx < 64 * tiles  # less than a wavefront idle per tile
+
+
+def test_even_tiles_for_all_fluid_lattices_and_the_3d_aa_pattern_only():
+    p = box(384, 3)
+    assert default_block(p, "aa") == (192, 2) and default_block(p, "pull") == (192, 1)
+    walls = box(384, 3, solid=True)
+    assert default_block(walls, "aa") == (192, 1)
+    assert default_block(walls, "pull") == (256, 1)  # power-of-two groups for the force layer
+    assert default_block(box(1536, 2), "pull") == (768, 1)
+    assert default_block(box(1536, 2, solid=True), "pull") == (512, 1)

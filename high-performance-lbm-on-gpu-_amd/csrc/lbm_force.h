@@ -99,10 +99,12 @@ __device__ __forceinline__ void force_tree(const double (&acc)[3], double (&part
     force_waves_total(part);
 }
 
-// One group of the reduction hosted by a workgroup of `lanes` threads (a multiple of 64
-// that divides FORCE_THREADS; tid is the linear thread index): thread tid plays lanes tid,
-// tid + lanes, ... of the group in turn, each a whole wavefront of consecutive lanes, so
-// every lane sum and every tree node is the one a FORCE_THREADS-lane workgroup computes.
+// One group of the reduction hosted by a workgroup of `lanes` threads (a multiple of 64; tid
+// is the linear thread index): thread tid plays lanes tid, tid + lanes, ... (below
+// FORCE_THREADS) of the group in turn. Each virtual lane is played by exactly one thread, a
+// wavefront always plays 64 consecutive virtual lanes (lanes and FORCE_THREADS are multiples
+// of 64, so its trip count is uniform), and every lane sum and every tree node is the one a
+// FORCE_THREADS-lane workgroup computes.
 // Writes out[a] (a < d) for a single group, else partial[3 grp + a]. Call from all threads
 // of the workgroup; ends with a barrier, so `part` may be reused at once.
 template <class T>

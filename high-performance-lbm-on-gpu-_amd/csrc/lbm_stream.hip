@@ -484,8 +484,7 @@ int run_pull_links(int lattice, int dtype, const void* src, void* dst, const uin
         if (groups) {
             require(force_slot && force_coeff && force_constant && force_weight && force_m >= 1 && force_m <= 128,
                     "force rows need slot, coeff, constant, weight and 1 <= m <= 128");
-            require(lanes % 64 == 0 && FORCE_THREADS % lanes == 0,
-                    "the force layer needs 64, 128, 256, 512 or 1024 threads per workgroup");
+            require(lanes % 64 == 0, "the force layer needs a whole number of wavefronts per workgroup");
             require(groups == 1 ? force_out != nullptr : force_partial != nullptr,
                     "force rows need out (up to 2048 rows) or partial (above)");
         }

@@ -98,21 +98,18 @@ def default_block(params, variant: str) -> tuple[int, int]:
     for 32 x 8). D3Q19: bx up to 256 with the measured lanes per workgroup of
     ``_SHAPE_3D`` (``_SHAPE_3D_MASKED`` for lattices with solid or forced nodes).
 
-    A row longer than the widest tile is split into equal tiles (``_even_tiles``) for
-    all-fluid lattices and for the AA pattern in 3D: at 1536^2 D2Q9 two tiles of 768 ran
-    13-14 % faster than 1024 + 512, at 384^3 D3Q19 two of 192 10 % faster than 256 + 128,
-    at 320^3 3-5 %, and the 384 x 96 x 96 duct 13 % (AA). Masked lattices of the pull
-    variant keep power-of-two workgroups, which the force layer of the pull step with its
-    link rules needs (``record_forces``)."""
+    A row longer than the widest tile is split into equal tiles (``_even_tiles``): at 1536^2
+    D2Q9 two tiles of 768 ran 13-14 % faster than 1024 + 512, at 384^3 D3Q19 two of 192
+    10 % faster than 256 + 128, at 320^3 3-5 %, and the 384 x 96 x 96 duct 6 % (pull) and
+    13 % (AA)."""
     if params.d == 2:
         masked = _masked(params) and variant != "lds"
         small = params.nx * params.ny <= _D2Q9_SMALL_MASKED_NODES
         lanes = _D2Q9_MASKED_LANES if (masked and not small) else _D2Q9_LANES
         row = 256 if variant == "lds" else (_D2Q9_MASKED_LANES if masked else lanes)
-        bx = min(row, 64 * math.ceil(params.nx / 64)) if _masked(params) else _even_tiles(params.nx, row)
+        bx = _even_tiles(params.nx, row)
         return bx, max(1, min(params.ny, lanes // bx))
-    balanced = variant == "aa" or not _masked(params)
-    bx = _even_tiles(params.nx, 256) if balanced else min(256, 64 * math.ceil(params.nx / 64))
+    bx = _even_tiles(params.nx, 256)
     return bx, max(1, min(params.ny, _shape_3d(params, variant)[0] // bx))
 
 
@@ -800,7 +797,7 @@ class HipSolver(Solver):
         stream = self._stream
         lanes = self.block[0] * self.block[1]
         rules_ride = self._links is not None or "pull" not in self._rule_passes
-        if self.variant == "pull" and self._mask_mode and rules_ride and lanes % 64 == 0 and 1024 % lanes == 0:
+        if self.variant == "pull" and self._mask_mode and rules_ride and lanes % 64 == 0 and lanes <= 1024:
             # (with a rule pre-pass the force must be read before the pre-pass writes the
             # slots it reads, so that case keeps the launches below)
             return self._record_forces_in_step(pack, steps, out)

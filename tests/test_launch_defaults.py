@@ -52,7 +52,7 @@ def test_d2q9_rows_and_the_cap_rule():
     small = box(512, 2, solid=True)  # masked up to 2^18 nodes: rows of 512 lanes, two per group
     assert default_block(small, "pull") == default_block(small, "aa") == (512, 2)
     channel = SimulationParams(nx=880, ny=164, tau=0.6, solid=np.zeros((164, 880), dtype=bool) | (np.arange(164) == 0)[:, None])
-    assert default_block(channel, "pull") == (512, 2)
+    assert default_block(channel, "pull") == (448, 2)  # two even 448-lane tiles per row
     assert default_block(box(1024, 2, solid=True), "aa") == (512, 1)
     narrow = SimulationParams(nx=45, ny=29, tau=0.6)
     assert default_block(narrow, "pull") == (64, 16)
@@ -83,11 +83,11 @@ def test_rows_split_into_even_tiles(nx, widest, width):
     assert tiles * bx - nx < 64 * tiles  # less than a wavefront idle per tile
 
 
-def test_even_tiles_for_all_fluid_lattices_and_the_3d_aa_pattern_only():
+def test_long_rows_take_even_tiles():
     p = box(384, 3)
     assert default_block(p, "aa") == (192, 2) and default_block(p, "pull") == (192, 1)
     walls = box(384, 3, solid=True)
-    assert default_block(walls, "aa") == (192, 1)
-    assert default_block(walls, "pull") == (256, 1)  # power-of-two groups for the force layer
+    assert default_block(walls, "aa") == default_block(walls, "pull") == (192, 1)
     assert default_block(box(1536, 2), "pull") == (768, 1)
-    assert default_block(box(1536, 2, solid=True), "pull") == (512, 1)
+    assert default_block(box(1536, 2, solid=True), "pull") == (512, 1)  # three 512-lane rows
+    assert default_block(box(1200, 2, solid=True), "pull") == (448, 1)

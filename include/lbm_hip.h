@@ -93,7 +93,7 @@ LBM_API int lbm_pull_step(int lattice, int dtype, const void* f_src, void* f_dst
  * result equals lbm_rules_apply + lbm_pull_step bit for bit, without the pre-pass launch.
  * n_rules = 0 (rule_nodes = 0): no rules (the rule arrays may be NULL).
  * Force: when n_force > 0, the extra layer also reduces the force rows on f_src exactly as
- * lbm_affine_force does (same groups, same sums; bx * by must be 64, 128, 256, 512 or 1024)
+ * lbm_affine_force does (same groups, same sums; bx * by must be a multiple of 64)
  * into force_out[d] (up to 2048 rows) or force_partial (3 doubles per group of
  * lbm_affine_force_groups; lbm_force_totals adds them).
  * Replaces _RulePass.__call__ followed by stream_collide_naive
